@@ -1,0 +1,191 @@
+/*
+ * mps_amd.h — C ABI of the MI355X MPS time-step library (libmps_amd.so).
+ *
+ * The library owns all device memory, one HIP stream per handle and the
+ * captured graphs; the caller owns every host buffer and the library copies in
+ * and out.  Every entry point returns 0 on success and a non-zero status
+ * (MPS_E_*) on failure; mps_last_error() then returns a message.  No C++
+ * exception crosses this boundary.  One handle is driven by one host thread.
+ *
+ * Particle arrays are indexed by particle id and ids must be 0..N-1 in order,
+ * exactly as the reference requires (src/mps.cpp:192 indexes particles[id],
+ * src/mps.cpp:290 indexes sourceTerm[pi.id]).  Vectors are 3 doubles per
+ * particle (x,y,z), as the reference's Eigen::Vector3d (src/particle.hpp:27).
+ *
+ * Enumerations keep the reference's integer values (src/particle.hpp:5,19):
+ *   type: Ghost=-1, Fluid=0, Wall=1, DummyWall=2
+ *   boundary condition: Ignored=0, Surface=1, Inner=2
+ */
+#ifndef MPS_AMD_H
+#define MPS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPS_OK 0
+#define MPS_E_INVALID 1   /* bad argument or input state */
+#define MPS_E_HIP 2       /* HIP runtime error */
+#define MPS_E_NOMEM 3     /* device allocation failed */
+#define MPS_E_RANGE 4     /* a particle fell outside the bucket grid */
+#define MPS_E_SOLVER 5    /* PPE solve did not reach its tolerance */
+#define MPS_E_ASYM 6      /* PPE matrix not symmetric (src/mps.cpp:418-443) */
+
+#define MPS_GHOST (-1)
+#define MPS_FLUID 0
+#define MPS_WALL 1
+#define MPS_DUMMY_WALL 2
+
+#define MPS_BC_IGNORED 0
+#define MPS_BC_SURFACE 1
+#define MPS_BC_INNER 2
+
+/* POD mirror of Settings (src/settings.hpp:35-86) after Settings::load
+ * (src/settings.cpp:21-84): radii and the collision distance are already
+ * multiplied by the particle distance, re_max = max(re_pressure, re_viscosity,
+ * re_surface) (src/settings.cpp:56-61). */
+typedef struct mps_settings {
+    int32_t dim;               /* 2 or 3 */
+    int32_t qc_on;             /* pressure stabilization: quasi compressibility on */
+    int32_t hos_on;            /* pressure stabilization: higher order source term on */
+    int32_t reserved0;
+    double particle_distance;  /* l0 */
+    double dt;
+    double finish_time;
+    double output_interval;
+    double cfl_condition;
+    double domain_min[3];      /* domain range X/Y/Z [min, max] */
+    double domain_max[3];
+    double density;
+    double kinematic_viscosity;
+    double gravity[3];
+    double re_pressure;
+    double re_viscosity;
+    double re_surface;
+    double re_max;
+    double surface_threshold;  /* threshold for surface detection (beta) */
+    double compressibility;    /* quasi compressibility alpha */
+    double hos_gamma;
+    double relaxation;         /* relaxation coefficient for pressure */
+    double collision_distance; /* collision distance ratio * l0 */
+    double restitution;        /* coefficient of restitution */
+} mps_settings;
+
+/* Pressure-solver controls.  No counterpart in the reference, whose solver is
+ * Eigen's ConjugateGradient with fixed defaults (src/mps.cpp:452-460). */
+#define MPS_PREC_FP64 0        /* FP64 Jacobi-PCG */
+#define MPS_PREC_MIXED 1       /* FP32 inner Jacobi-PCG + FP64 iterative refinement */
+typedef struct mps_solver_options {
+    int32_t precision;         /* MPS_PREC_* */
+    int32_t warm_start;        /* 1: start from the previous step's unclamped solution */
+    int32_t max_iterations;    /* cap on total CG iterations per solve; <=0: 2*rows */
+    int32_t max_refinements;   /* mixed: cap on outer refinement steps; <=0: 20 */
+    double tolerance;          /* stop when ||b-Ax||_2 <= tolerance*||b||_2 */
+    double inner_tolerance;    /* mixed: relative reduction asked of each inner solve */
+    int32_t check_interval;    /* CG iterations between host convergence checks (>=1) */
+    int32_t use_graph;         /* 1: replay captured HIP graphs for the CG chunks */
+} mps_solver_options;
+
+/* Statistics of the last pressure solve. */
+typedef struct mps_solver_stats {
+    int32_t rows;              /* Inner rows in the system */
+    int32_t iterations;        /* CG iterations (mixed: inner iterations summed) */
+    int32_t refinements;       /* mixed: outer refinement steps (FP64 residuals) */
+    int32_t converged;         /* 1 if the true relative residual met the tolerance */
+    double rhs_norm;           /* ||b||_2 */
+    double rel_residual;       /* ||b-Ax||_2 / ||b||_2, true FP64 residual */
+} mps_solver_stats;
+
+typedef struct mps_handle mps_handle;
+
+/* Human-readable message of the last failure on h (or of the last failed
+ * mps_create when h is NULL). Never NULL. */
+const char* mps_last_error(const mps_handle* h);
+
+/* Replaces MPS::MPS(path) (src/mps.cpp:18-36) plus the particle import of
+ * MPS::loadInitialState (src/mps.cpp:96-102): takes settings and the initial
+ * particle table, computes the reference values n0/lambda (src/mps.cpp:132-163),
+ * sizes the bucket grid (src/bucket.cpp:10-22) and uploads the state.
+ * pos/vel are N x 3 doubles; type holds MPS_* type codes. Pressure starts at 0
+ * and the boundary condition at Ignored (the reference leaves them
+ * uninitialised, src/particle.hpp:28,34). */
+int mps_create(const mps_settings* s, int32_t n, const int32_t* id,
+               const int8_t* type, const double* pos, const double* vel,
+               mps_handle** out);
+int mps_destroy(mps_handle* h);
+
+int mps_set_solver(mps_handle* h, const mps_solver_options* o);
+int mps_get_solver_stats(const mps_handle* h, mps_solver_stats* out);
+
+/* Replaces MPS::loadInitialState (src/mps.cpp:96-109): the initial
+ * setNumberDensity (src/mps.cpp:627-641). */
+int mps_load_initial_state(mps_handle* h);
+
+/* Replaces MPS::stepForward (src/mps.cpp:111-130). Writes the Courant number
+ * (src/mps.cpp:547-571) to *courant when courant is not NULL. */
+int mps_step(mps_handle* h, int32_t is_export, double* courant);
+
+/* Runs n_steps steps with no export and no host synchronisation between
+ * them; writes each step's Courant number to courants[n_steps] if not NULL. */
+int mps_run_steps(mps_handle* h, int32_t n_steps, double* courants);
+
+/* Copies the particle state out in id order (the fields Exporter reads,
+ * src/exporter.cpp:35-51,96-136). Any pointer may be NULL. */
+int mps_download(const mps_handle* h, int8_t* type, double* pos, double* vel,
+                 double* pressure, double* ndr, int8_t* bc);
+
+/* Minimum pressure per particle (src/mps.cpp:481-503), id order. */
+int mps_download_min_pressure(const mps_handle* h, double* pmin);
+
+/* Overwrites the particle state (id order); used for restarts from a
+ * lossless checkpoint. Any pointer may be NULL (field unchanged). */
+int mps_upload(mps_handle* h, const int8_t* type, const double* pos,
+               const double* vel, const double* pressure, const double* ndr,
+               const int8_t* bc);
+
+/* Debug-level events the reference logs with spdlog::debug and counts in
+ * debugLogCount (src/mps.cpp:405-410, 559-564, 655-663), accumulated since
+ * mps_create or the last mps_clear_debug_events:
+ *   counts[0] = particles ghosted (out of domain),
+ *   counts[1] = Inner particles with no Dirichlet connection,
+ *   counts[2] = steps whose Courant number exceeded the CFL condition.
+ * ghost_ids / unconnected_ids receive up to cap ids each, ascending. */
+int mps_debug_events(const mps_handle* h, int64_t counts[3],
+                     int32_t* ghost_ids, int32_t* unconnected_ids, int32_t cap);
+int mps_clear_debug_events(mps_handle* h);
+
+/* ---- test and inspection entry points ---- */
+
+/* Bucket grid geometry (src/bucket.hpp:11-14). */
+int mps_bucket_shape(const mps_handle* h, int32_t dims[4] /* num,numX,numY,numZ */,
+                     double* length);
+/* Rebuilds the bucket grid from the current positions
+ * (Bucket::storeParticles, src/bucket.cpp:24-51) and writes the reference's
+ * linked-list view of it: first/last per cell (num entries), next per
+ * particle (N entries), -1 for none. */
+int mps_dump_bucket(mps_handle* h, int32_t* first, int32_t* last, int32_t* next);
+/* Neighbour lists as MPS::setNeighbors (src/mps.cpp:585-619) builds them from
+ * the current positions, in the reference's order. offsets has N+1 entries;
+ * call with ids == NULL to get offsets only. Ghost particles get an empty
+ * list here (the reference keeps a stale one). */
+int mps_dump_neighbors(mps_handle* h, int32_t* offsets, int32_t* ids,
+                       double* dists);
+/* Pressure system of the last step in id order: source term b (N), matrix
+ * diagonal after the Dirichlet-connection doubling (N, 0 for non-Inner rows),
+ * and the off-diagonal entries in CSR by id (row_ptr N+1, col, val: rows
+ * sorted by column). Call with col == NULL to get row_ptr only. Also the
+ * solution before clamping (N). */
+int mps_get_pressure_system(const mps_handle* h, double* b, double* diag,
+                            int32_t* row_ptr, int32_t* col, double* val,
+                            double* x);
+/* Positions and velocities as they were just before the pressure stage of the
+ * last step (after collision), id order; needs the snapshot flag. */
+int mps_set_snapshot(mps_handle* h, int32_t on);
+int mps_get_snapshot(const mps_handle* h, double* pos, double* vel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPS_AMD_H */

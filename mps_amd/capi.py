@@ -1,0 +1,246 @@
+"""ctypes binding of libmps_amd.so (include/mps_amd.h).
+
+The library is the product; this module only marshals numpy arrays across
+the C ABI.  Loading fails loudly if the shared library has not been built:
+there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .configs import MpsSettings, Scene
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmps_amd.so")
+
+MPS_PREC_FP64 = 0
+MPS_PREC_MIXED = 1
+
+_ERRORS = {1: "MPS_E_INVALID", 2: "MPS_E_HIP", 3: "MPS_E_NOMEM", 4: "MPS_E_RANGE", 5: "MPS_E_SOLVER",
+           6: "MPS_E_ASYM"}
+
+EXPORTED = [
+    "mps_last_error", "mps_create", "mps_destroy", "mps_set_solver", "mps_get_solver_stats",
+    "mps_load_initial_state", "mps_step", "mps_run_steps", "mps_download", "mps_download_min_pressure",
+    "mps_upload", "mps_debug_events", "mps_clear_debug_events", "mps_bucket_shape", "mps_dump_bucket",
+    "mps_dump_neighbors", "mps_get_pressure_system", "mps_set_snapshot", "mps_get_snapshot",
+]
+
+
+class MpsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class SolverOptions(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int32),
+        ("warm_start", ctypes.c_int32),
+        ("max_iterations", ctypes.c_int32),
+        ("max_refinements", ctypes.c_int32),
+        ("tolerance", ctypes.c_double),
+        ("inner_tolerance", ctypes.c_double),
+        ("check_interval", ctypes.c_int32),
+        ("use_graph", ctypes.c_int32),
+    ]
+
+
+class SolverStats(ctypes.Structure):
+    _fields_ = [
+        ("rows", ctypes.c_int32),
+        ("iterations", ctypes.c_int32),
+        ("refinements", ctypes.c_int32),
+        ("converged", ctypes.c_int32),
+        ("rhs_norm", ctypes.c_double),
+        ("rel_residual", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Loads libmps_amd.so; raises if it is missing (build it first)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run `python -m mps_amd.build` (no fallback path exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    i32 = ctypes.c_int32
+    L.mps_last_error.restype = ctypes.c_char_p
+    L.mps_last_error.argtypes = [vp]
+    L.mps_create.restype = i32
+    L.mps_create.argtypes = [ctypes.POINTER(MpsSettings), i32, vp, vp, vp, vp, ctypes.POINTER(vp)]
+    L.mps_destroy.argtypes = [vp]
+    L.mps_set_solver.argtypes = [vp, ctypes.POINTER(SolverOptions)]
+    L.mps_get_solver_stats.argtypes = [vp, ctypes.POINTER(SolverStats)]
+    L.mps_load_initial_state.argtypes = [vp]
+    L.mps_step.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_double)]
+    L.mps_run_steps.argtypes = [vp, i32, vp]
+    L.mps_download.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.mps_download_min_pressure.argtypes = [vp, vp]
+    L.mps_upload.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.mps_debug_events.argtypes = [vp, vp, vp, vp, i32]
+    L.mps_clear_debug_events.argtypes = [vp]
+    L.mps_bucket_shape.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_double)]
+    L.mps_dump_bucket.argtypes = [vp, vp, vp, vp]
+    L.mps_dump_neighbors.argtypes = [vp, vp, vp, vp]
+    L.mps_get_pressure_system.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.mps_set_snapshot.argtypes = [vp, i32]
+    L.mps_get_snapshot.argtypes = [vp, vp, vp]
+    for name in EXPORTED:
+        if name not in ("mps_last_error",):
+            getattr(L, name).restype = i32
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Simulation:
+    """One particle system on the GPU (the C ABI handle)."""
+
+    def __init__(self, settings, scene: Scene):
+        self.L = lib()
+        self.n = scene.n
+        self._cs = settings.to_c() if hasattr(settings, "to_c") else settings
+        ids = np.arange(self.n, dtype=np.int32)
+        t = np.ascontiguousarray(scene.type, np.int8)
+        pos = np.ascontiguousarray(scene.pos, np.float64)
+        vel = np.ascontiguousarray(scene.vel, np.float64)
+        h = ctypes.c_void_p()
+        rc = self.L.mps_create(ctypes.byref(self._cs), self.n, _p(ids), _p(t), _p(pos), _p(vel), ctypes.byref(h))
+        if rc:
+            raise MpsError(rc, self.L.mps_last_error(None).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc:
+            raise MpsError(rc, self.L.mps_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mps_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_solver(self, precision=MPS_PREC_FP64, tolerance=None, warm_start=False, max_iterations=0,
+                   max_refinements=0, inner_tolerance=1e-4, check_interval=32, use_graph=True):
+        o = SolverOptions()
+        o.precision = precision
+        o.warm_start = int(bool(warm_start))
+        o.max_iterations = max_iterations
+        o.max_refinements = max_refinements
+        o.tolerance = float(np.finfo(np.float64).eps) if tolerance is None else float(tolerance)
+        o.inner_tolerance = inner_tolerance
+        o.check_interval = check_interval
+        o.use_graph = int(bool(use_graph))
+        self._check(self.L.mps_set_solver(self.h, ctypes.byref(o)))
+
+    def solver_stats(self):
+        s = SolverStats()
+        self._check(self.L.mps_get_solver_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def load_initial_state(self):
+        self._check(self.L.mps_load_initial_state(self.h))
+
+    def step(self, is_export=False):
+        c = ctypes.c_double()
+        self._check(self.L.mps_step(self.h, int(bool(is_export)), ctypes.byref(c)))
+        return c.value
+
+    def run_steps(self, k):
+        out = np.zeros(max(k, 1))
+        self._check(self.L.mps_run_steps(self.h, k, _p(out)))
+        return out[:k]
+
+    def state(self):
+        n = self.n
+        t = np.zeros(n, np.int8)
+        pos = np.zeros((n, 3))
+        vel = np.zeros((n, 3))
+        p = np.zeros(n)
+        ndr = np.zeros(n)
+        bc = np.zeros(n, np.int8)
+        self._check(self.L.mps_download(self.h, _p(t), _p(pos), _p(vel), _p(p), _p(ndr), _p(bc)))
+        pmin = np.zeros(n)
+        self._check(self.L.mps_download_min_pressure(self.h, _p(pmin)))
+        return dict(type=t, pos=pos, vel=vel, p=p, ndr=ndr, bc=bc, pmin=pmin)
+
+    def upload(self, type=None, pos=None, vel=None, p=None, ndr=None, bc=None):
+        def c(a, dt, shape=None):
+            if a is None:
+                return None
+            return np.ascontiguousarray(a, dt)
+        arrs = [c(type, np.int8), c(pos, np.float64), c(vel, np.float64), c(p, np.float64), c(ndr, np.float64),
+                c(bc, np.int8)]
+        self._check(self.L.mps_upload(self.h, *[_p(a) for a in arrs]))
+
+    def debug_events(self):
+        counts = np.zeros(3, np.int64)
+        cap = self.n + 16
+        g = np.zeros(cap, np.int32)
+        u = np.zeros(cap, np.int32)
+        self._check(self.L.mps_debug_events(self.h, _p(counts), _p(g), _p(u), cap))
+        return counts, g[:min(int(counts[0]), cap)], u[:min(int(counts[1]), cap)]
+
+    def clear_debug_events(self):
+        self._check(self.L.mps_clear_debug_events(self.h))
+
+    def bucket(self):
+        dims = np.zeros(4, np.int32)
+        ln = ctypes.c_double()
+        self._check(self.L.mps_bucket_shape(self.h, _p(dims), ctypes.byref(ln)))
+        first = np.zeros(dims[0], np.int32)
+        last = np.zeros(dims[0], np.int32)
+        nxt = np.zeros(max(self.n, 1), np.int32)
+        self._check(self.L.mps_dump_bucket(self.h, _p(first), _p(last), _p(nxt)))
+        return dict(dims=dims, length=ln.value, first=first, last=last, next=nxt[:self.n])
+
+    def neighbors(self):
+        off = np.zeros(self.n + 1, np.int32)
+        self._check(self.L.mps_dump_neighbors(self.h, _p(off), None, None))
+        tot = int(off[-1])
+        ids = np.zeros(max(tot, 1), np.int32)
+        d = np.zeros(max(tot, 1))
+        self._check(self.L.mps_dump_neighbors(self.h, _p(off), _p(ids), _p(d)))
+        return off, ids[:tot], d[:tot]
+
+    def set_snapshot(self, on=True):
+        self._check(self.L.mps_set_snapshot(self.h, int(bool(on))))
+
+    def snapshot(self):
+        pos = np.zeros((self.n, 3))
+        vel = np.zeros((self.n, 3))
+        self._check(self.L.mps_get_snapshot(self.h, _p(pos), _p(vel)))
+        return pos, vel
+
+    def pressure_system(self):
+        n = self.n
+        rp = np.zeros(n + 1, np.int32)
+        self._check(self.L.mps_get_pressure_system(self.h, None, None, _p(rp), None, None, None))
+        nnz = int(rp[-1])
+        b = np.zeros(n)
+        diag = np.zeros(n)
+        col = np.zeros(max(nnz, 1), np.int32)
+        val = np.zeros(max(nnz, 1))
+        x = np.zeros(n)
+        self._check(self.L.mps_get_pressure_system(self.h, _p(b), _p(diag), _p(rp), _p(col), _p(val), _p(x)))
+        return dict(b=b, diag=diag, row_ptr=rp, col=col[:nnz], val=val[:nnz], x=x)

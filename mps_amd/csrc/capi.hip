@@ -1,0 +1,876 @@
+// capi.hip — the C ABI of libmps_amd.so (include/mps_amd.h): handle, device
+// allocation, input validation and the orchestration of one MPS step
+// (MPS::stepForward, src/mps.cpp:111-130) on one HIP stream.
+// Contraction off: distances and weights must round as the reference's
+// scalar FP64 code does (no fused multiply-add).
+#pragma clang fp contract(off)
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "mps_internal.hpp"
+
+using namespace mps;
+
+struct mps_handle {
+    std::string err;
+    mps_settings s{};
+    Consts c{};
+    GridView g{};
+    int n = 0;
+    int dim = 2;
+    int iz0 = 1;  // 2D: the common z layer of the reference's key
+    hipStream_t st = nullptr;
+    // hot sets
+    double* P[2][6] = {};  // px py pz vx vy vz
+    int32_t* ID[2] = {};
+    int8_t* TY[2] = {};
+    int32_t* KEY[2] = {};
+    int pv = 0, it = 0;
+    BuildBufs bb{};
+    Cold cold{};
+    StepScratch w{};
+    double* x_s = nullptr;
+    int32_t* dbg_id = nullptr;  // slot -> id at the pressure stage of the last step
+    double* snap_pos = nullptr;
+    double* snap_vel = nullptr;
+    double* tmp3a = nullptr;  // 3n scratch for downloads
+    double* tmp3b = nullptr;
+    int8_t* tmp8 = nullptr;
+    Sell A{};
+    CgWork* cg = nullptr;
+    mps_solver_options opt{};
+    mps_solver_stats stats{};
+    Events ev{};
+    int* dev_counters = nullptr;  // [0] ghost, [1] unconnected, [2] err
+    int ev_cap = 0;
+    std::vector<int32_t> ghost_ids, unconn_ids;
+    int64_t counts[3] = {0, 0, 0};
+    int* pinned_i = nullptr;  // host pinned small buffer
+    double* pinned_d = nullptr;
+    double* vmax = nullptr;
+    int want_raw = 0;
+    int want_snap = 0;
+    int have_system = 0;
+    int built_once = 0;
+};
+
+static std::string g_create_error = "no error";
+
+static int fail(mps_handle* h, int code, const std::string& m) {
+    if (h) h->err = m;
+    return code;
+}
+
+static int hip_fail(mps_handle* h, hipError_t e, const char* where) {
+    return fail(h, MPS_E_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(h, expr)                                       \
+    do {                                                      \
+        hipError_t _e = (expr);                               \
+        if (_e != hipSuccess) return hip_fail((h), _e, #expr); \
+    } while (0)
+
+template <typename T>
+static hipError_t dalloc(T** p, size_t count) {
+    if (count == 0) count = 1;
+    return hipMalloc((void**) p, count * sizeof(T));
+}
+
+static Hot hot_view(mps_handle* h, int pv, int it) {
+    Hot v;
+    v.px = h->P[pv][0];
+    v.py = h->P[pv][1];
+    v.pz = h->P[pv][2];
+    v.vx = h->P[pv][3];
+    v.vy = h->P[pv][4];
+    v.vz = h->P[pv][5];
+    v.id = h->ID[it];
+    v.type = h->TY[it];
+    v.key = h->KEY[it];
+    return v;
+}
+
+static Hot cur(mps_handle* h) { return hot_view(h, h->pv, h->it); }
+
+static void ref_values(int dim, double l0, double re, double* n0_out, double* lam_out) {
+    // RefValues (src/mps.cpp:132-163), host-side, FP64 in the reference's order
+    int iz0 = dim == 2 ? 0 : -4, iz1 = dim == 2 ? 1 : 5;
+    double n0 = 0.0, lam = 0.0;
+    for (int ix = -4; ix < 5; ix++)
+        for (int iy = -4; iy < 5; iy++)
+            for (int iz = iz0; iz < iz1; iz++) {
+                if (ix == 0 && iy == 0 && iz == 0) continue;
+                const double rx = l0 * (double) ix, ry = l0 * (double) iy, rz = l0 * (double) iz;
+                const double dist = sqrt((rx * rx + ry * ry) + rz * rz);
+                const double dist2 = dist * dist;
+                double w = 0.0;
+                if (dist < re) w = (re / dist) - 1.0;
+                n0 += w;
+                lam += dist2 * w;
+            }
+    lam /= n0;
+    *n0_out = n0;
+    *lam_out = lam;
+}
+
+static int gridview_build(mps_handle* h) {
+    const mps_settings& s = h->s;
+    GridView& g = h->g;
+    g.xmin = s.domain_min[0];
+    g.ymin = s.domain_min[1];
+    g.zmin = s.domain_min[2];
+    g.len = s.re_max;
+    // Bucket::Bucket (src/bucket.cpp:10-22)
+    const double lx = s.domain_max[0] - s.domain_min[0];
+    const double ly = s.domain_max[1] - s.domain_min[1];
+    const double lz = s.domain_max[2] - s.domain_min[2];
+    const double fx = lx / g.len, fy = ly / g.len, fz = lz / g.len;
+    if (!(fx >= 0 && fy >= 0 && fz >= 0) || fx > 1e8 || fy > 1e8 || fz > 1e8)
+        return fail(h, MPS_E_INVALID, "domain range is not a valid bucket grid");
+    g.nx = (int) fx + 3;
+    g.ny = (int) fy + 3;
+    g.nz = (int) fz + 3;
+    const double cells = (double) g.nx * g.ny * (h->dim == 2 ? 1 : g.nz);
+    if (cells > 1.0e9) return fail(h, MPS_E_INVALID, "bucket grid too large");
+    g.flat = (h->dim == 2 || g.nz == 3) ? 1 : 0;
+    g.ncell = g.flat ? g.nx * g.ny : g.nx * g.ny * g.nz;
+    g.cs = nullptr;
+    return MPS_OK;
+}
+
+static int alloc_all(mps_handle* h) {
+    const size_t n = (size_t) h->n;
+    const int d3 = h->dim == 3;
+    for (int k = 0; k < 2; k++) {
+        for (int f = 0; f < 6; f++) {
+            if (!d3 && (f == 2 || f == 5)) continue;
+            HIPCHK(h, dalloc(&h->P[k][f], n));
+        }
+        HIPCHK(h, dalloc(&h->ID[k], n));
+        HIPCHK(h, dalloc(&h->TY[k], n));
+        HIPCHK(h, dalloc(&h->KEY[k], n));
+    }
+    const size_t nc = (size_t) h->g.ncell;
+    HIPCHK(h, dalloc(&h->bb.cnt, nc + 1));
+    HIPCHK(h, dalloc(&h->bb.cs, nc + 2));
+    HIPCHK(h, dalloc(&h->bb.bin_id, n));
+    HIPCHK(h, dalloc(&h->bb.dst, n));
+    const size_t scan_m = std::max(nc + 1, n + 1);
+    HIPCHK(h, dalloc(&h->bb.scan_tmp, scan_tmp_ints((int) scan_m) + 16));
+    HIPCHK(h, dalloc(&h->bb.slot_of, n));
+    HIPCHK(h, dalloc(&h->cold.p, n));
+    HIPCHK(h, dalloc(&h->cold.pmin, n));
+    HIPCHK(h, dalloc(&h->cold.ndr, n));
+    HIPCHK(h, dalloc(&h->cold.bc, n));
+    HIPCHK(h, dalloc(&h->cold.pprev, n));
+    HIPCHK(h, dalloc(&h->cold.ghost_n, n));
+    HIPCHK(h, dalloc(&h->cold.z, n));
+    HIPCHK(h, dalloc(&h->cold.vz, n));
+    StepScratch& w = h->w;
+    HIPCHK(h, dalloc(&w.bc_s, n));
+    HIPCHK(h, dalloc(&w.b_s, n));
+    HIPCHK(h, dalloc(&w.diag_s, n));
+    HIPCHK(h, dalloc(&w.nnz_s, n));
+    HIPCHK(h, dalloc(&w.parent, n));
+    HIPCHK(h, dalloc(&w.surf, n + 8));
+    HIPCHK(h, dalloc(&w.inner_flag, n));
+    HIPCHK(h, dalloc(&w.row_of, n + 1));
+    HIPCHK(h, dalloc(&w.p_s, n));
+    HIPCHK(h, dalloc(&w.pmin_s, n));
+    HIPCHK(h, dalloc(&w.vmax, 1));
+    HIPCHK(h, dalloc(&w.coll_cnt, n));
+    HIPCHK(h, dalloc(&w.coll_off, n + 1));
+    w.coll_cap = (int) std::min<size_t>(std::max<size_t>(4 * n, 4096), (size_t) 1 << 26);
+    HIPCHK(h, dalloc(&w.coll_pairs, 2 * (size_t) w.coll_cap));
+    HIPCHK(h, dalloc(&w.coll_dist, (size_t) w.coll_cap));
+    HIPCHK(h, dalloc(&w.coll_level, (size_t) w.coll_cap));
+    HIPCHK(h, dalloc(&w.coll_last, n));
+    w.scan_tmp = h->bb.scan_tmp;
+    HIPCHK(h, dalloc(&h->x_s, n));
+    HIPCHK(h, dalloc(&h->dbg_id, n));
+    HIPCHK(h, dalloc(&h->tmp3a, 3 * n));
+    HIPCHK(h, dalloc(&h->tmp3b, 3 * n));
+    HIPCHK(h, dalloc(&h->tmp8, n));
+    const size_t nsl = (n + 63) / 64 + 2;
+    HIPCHK(h, dalloc(&h->A.slice_off, nsl + 1));
+    HIPCHK(h, dalloc(&h->A.slice_w, nsl + 1));
+    HIPCHK(h, dalloc(&h->A.row_slot, n));
+    HIPCHK(h, dalloc(&h->A.sdiag, n));
+    HIPCHK(h, dalloc(&h->A.dgl, n));
+    HIPCHK(h, dalloc(&h->A.raw_off, n + 1));
+    h->ev_cap = (int) std::max<size_t>(n, 1024);
+    HIPCHK(h, dalloc(&h->dev_counters, 4));
+    HIPCHK(h, dalloc(&h->ev.ghost_ids, (size_t) h->ev_cap));
+    HIPCHK(h, dalloc(&h->ev.unconn_ids, (size_t) h->ev_cap));
+    h->ev.n_ghost = h->dev_counters;
+    h->ev.n_unconn = h->dev_counters + 1;
+    h->ev.err = h->dev_counters + 2;
+    h->ev.cap = h->ev_cap;
+    HIPCHK(h, hipHostMalloc((void**) &h->pinned_i, 16 * sizeof(int), hipHostMallocDefault));
+    HIPCHK(h, hipHostMalloc((void**) &h->pinned_d, 16 * sizeof(double), hipHostMallocDefault));
+    HIPCHK(h, fill_int(h->bb.cnt, 0, nc + 1, h->st));
+    HIPCHK(h, fill_int(h->w.coll_last, 0, n, h->st));
+    HIPCHK(h, fill_int(h->dev_counters, 0, 4, h->st));
+    HIPCHK(h, fill_double(h->cold.p, 0.0, n, h->st));
+    HIPCHK(h, fill_double(h->cold.pmin, 0.0, n, h->st));
+    HIPCHK(h, fill_double(h->cold.ndr, 0.0, n, h->st));
+    HIPCHK(h, fill_double(h->cold.pprev, 0.0, n, h->st));
+    HIPCHK(h, fill_double(h->cold.ghost_n, 0.0, n, h->st));
+    HIPCHK(h, hipMemsetAsync(h->cold.bc, 0, n, h->st));  // Ignored
+    h->cg = cg_create();
+    return MPS_OK;
+}
+
+static void free_all(mps_handle* h) {
+    for (int k = 0; k < 2; k++) {
+        for (int f = 0; f < 6; f++) hipFree(h->P[k][f]);
+        hipFree(h->ID[k]);
+        hipFree(h->TY[k]);
+        hipFree(h->KEY[k]);
+    }
+    void* ptrs[] = {h->bb.cnt,      h->bb.cs,         h->bb.bin_id,  h->bb.dst,       h->bb.scan_tmp, h->bb.slot_of,
+                    h->cold.p,      h->cold.pmin,     h->cold.ndr,   h->cold.bc,      h->cold.pprev,  h->cold.ghost_n,
+                    h->cold.z,      h->cold.vz,       h->w.bc_s,     h->w.b_s,        h->w.diag_s,    h->w.nnz_s,
+                    h->w.parent,    h->w.surf,        h->w.inner_flag, h->w.row_of,   h->w.p_s,       h->w.pmin_s,
+                    h->w.vmax,      h->w.coll_cnt,    h->w.coll_off, h->w.coll_pairs, h->w.coll_dist, h->w.coll_level,
+                    h->w.coll_last, h->x_s,           h->dbg_id,     h->tmp3a,        h->tmp3b,       h->tmp8,
+                    h->A.slice_off, h->A.slice_w,     h->A.row_slot, h->A.sdiag,      h->A.dgl,       h->A.raw_off,
+                    h->A.col,       h->A.val64,       h->A.val32,    h->A.raw_col,    h->A.raw_val,   h->dev_counters,
+                    h->ev.ghost_ids, h->ev.unconn_ids, h->snap_pos,  h->snap_vel};
+    for (void* p : ptrs) hipFree(p);
+    if (h->pinned_i) hipHostFree(h->pinned_i);
+    if (h->pinned_d) hipHostFree(h->pinned_d);
+    if (h->cg) cg_destroy(h->cg);
+    if (h->st) hipStreamDestroy(h->st);
+}
+
+// ---------------- internal stage drivers ----------------
+
+static int do_build(mps_handle* h) {
+    Hot c = cur(h);
+    Hot nx = hot_view(h, 1 - h->pv, 1 - h->it);
+    HIPCHK(h, build_grid(h->dim, h->g, h->n, c, nx, h->bb, h->ev.err, h->st));
+    h->pv ^= 1;
+    h->it ^= 1;
+    h->built_once = 1;
+    return MPS_OK;
+}
+
+static GridView gview(mps_handle* h) {
+    GridView g = h->g;
+    g.cs = h->bb.cs;
+    return g;
+}
+
+// Reads back the device error mask and event counters (synchronises).
+static int drain(mps_handle* h) {
+    HIPCHK(h, hipMemcpyAsync(h->pinned_i, h->dev_counters, 4 * sizeof(int), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    const int ng = h->pinned_i[0], nu = h->pinned_i[1], er = h->pinned_i[2];
+    if (ng > 0 || nu > 0) {
+        std::vector<int32_t> tmp((size_t) std::max(std::min(ng, h->ev_cap), std::min(nu, h->ev_cap)));
+        if (ng > 0) {
+            const int m = std::min(ng, h->ev_cap);
+            HIPCHK(h, hipMemcpy(tmp.data(), h->ev.ghost_ids, sizeof(int32_t) * (size_t) m, hipMemcpyDeviceToHost));
+            h->ghost_ids.insert(h->ghost_ids.end(), tmp.begin(), tmp.begin() + m);
+        }
+        if (nu > 0) {
+            const int m = std::min(nu, h->ev_cap);
+            HIPCHK(h, hipMemcpy(tmp.data(), h->ev.unconn_ids, sizeof(int32_t) * (size_t) m, hipMemcpyDeviceToHost));
+            h->unconn_ids.insert(h->unconn_ids.end(), tmp.begin(), tmp.begin() + m);
+        }
+        h->counts[0] += ng;
+        h->counts[1] += nu;
+        HIPCHK(h, fill_int(h->dev_counters, 0, 2, h->st));
+    }
+    if (er) {
+        HIPCHK(h, fill_int(h->dev_counters + 2, 0, 1, h->st));
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        if (er & kErrRange) return fail(h, MPS_E_RANGE, "a particle fell outside the bucket grid");
+        if (er & kErrCollCap) return fail(h, MPS_E_INVALID, "collision pair buffer overflow");
+        return fail(h, MPS_E_INVALID, "device error flag set");
+    }
+    return MPS_OK;
+}
+
+static int number_density(mps_handle* h) {
+    int rc = do_build(h);
+    if (rc) return rc;
+    GridView g = gview(h);
+    HIPCHK(h, launch_ndr(h->dim, g, h->n, h->c, cur(h), h->cold, h->st));
+    return MPS_OK;
+}
+
+static int step_core(mps_handle* h, int is_export, double* vmax_out) {
+    int rc;
+    const int n = h->n;
+    // setNeighbors #1, gravity, viscosity, move
+    if ((rc = do_build(h))) return rc;
+    {
+        GridView g = gview(h);
+        Hot c = cur(h);
+        Hot o = hot_view(h, 1 - h->pv, h->it);
+        HIPCHK(h, launch_visc_move(h->dim, g, n, h->c, c, o, h->cold, h->ev, h->st));
+        h->pv ^= 1;
+    }
+    // setNeighbors #2, collision
+    if ((rc = do_build(h))) return rc;
+    {
+        GridView g = gview(h);
+        Hot c = cur(h);
+        HIPCHK(h, launch_collision(h->dim, g, n, h->c, c, h->w, h->ev.err, h->st));
+    }
+    // setNeighbors #3, pressure
+    if ((rc = do_build(h))) return rc;
+    GridView g = gview(h);
+    Hot c = cur(h);
+    if (h->want_snap) HIPCHK(h, launch_snapshot(h->dim, n, c, h->cold, h->snap_pos, h->snap_vel, h->st));
+    if (h->want_raw) HIPCHK(h, hipMemcpyAsync(h->dbg_id, c.id, sizeof(int32_t) * (size_t) n, hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(h, launch_bc_source(h->dim, g, n, h->c, c, h->cold, h->w, h->st));
+    HIPCHK(h, launch_diag_cc(h->dim, g, n, h->c, c, h->w, h->ev, h->st));
+    int rows = 0;
+    size_t entries = 0;
+    HIPCHK(h, assemble_sell(h->dim, g, n, h->c, c, h->w, h->A, h->opt.precision == MPS_PREC_MIXED, h->want_raw,
+                            h->ev.err, &rows, &entries, h->st));
+    std::string msg;
+    HIPCHK(h, cg_solve(h->cg, h->A, c, n, h->w.b_s, h->cold.pprev, h->x_s, h->opt, h->stats, h->st, msg));
+    HIPCHK(h, launch_pressure_out(h->dim, g, n, h->c, c, h->x_s, h->cold, h->w, h->st));
+    h->have_system = h->want_raw;
+    // pressure gradient + move
+    {
+        Hot o = hot_view(h, 1 - h->pv, h->it);
+        HIPCHK(h, launch_grad_move(h->dim, g, n, h->c, c, o, h->cold, h->w, h->ev, h->st));
+        h->pv ^= 1;
+    }
+    if (is_export || h->c.hos_on) {
+        if ((rc = number_density(h))) return rc;
+    }
+    HIPCHK(h, launch_courant(h->dim, n, cur(h), h->w.vmax, h->st));
+    if (vmax_out) {
+        HIPCHK(h, hipMemcpyAsync(h->pinned_d, h->w.vmax, sizeof(double), hipMemcpyDeviceToHost, h->st));
+    }
+    return MPS_OK;
+}
+
+static double courant_of(mps_handle* h, double vmax) {
+    // src/mps.cpp:554-565
+    double cn = (vmax * h->s.dt) / h->s.particle_distance;
+    if (cn > h->s.cfl_condition) h->counts[2]++;
+    return cn;
+}
+
+// ---------------- C ABI ----------------
+
+extern "C" {
+
+const char* mps_last_error(const mps_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int mps_create(const mps_settings* s, int32_t n, const int32_t* id, const int8_t* type, const double* pos,
+               const double* vel, mps_handle** out) {
+    if (!out) {
+        g_create_error = "out is NULL";
+        return MPS_E_INVALID;
+    }
+    *out = nullptr;
+    if (!s || n < 0 || (n > 0 && (!id || !type || !pos || !vel))) {
+        g_create_error = "invalid arguments";
+        return MPS_E_INVALID;
+    }
+    if (s->dim != 2 && s->dim != 3) {
+        g_create_error = "dim must be 2 or 3";
+        return MPS_E_INVALID;
+    }
+    if (!(s->re_max > 0) || s->re_max < s->re_pressure || s->re_max < s->re_viscosity || s->re_max < s->re_surface ||
+        !(s->re_pressure > 0) || !(s->re_viscosity > 0) || !(s->re_surface > 0) || !(s->particle_distance > 0) ||
+        !(s->dt > 0) || !(s->density > 0)) {
+        g_create_error = "invalid settings (radii, particle distance, dt or density)";
+        return MPS_E_INVALID;
+    }
+    for (int32_t i = 0; i < n; i++) {
+        if (id[i] != i) {
+            g_create_error = "particle ids must be 0..N-1 in order";
+            return MPS_E_INVALID;
+        }
+        if (type[i] < -1 || type[i] > 2) {
+            g_create_error = "particle type out of range";
+            return MPS_E_INVALID;
+        }
+    }
+    mps_handle* h = new mps_handle();
+    h->s = *s;
+    h->n = n;
+    h->dim = s->dim;
+    if (h->dim == 2) {
+        // In 2D every non-ghost particle must sit on one z plane with zero
+        // z velocity and zero z gravity: the reference would ghost any other
+        // particle at its first move (src/mps.cpp:651-653).
+        if (s->gravity[2] != 0.0) {
+            g_create_error = "2D requires gravity z = 0";
+            delete h;
+            return MPS_E_INVALID;
+        }
+        bool have = false;
+        double z0 = 0.0;
+        for (int32_t i = 0; i < n; i++) {
+            if (type[i] == MPS_GHOST) continue;
+            if (vel[3 * (size_t) i + 2] != 0.0 || (have && pos[3 * (size_t) i + 2] != z0)) {
+                g_create_error = "2D requires every non-ghost particle on one z plane with zero z velocity";
+                delete h;
+                return MPS_E_INVALID;
+            }
+            z0 = pos[3 * (size_t) i + 2];
+            have = true;
+        }
+        if (have) {
+            const double q = (z0 - s->domain_min[2]) / s->re_max;
+            if (!(q > -1.0 && q < 1e8)) {
+                g_create_error = "z plane outside the bucket grid";
+                delete h;
+                return MPS_E_INVALID;
+            }
+            h->iz0 = (int) q + 1;
+        }
+    }
+    // constants (Settings-derived, src/settings.cpp:43-61; RefValues src/mps.cpp:21-35)
+    Consts& c = h->c;
+    c.dim = s->dim;
+    c.dt = s->dt;
+    c.l0 = s->particle_distance;
+    c.rho = s->density;
+    c.nu = s->kinematic_viscosity;
+    for (int k = 0; k < 3; k++) {
+        c.g[k] = s->gravity[k];
+        c.dmin[k] = s->domain_min[k];
+        c.dmax[k] = s->domain_max[k];
+    }
+    c.re_p = s->re_pressure;
+    c.re_v = s->re_viscosity;
+    c.re_s = s->re_surface;
+    c.re_max = s->re_max;
+    ref_values(s->dim, s->particle_distance, s->re_pressure, &c.n0_p, &c.lam_p);
+    ref_values(s->dim, s->particle_distance, s->re_viscosity, &c.n0_v, &c.lam_v);
+    ref_values(s->dim, s->particle_distance, s->re_surface, &c.n0_s, &c.lam_s);
+    c.beta = s->surface_threshold;
+    c.relax = s->relaxation;
+    c.qc_on = s->qc_on;
+    c.alpha = s->qc_on ? s->compressibility : 0.0;
+    c.hos_on = s->hos_on;
+    c.hos_gamma = s->hos_gamma;
+    c.dc = s->collision_distance;
+    c.restitution = s->restitution;
+    int rc = gridview_build(h);
+    if (rc) {
+        g_create_error = h->err;
+        delete h;
+        return rc;
+    }
+    if (h->dim == 2 && (h->iz0 < 1 || h->iz0 > h->g.nz - 2)) {
+        g_create_error = "z plane outside the bucket grid";
+        delete h;
+        return MPS_E_INVALID;
+    }
+    hipError_t e = hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        g_create_error = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+        delete h;
+        return MPS_E_HIP;
+    }
+    rc = alloc_all(h);
+    if (rc) {
+        g_create_error = h->err;
+        free_all(h);
+        delete h;
+        return rc;
+    }
+    // upload (slots = ids until the first build)
+    const size_t N = (size_t) n;
+    std::vector<double> comp(N);
+    const int fields = h->dim == 3 ? 3 : 2;
+    for (int f = 0; f < fields; f++) {
+        for (size_t i = 0; i < N; i++) comp[i] = pos[3 * i + f];
+        e = hipMemcpy(h->P[0][f], comp.data(), N * sizeof(double), hipMemcpyHostToDevice);
+        for (size_t i = 0; i < N && e == hipSuccess; i++) comp[i] = vel[3 * i + f];
+        if (e == hipSuccess) e = hipMemcpy(h->P[0][3 + f], comp.data(), N * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) break;
+    }
+    if (e == hipSuccess) {
+        for (size_t i = 0; i < N; i++) comp[i] = pos[3 * i + 2];
+        e = hipMemcpy(h->cold.z, comp.data(), N * sizeof(double), hipMemcpyHostToDevice);
+        for (size_t i = 0; i < N && e == hipSuccess; i++) comp[i] = vel[3 * i + 2];
+        if (e == hipSuccess) e = hipMemcpy(h->cold.vz, comp.data(), N * sizeof(double), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess && n > 0) {
+        std::vector<int32_t> ids(N);
+        for (size_t i = 0; i < N; i++) ids[i] = (int32_t) i;
+        e = hipMemcpy(h->ID[0], ids.data(), N * sizeof(int32_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h->bb.slot_of, ids.data(), N * sizeof(int32_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h->TY[0], type, N, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        g_create_error = std::string("upload: ") + hipGetErrorString(e);
+        free_all(h);
+        delete h;
+        return MPS_E_HIP;
+    }
+    // reference-faithful solver defaults: FP64, tolerance = machine epsilon,
+    // at most 2 x rows iterations, zero initial guess (Eigen defaults)
+    h->opt.precision = MPS_PREC_FP64;
+    h->opt.warm_start = 0;
+    h->opt.max_iterations = 0;
+    h->opt.max_refinements = 0;
+    h->opt.tolerance = DBL_EPSILON;
+    h->opt.inner_tolerance = 1e-4;
+    h->opt.check_interval = 32;
+    h->opt.use_graph = 1;
+    *out = h;
+    return MPS_OK;
+}
+
+int mps_destroy(mps_handle* h) {
+    if (!h) return MPS_OK;
+    if (h->st) hipStreamSynchronize(h->st);
+    free_all(h);
+    delete h;
+    return MPS_OK;
+}
+
+int mps_set_solver(mps_handle* h, const mps_solver_options* o) {
+    if (!h || !o) return MPS_E_INVALID;
+    if (o->precision != MPS_PREC_FP64 && o->precision != MPS_PREC_MIXED)
+        return fail(h, MPS_E_INVALID, "unknown precision");
+    if (!(o->tolerance > 0)) return fail(h, MPS_E_INVALID, "tolerance must be positive");
+    h->opt = *o;
+    if (h->opt.check_interval < 1) h->opt.check_interval = 1;
+    return MPS_OK;
+}
+
+int mps_get_solver_stats(const mps_handle* h, mps_solver_stats* out) {
+    if (!h || !out) return MPS_E_INVALID;
+    *out = h->stats;
+    return MPS_OK;
+}
+
+int mps_load_initial_state(mps_handle* h) {
+    if (!h) return MPS_E_INVALID;
+    int rc = number_density(h);
+    if (rc) return rc;
+    return drain(h);
+}
+
+int mps_step(mps_handle* h, int32_t is_export, double* courant) {
+    if (!h) return MPS_E_INVALID;
+    int rc = step_core(h, is_export, h->pinned_d);
+    if (rc) return rc;
+    rc = drain(h);
+    if (rc) return rc;
+    const double cn = courant_of(h, h->pinned_d[0]);
+    if (courant) *courant = cn;
+    return MPS_OK;
+}
+
+int mps_run_steps(mps_handle* h, int32_t n_steps, double* courants) {
+    if (!h || n_steps < 0) return MPS_E_INVALID;
+    for (int32_t k = 0; k < n_steps; k++) {
+        int rc = step_core(h, 0, h->pinned_d);
+        if (rc) return rc;
+        if ((rc = drain(h))) return rc;
+        const double cn = courant_of(h, h->pinned_d[0]);
+        if (courants) courants[k] = cn;
+    }
+    return MPS_OK;
+}
+
+static int gather_hot(const mps_handle* hc, int8_t* type, double* pos, double* vel) {
+    mps_handle* h = const_cast<mps_handle*>(hc);
+    const size_t N = (size_t) h->n;
+    if (N == 0) return MPS_OK;
+    Hot c = cur(h);
+    if (pos || vel) {
+        HIPCHK(h, launch_snapshot(h->dim, h->n, c, h->cold, h->tmp3a, h->tmp3b, h->st));
+        if (pos) HIPCHK(h, hipMemcpyAsync(pos, h->tmp3a, 3 * N * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        if (vel) HIPCHK(h, hipMemcpyAsync(vel, h->tmp3b, 3 * N * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    }
+    if (type) {
+        std::vector<int8_t> ts(N);
+        std::vector<int32_t> ids(N);
+        HIPCHK(h, hipMemcpyAsync(ts.data(), c.type, N, hipMemcpyDeviceToHost, h->st));
+        HIPCHK(h, hipMemcpyAsync(ids.data(), c.id, N * sizeof(int32_t), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        for (size_t s = 0; s < N; s++) type[ids[s]] = ts[s];
+    }
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return MPS_OK;
+}
+
+int mps_download(const mps_handle* hc, int8_t* type, double* pos, double* vel, double* pressure, double* ndr,
+                 int8_t* bc) {
+    if (!hc) return MPS_E_INVALID;
+    mps_handle* h = const_cast<mps_handle*>(hc);
+    int rc = gather_hot(h, type, pos, vel);
+    if (rc) return rc;
+    const size_t N = (size_t) h->n;
+    if (N == 0) return MPS_OK;
+    if (pressure) HIPCHK(h, hipMemcpyAsync(pressure, h->cold.p, N * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    if (ndr) HIPCHK(h, hipMemcpyAsync(ndr, h->cold.ndr, N * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    if (bc) HIPCHK(h, hipMemcpyAsync(bc, h->cold.bc, N, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return MPS_OK;
+}
+
+int mps_download_min_pressure(const mps_handle* hc, double* pmin) {
+    if (!hc || !pmin) return MPS_E_INVALID;
+    mps_handle* h = const_cast<mps_handle*>(hc);
+    if (h->n == 0) return MPS_OK;
+    HIPCHK(h, hipMemcpyAsync(pmin, h->cold.pmin, (size_t) h->n * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return MPS_OK;
+}
+
+int mps_upload(mps_handle* h, const int8_t* type, const double* pos, const double* vel, const double* pressure,
+               const double* ndr, const int8_t* bc) {
+    if (!h) return MPS_E_INVALID;
+    const size_t N = (size_t) h->n;
+    if (N == 0) return MPS_OK;
+    // host copies of slot_of and the current hot arrays, rewritten in slot order
+    std::vector<int32_t> slot(N);
+    HIPCHK(h, hipMemcpy(slot.data(), h->bb.slot_of, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+    Hot c = cur(h);
+    if (type) {
+        for (size_t i = 0; i < N; i++)
+            if (type[i] < -1 || type[i] > 2) return fail(h, MPS_E_INVALID, "particle type out of range");
+        std::vector<int8_t> ts(N);
+        for (size_t i = 0; i < N; i++) ts[slot[i]] = type[i];
+        HIPCHK(h, hipMemcpy(c.type, ts.data(), N, hipMemcpyHostToDevice));
+    }
+    if (h->dim == 2 && (pos || vel)) {
+        // keep the 2D plane invariant (see mps_create)
+        std::vector<int8_t> ts(N);
+        std::vector<int8_t> tid(N);
+        HIPCHK(h, hipMemcpy(ts.data(), c.type, N, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; i++) tid[i] = ts[slot[i]];
+        std::vector<double> zc(N);
+        HIPCHK(h, hipMemcpy(zc.data(), h->cold.z, N * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; i++) {
+            if (tid[i] == MPS_GHOST) continue;
+            if ((pos && pos[3 * i + 2] != zc[i]) || (vel && vel[3 * i + 2] != 0.0))
+                return fail(h, MPS_E_INVALID, "2D upload must keep z and z velocity of non-ghost particles");
+        }
+    }
+    std::vector<double> comp(N);
+    const int fields = h->dim == 3 ? 3 : 2;
+    double* dstp[6] = {c.px, c.py, c.pz, c.vx, c.vy, c.vz};
+    for (int f = 0; f < fields; f++) {
+        if (pos) {
+            for (size_t i = 0; i < N; i++) comp[slot[i]] = pos[3 * i + f];
+            HIPCHK(h, hipMemcpy(dstp[f], comp.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        }
+        if (vel) {
+            for (size_t i = 0; i < N; i++) comp[slot[i]] = vel[3 * i + f];
+            HIPCHK(h, hipMemcpy(dstp[3 + f], comp.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    if (h->dim == 2) {
+        if (pos) {
+            for (size_t i = 0; i < N; i++) comp[i] = pos[3 * i + 2];
+            HIPCHK(h, hipMemcpy(h->cold.z, comp.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        }
+        if (vel) {
+            for (size_t i = 0; i < N; i++) comp[i] = vel[3 * i + 2];
+            HIPCHK(h, hipMemcpy(h->cold.vz, comp.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    if (pressure) {
+        HIPCHK(h, hipMemcpy(h->cold.p, pressure, N * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->cold.pprev, pressure, N * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (ndr) HIPCHK(h, hipMemcpy(h->cold.ndr, ndr, N * sizeof(double), hipMemcpyHostToDevice));
+    if (bc) {
+        for (size_t i = 0; i < N; i++)
+            if (bc[i] < 0 || bc[i] > 2) return fail(h, MPS_E_INVALID, "boundary condition out of range");
+        HIPCHK(h, hipMemcpy(h->cold.bc, bc, N, hipMemcpyHostToDevice));
+    }
+    return MPS_OK;
+}
+
+int mps_debug_events(const mps_handle* hc, int64_t counts[3], int32_t* ghost_ids, int32_t* unconnected_ids,
+                     int32_t cap) {
+    if (!hc || !counts) return MPS_E_INVALID;
+    mps_handle* h = const_cast<mps_handle*>(hc);
+    int rc = drain(h);
+    if (rc) return rc;
+    for (int k = 0; k < 3; k++) counts[k] = h->counts[k];
+    std::vector<int32_t> a = h->ghost_ids, b = h->unconn_ids;
+    std::sort(a.begin(), a.end());
+    std::sort(b.begin(), b.end());
+    if (ghost_ids)
+        for (size_t k = 0; k < a.size() && (int32_t) k < cap; k++) ghost_ids[k] = a[k];
+    if (unconnected_ids)
+        for (size_t k = 0; k < b.size() && (int32_t) k < cap; k++) unconnected_ids[k] = b[k];
+    return MPS_OK;
+}
+
+int mps_clear_debug_events(mps_handle* h) {
+    if (!h) return MPS_E_INVALID;
+    int rc = drain(h);
+    if (rc) return rc;
+    h->ghost_ids.clear();
+    h->unconn_ids.clear();
+    h->counts[0] = h->counts[1] = h->counts[2] = 0;
+    return MPS_OK;
+}
+
+int mps_bucket_shape(const mps_handle* h, int32_t dims[4], double* length) {
+    if (!h || !dims) return MPS_E_INVALID;
+    dims[0] = h->g.nx * h->g.ny * h->g.nz;
+    dims[1] = h->g.nx;
+    dims[2] = h->g.ny;
+    dims[3] = h->g.nz;
+    if (length) *length = h->g.len;
+    return MPS_OK;
+}
+
+int mps_dump_bucket(mps_handle* h, int32_t* first, int32_t* last, int32_t* next) {
+    if (!h || !first || !last || !next) return MPS_E_INVALID;
+    int rc = do_build(h);
+    if (rc) return rc;
+    if ((rc = drain(h))) return rc;
+    const size_t N = (size_t) h->n;
+    const size_t nc = (size_t) h->g.ncell;
+    std::vector<int> cs(nc + 2);
+    std::vector<int32_t> ids(N);
+    HIPCHK(h, hipMemcpy(cs.data(), h->bb.cs, (nc + 2) * sizeof(int), hipMemcpyDeviceToHost));
+    if (N) HIPCHK(h, hipMemcpy(ids.data(), cur(h).id, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+    const size_t num = (size_t) h->g.nx * h->g.ny * h->g.nz;
+    for (size_t k = 0; k < num; k++) first[k] = last[k] = -1;
+    for (size_t i = 0; i < N; i++) next[i] = -1;
+    const size_t layer = h->g.flat && h->dim == 2 ? (size_t) h->iz0 * h->g.nx * h->g.ny
+                                                  : (h->g.flat ? (size_t) h->g.nx * h->g.ny : 0);
+    for (size_t k = 0; k < nc; k++) {
+        const int b = cs[k], e = cs[k + 1];
+        if (b == e) continue;
+        const size_t rk = k + layer;  // the reference's key: ix + iy*numX + iz*numX*numY
+        first[rk] = ids[b];
+        last[rk] = ids[e - 1];
+        for (int q = b; q + 1 < e; q++) next[ids[q]] = ids[q + 1];
+    }
+    return MPS_OK;
+}
+
+int mps_dump_neighbors(mps_handle* h, int32_t* offsets, int32_t* ids, double* dists) {
+    if (!h || !offsets) return MPS_E_INVALID;
+    int rc = do_build(h);
+    if (rc) return rc;
+    if ((rc = drain(h))) return rc;
+    const int n = h->n;
+    GridView g = gview(h);
+    Hot c = cur(h);
+    int* cnt = h->w.coll_cnt;
+    int* off = h->w.coll_off;
+    HIPCHK(h, dump_neighbor_counts(h->dim, g, n, c, cnt, h->s.re_max, h->st));
+    HIPCHK(h, scan_exclusive(cnt, off, n, h->bb.scan_tmp, h->st));
+    HIPCHK(h, hipMemcpyAsync(offsets, off, ((size_t) n + 1) * sizeof(int), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    if (!ids && !dists) return MPS_OK;
+    const size_t tot = (size_t) offsets[n];
+    int32_t* d_ids = nullptr;
+    double* d_dist = nullptr;
+    HIPCHK(h, dalloc(&d_ids, tot));
+    hipError_t e = dalloc(&d_dist, tot);
+    if (e == hipSuccess) e = dump_neighbor_lists(h->dim, g, n, c, off, d_ids, d_dist, h->s.re_max, h->st);
+    if (e == hipSuccess && ids && tot)
+        e = hipMemcpyAsync(ids, d_ids, tot * sizeof(int32_t), hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess && dists && tot)
+        e = hipMemcpyAsync(dists, d_dist, tot * sizeof(double), hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    hipFree(d_ids);
+    hipFree(d_dist);
+    if (e != hipSuccess) return hip_fail(h, e, "mps_dump_neighbors");
+    return MPS_OK;
+}
+
+int mps_get_pressure_system(const mps_handle* hc, double* b, double* diag, int32_t* row_ptr, int32_t* col,
+                            double* val, double* x) {
+    if (!hc) return MPS_E_INVALID;
+    mps_handle* h = const_cast<mps_handle*>(hc);
+    if (!h->have_system) return fail(h, MPS_E_INVALID, "no captured pressure system: enable capture and step");
+    const size_t N = (size_t) h->n;
+    std::vector<int32_t> sid(N);
+    std::vector<double> bs(N), ds(N), xs(N);
+    std::vector<int8_t> bcs(N);
+    std::vector<int> ro(N + 1);
+    HIPCHK(h, hipMemcpy(sid.data(), h->dbg_id, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(bs.data(), h->w.b_s, N * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(ds.data(), h->w.diag_s, N * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(xs.data(), h->x_s, N * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(bcs.data(), h->w.bc_s, N, hipMemcpyDeviceToHost));
+    HIPCHK(h, hipMemcpy(ro.data(), h->A.raw_off, (N + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    const size_t nnz = (size_t) ro[N];
+    std::vector<int> rc(nnz);
+    std::vector<double> rv(nnz);
+    if (nnz) {
+        HIPCHK(h, hipMemcpy(rc.data(), h->A.raw_col, nnz * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(h, hipMemcpy(rv.data(), h->A.raw_val, nnz * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    std::vector<int32_t> cnt(N, 0);
+    for (size_t s = 0; s < N; s++) {
+        const int32_t i = sid[s];
+        if (b) b[i] = bs[s];
+        if (diag) diag[i] = ds[s];
+        if (x) x[i] = bcs[s] == MPS_BC_INNER ? xs[s] : 0.0;
+        cnt[i] = ro[s + 1] - ro[s];
+    }
+    if (row_ptr) {
+        std::vector<int32_t> rp(N + 1, 0);
+        for (size_t i = 0; i < N; i++) rp[i + 1] = rp[i] + cnt[i];
+        memcpy(row_ptr, rp.data(), (N + 1) * sizeof(int32_t));
+        if (col || val) {
+            std::vector<std::pair<int32_t, double>> row;
+            for (size_t s = 0; s < N; s++) {
+                const int32_t i = sid[s];
+                row.clear();
+                for (int k = ro[s]; k < ro[s + 1]; k++) row.emplace_back(sid[rc[k]], rv[k]);
+                std::sort(row.begin(), row.end(),
+                          [](const std::pair<int32_t, double>& a, const std::pair<int32_t, double>& b2) {
+                              return a.first < b2.first;
+                          });
+                for (size_t k = 0; k < row.size(); k++) {
+                    if (col) col[rp[i] + k] = row[k].first;
+                    if (val) val[rp[i] + k] = row[k].second;
+                }
+            }
+        }
+    }
+    return MPS_OK;
+}
+
+int mps_set_snapshot(mps_handle* h, int32_t on) {
+    if (!h) return MPS_E_INVALID;
+    const size_t N = (size_t) h->n;
+    if (on && !h->snap_pos) {
+        HIPCHK(h, dalloc(&h->snap_pos, 3 * N));
+        HIPCHK(h, dalloc(&h->snap_vel, 3 * N));
+    }
+    h->want_snap = on ? 1 : 0;
+    h->want_raw = on ? 1 : 0;
+    return MPS_OK;
+}
+
+int mps_get_snapshot(const mps_handle* hc, double* pos, double* vel) {
+    if (!hc) return MPS_E_INVALID;
+    mps_handle* h = const_cast<mps_handle*>(hc);
+    if (!h->snap_pos) return fail(h, MPS_E_INVALID, "snapshot not enabled");
+    const size_t N = (size_t) h->n;
+    if (pos) HIPCHK(h, hipMemcpy(pos, h->snap_pos, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+    if (vel) HIPCHK(h, hipMemcpy(vel, h->snap_vel, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+    return MPS_OK;
+}
+
+}  // extern "C"

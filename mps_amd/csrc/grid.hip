@@ -1,0 +1,300 @@
+// grid.hip — bucket grid as a GPU counting sort by cell key.
+//
+// Replaces Bucket::storeParticles (src/bucket.cpp:24-51).  The reference keeps
+// first/last per cell and next per particle, appending particles in id order,
+// so each cell's list ascends by id.  Here one build is:
+//   1. key + count   : cell key per slot (src/bucket.cpp:40-43), atomic count
+//   2. scan          : exclusive scan of counts -> cell starts cs[]
+//   3. bin           : ids binned into their cell's range (arbitrary order)
+//   4. rank + permute: each particle's rank within its cell = number of ids
+//                      in the cell smaller than its own; the hot arrays are
+//                      copied to their new slot.
+// The result is deterministic: cells ordered by key, ids ascending inside.
+// Contraction off: distances and weights must round as the reference's
+// scalar FP64 code does (no fused multiply-add).
+#pragma clang fp contract(off)
+
+#include "mps_internal.hpp"
+
+namespace mps {
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;  // 2048 elements per block
+constexpr int kScanTopThreads = 1024;
+
+// Exclusive scan within a block of kBlock threads; returns the exclusive prefix
+// of v and the block total in *total.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int* total) {
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int w = 0; w < nw; w++) {
+            int t = lds[w];
+            lds[w] = acc;
+            acc += t;
+        }
+        lds[nw] = acc;
+    }
+    __syncthreads();
+    int res = x - v + lds[wid];
+    *total = lds[nw];
+    __syncthreads();
+    return res;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_partials(const int* __restrict__ in, int m,
+                                                          int* __restrict__ part) {
+    __shared__ int lds[kBlock / 64 + 1];
+    const int base = blockIdx.x * kScanTile;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        int i = base + k * kBlock + threadIdx.x;
+        if (i < m) s += in[i];
+    }
+    int tot;
+    block_exclusive_scan(s, lds, &tot);
+    if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kScanTopThreads) k_scan_top(int* __restrict__ part, int nb) {
+    __shared__ int lds[kScanTopThreads / 64 + 1];
+    // each thread owns a contiguous run of ceil(nb / threads) partials
+    const int per = (nb + blockDim.x - 1) / blockDim.x;
+    const int b0 = threadIdx.x * per;
+    int s = 0;
+    for (int k = 0; k < per; k++)
+        if (b0 + k < nb) s += part[b0 + k];
+    int tot;
+    int ex = block_exclusive_scan(s, lds, &tot);
+    for (int k = 0; k < per; k++) {
+        if (b0 + k < nb) {
+            int t = part[b0 + k];
+            part[b0 + k] = ex;
+            ex += t;
+        }
+    }
+    if (threadIdx.x == 0) part[nb] = tot;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_final(const int* __restrict__ in, int m,
+                                                       const int* __restrict__ part,
+                                                       int* __restrict__ out, int nb) {
+    __shared__ int tile[kScanTile];
+    __shared__ int lds[kBlock / 64 + 1];
+    const int base = blockIdx.x * kScanTile;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        int i = base + k * kBlock + threadIdx.x;
+        tile[k * kBlock + threadIdx.x] = i < m ? in[i] : 0;
+    }
+    __syncthreads();
+    int loc[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        loc[k] = s;
+        s += tile[threadIdx.x * kScanItems + k];
+    }
+    int tot;
+    int ex = block_exclusive_scan(s, lds, &tot) + part[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) tile[threadIdx.x * kScanItems + k] = ex + loc[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        int i = base + k * kBlock + threadIdx.x;
+        if (i < m) out[i] = tile[k * kBlock + threadIdx.x];
+    }
+    if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[m] = part[nb];
+}
+
+size_t scan_tmp_ints(int m) { return (size_t) ((m + kScanTile - 1) / kScanTile) + 2; }
+
+// out[0..m] = exclusive prefix sums of in[0..m-1], out[m] = total.
+hipError_t scan_exclusive(const int* in, int* out, int m, int* tmp, hipStream_t st) {
+    if (m <= 0) return fill_int(out, 0, 1, st);
+    const int nb = (m + kScanTile - 1) / kScanTile;
+    if (nb > kScanTopThreads * 64) return hipErrorInvalidValue;  // > 134M elements
+    hipLaunchKernelGGL(k_scan_partials, dim3(nb), dim3(kBlock), 0, st, in, m, tmp);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanTopThreads), 0, st, tmp, nb);
+    hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(kBlock), 0, st, in, m, tmp, out, nb);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ void k_fill(T* __restrict__ p, T v, size_t n) {
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t) gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+static dim3 fill_grid(size_t n) {
+    size_t b = (n + kBlock - 1) / kBlock;
+    if (b > 65536) b = 65536;
+    if (b == 0) b = 1;
+    return dim3((unsigned) b);
+}
+
+hipError_t fill_int(int* p, int v, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill<int>, fill_grid(n), dim3(kBlock), 0, st, p, v, n);
+    return hipGetLastError();
+}
+hipError_t fill_double(double* p, double v, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill<double>, fill_grid(n), dim3(kBlock), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+// ---------------- build ----------------
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_key_count(GridView g, int n, Hot cur, int* __restrict__ cnt,
+                                                      int* __restrict__ err) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int key = g.ncell;
+    if (cur.type[s] != MPS_GHOST) {
+        int ix, iy, iz = 1;
+        bool ok = cell_coord(cur.px[s], g.xmin, g.len, g.nx, ix) &&
+                  cell_coord(cur.py[s], g.ymin, g.len, g.ny, iy);
+        if (DIM == 3) ok = ok && cell_coord(cur.pz[s], g.zmin, g.len, g.nz, iz);
+        if (ok && g.flat && iz != 1) ok = false;
+        if (ok) {
+            key = g.flat ? ix + iy * g.nx : ix + iy * g.nx + iz * g.nx * g.ny;
+        } else {
+            atomicOr(err, kErrRange);
+        }
+    }
+    cur.key[s] = key;
+    atomicAdd(&cnt[key], 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_bin(int n, Hot cur, int* __restrict__ cnt, const int* __restrict__ cs,
+                                                int* __restrict__ bin_id, int* __restrict__ dst) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int key = cur.key[s];
+    const int pos = atomicSub(&cnt[key], 1) - 1;  // leaves cnt at zero for the next build
+    const int d = cs[key] + pos;
+    bin_id[d] = cur.id[s];
+    dst[s] = d;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_rank_permute(GridView g, int n, Hot cur, Hot nxt,
+                                                         const int* __restrict__ bin_id,
+                                                         const int* __restrict__ dst,
+                                                         int* __restrict__ slot_of) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int key = cur.key[s];
+    const int myid = cur.id[s];
+    int t;
+    if (key < g.ncell) {
+        const int b = g.cs[key], e = g.cs[key + 1];
+        int r = 0;
+        for (int q = b; q < e; q++) r += bin_id[q] < myid;
+        t = b + r;
+    } else {
+        t = dst[s];  // ghosts: order inside the ghost segment is immaterial
+    }
+    nxt.px[t] = cur.px[s];
+    nxt.py[t] = cur.py[s];
+    nxt.vx[t] = cur.vx[s];
+    nxt.vy[t] = cur.vy[s];
+    if (DIM == 3) {
+        nxt.pz[t] = cur.pz[s];
+        nxt.vz[t] = cur.vz[s];
+    }
+    nxt.id[t] = myid;
+    nxt.type[t] = cur.type[s];
+    nxt.key[t] = key;
+    slot_of[myid] = t;
+}
+
+hipError_t build_grid(int dim, const GridView& g, int n, const Hot& cur, Hot& nxt, const BuildBufs& b,
+                      int* err, hipStream_t st) {
+    const dim3 grid((n + kBlock - 1) / kBlock), blk(kBlock);
+    if (n == 0) return scan_exclusive(b.cnt, b.cs, g.ncell + 1, b.scan_tmp, st);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_key_count<2>, grid, blk, 0, st, g, n, cur, b.cnt, err);
+    else
+        hipLaunchKernelGGL(k_key_count<3>, grid, blk, 0, st, g, n, cur, b.cnt, err);
+    hipError_t e = scan_exclusive(b.cnt, b.cs, g.ncell + 1, b.scan_tmp, st);
+    if (e != hipSuccess) return e;
+    int* dst = b.dst;
+    hipLaunchKernelGGL(k_bin, grid, blk, 0, st, n, cur, b.cnt, b.cs, b.bin_id, dst);
+    GridView gv = g;
+    gv.cs = b.cs;
+    if (dim == 2)
+        hipLaunchKernelGGL(k_rank_permute<2>, grid, blk, 0, st, gv, n, cur, nxt, b.bin_id, dst, b.slot_of);
+    else
+        hipLaunchKernelGGL(k_rank_permute<3>, grid, blk, 0, st, gv, n, cur, nxt, b.bin_id, dst, b.slot_of);
+    return hipGetLastError();
+}
+
+// ---------------- neighbour dumps (test entry points) ----------------
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_nb_count(GridView g, int n, Hot h, int* __restrict__ cnt_by_id,
+                                                     double re) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int c = 0;
+    if (h.type[s] != MPS_GHOST) {
+        const double zi = DIM == 3 ? h.pz[s] : 0.0;
+        for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
+                           [&](int, double, double, double, double) { c++; });
+    }
+    cnt_by_id[h.id[s]] = c;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_nb_list(GridView g, int n, Hot h, const int* __restrict__ off_by_id,
+                                                    int* __restrict__ ids, double* __restrict__ dists, double re) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    if (h.type[s] == MPS_GHOST) return;
+    int o = off_by_id[h.id[s]];
+    const double zi = DIM == 3 ? h.pz[s] : 0.0;
+    for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
+                       [&](int q, double dist, double, double, double) {
+                           ids[o] = h.id[q];
+                           dists[o] = dist;
+                           o++;
+                       });
+}
+
+hipError_t dump_neighbor_counts(int dim, const GridView& g, int n, const Hot& h, int* cnt_by_id, double re,
+                                hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid((n + kBlock - 1) / kBlock), blk(kBlock);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_nb_count<2>, grid, blk, 0, st, g, n, h, cnt_by_id, re);
+    else
+        hipLaunchKernelGGL(k_nb_count<3>, grid, blk, 0, st, g, n, h, cnt_by_id, re);
+    return hipGetLastError();
+}
+
+hipError_t dump_neighbor_lists(int dim, const GridView& g, int n, const Hot& h, const int* off_by_id, int* ids,
+                               double* dists, double re, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid((n + kBlock - 1) / kBlock), blk(kBlock);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_nb_list<2>, grid, blk, 0, st, g, n, h, off_by_id, ids, dists, re);
+    else
+        hipLaunchKernelGGL(k_nb_list<3>, grid, blk, 0, st, g, n, h, off_by_id, ids, dists, re);
+    return hipGetLastError();
+}
+
+}  // namespace mps

@@ -1,0 +1,274 @@
+// mps_internal.hpp — device state, grid view and shared device helpers of the
+// MI355X MPS step.  Internal to libmps_amd.so.
+//
+// Layout in HBM.  After every bucket build the hot particle arrays are
+// physically reordered by cell key ("slots"): positions, velocities, id and
+// type.  Cells are contiguous slot ranges [cs[k], cs[k+1]) and, within a
+// cell, slots ascend by particle id, which is the reference's in-cell order
+// (src/bucket.cpp:36-50 appends to the tail in id order).  Ghost particles sit
+// after the last cell.  Cold per-particle fields (pressure, minimum pressure,
+// number density ratio, boundary condition, warm-start solution, the stale
+// number density of ghosts) stay in id order and are reached through id[] /
+// slot_of[].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/mps_amd.h"
+
+namespace mps {
+
+constexpr int kBlock = 256;
+
+struct GridView {
+    double xmin, ymin, zmin;
+    double len;         // bucket side = re_max (src/bucket.cpp:11)
+    int nx, ny, nz;     // numX, numY, numZ (src/bucket.cpp:13-16)
+    int flat;           // 1: every in-domain particle has iz = 1 (numZ == 3), keys drop z
+    int ncell;          // keys in use: flat ? nx*ny : nx*ny*nz
+    const int* cs;      // ncell + 2 entries: cell starts, then ghost segment end
+};
+
+// Hot particle arrays in slot order (one of two ping-pong sets).
+struct Hot {
+    double* px;
+    double* py;
+    double* pz;
+    double* vx;
+    double* vy;
+    double* vz;
+    int32_t* id;
+    int8_t* type;
+    int32_t* key;   // cell key of the slot at the last build (ncell for ghosts)
+};
+
+struct Consts {
+    int dim;
+    double dt;
+    double l0;
+    double rho;
+    double nu;
+    double g[3];
+    double re_p, re_v, re_s, re_max;
+    double n0_p, lam_p, n0_v, lam_v, n0_s, lam_s;
+    double beta;
+    double relax;
+    int qc_on;
+    double alpha;
+    int hos_on;
+    double hos_gamma;
+    double dc;
+    double restitution;
+    double dmin[3], dmax[3];
+};
+
+// Device error flags (bit mask written with atomicOr).
+constexpr int kErrRange = 1;     // particle outside the bucket grid
+constexpr int kErrCollCap = 2;   // collision pair buffer overflow
+constexpr int kErrSellCap = 4;   // matrix storage overflow
+
+// Event buffers and counters on the device.
+struct Events {
+    int* n_ghost;        // appended count (may exceed cap)
+    int* ghost_ids;
+    int* n_unconn;
+    int* unconn_ids;
+    int cap;
+    int* err;
+};
+
+// ---------------- small device helpers ----------------
+
+// Cell coordinate as src/bucket.cpp:40-42 computes it: (int)((x-min)/len)+1.
+// Returns false if the value cannot index the grid.
+__device__ __forceinline__ bool cell_coord(double x, double xmin, double len, int n, int& ic) {
+    double q = (x - xmin) / len;
+    if (!(q > -1.0 && q < (double) (n - 2))) return false;  // NaN fails too
+    ic = (int) q + 1;
+    return ic >= 1 && ic <= n - 2;
+}
+
+__device__ __forceinline__ double wfun(double dist, double re) {
+    // src/mps.cpp:9-16
+    double w = 0.0;
+    if (dist < re) w = (re / dist) - 1.0;
+    return w;
+}
+
+// Visits the neighbours of slot s in the reference's order (src/mps.cpp:599-613):
+// jx outer, jy, jz inner, ascending id within a cell.  Calls f(q, dist, dx, dy, dz)
+// for every slot q != s with dist < re, dist = sqrt((dx*dx+dy*dy)+dz*dz) and
+// dx = x_q - x_s (Eigen's (pj.position - pi.position).norm()).
+// DIM == 2 runs on the flat key space with z identically zero.
+template <int DIM, typename F>
+__device__ __forceinline__ void for_neighbors(const GridView& g, const double* __restrict__ px,
+                                              const double* __restrict__ py,
+                                              const double* __restrict__ pz, int key, int s,
+                                              double xi, double yi, double zi, double re, F&& f) {
+    if constexpr (DIM == 2) {
+        const int ix = key % g.nx;
+        const int iy = key / g.nx;
+        for (int jx = ix - 1; jx <= ix + 1; jx++) {
+            if (jx < 0 || jx >= g.nx) continue;
+            for (int jy = iy - 1; jy <= iy + 1; jy++) {
+                if (jy < 0 || jy >= g.ny) continue;
+                const int k = jx + jy * g.nx;
+                const int e = g.cs[k + 1];
+                for (int q = g.cs[k]; q < e; q++) {
+                    const double dx = px[q] - xi;
+                    const double dy = py[q] - yi;
+                    const double dist = sqrt(dx * dx + dy * dy);
+                    if (q != s && dist < re) f(q, dist, dx, dy, 0.0);
+                }
+            }
+        }
+    } else {
+        int ix, iy, iz;
+        if (g.flat) {
+            ix = key % g.nx;
+            iy = key / g.nx;
+            iz = 1;
+        } else {
+            ix = key % g.nx;
+            iy = (key / g.nx) % g.ny;
+            iz = key / (g.nx * g.ny);
+        }
+        for (int jx = ix - 1; jx <= ix + 1; jx++) {
+            if (jx < 0 || jx >= g.nx) continue;
+            for (int jy = iy - 1; jy <= iy + 1; jy++) {
+                if (jy < 0 || jy >= g.ny) continue;
+                for (int jz = iz - 1; jz <= iz + 1; jz++) {
+                    if (g.flat) {
+                        if (jz != 1) continue;  // layers 0 and 2 are empty
+                    } else if (jz < 0 || jz >= g.nz) {
+                        continue;
+                    }
+                    const int k = g.flat ? (jx + jy * g.nx) : (jx + jy * g.nx + jz * g.nx * g.ny);
+                    const int e = g.cs[k + 1];
+                    for (int q = g.cs[k]; q < e; q++) {
+                        const double dx = px[q] - xi;
+                        const double dy = py[q] - yi;
+                        const double dz = pz[q] - zi;
+                        const double dist = sqrt((dx * dx + dy * dy) + dz * dz);
+                        if (q != s && dist < re) f(q, dist, dx, dy, dz);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------- launchers (defined in the .hip translation units) ----------------
+
+// grid.hip
+hipError_t scan_exclusive(const int* in, int* out, int m, int* tmp, hipStream_t st);
+size_t scan_tmp_ints(int m);
+hipError_t fill_int(int* p, int v, size_t n, hipStream_t st);
+hipError_t fill_double(double* p, double v, size_t n, hipStream_t st);
+
+struct BuildBufs {
+    int* cnt;        // ncell + 1 counters (kept at zero between builds)
+    int* cs;         // ncell + 2 cell starts
+    int* bin_id;     // n: ids binned per cell (arbitrary order within a cell)
+    int* dst;        // n: binned position of each slot
+    int* scan_tmp;
+    int* slot_of;    // n: id -> slot
+};
+hipError_t build_grid(int dim, const GridView& g, int n, const Hot& cur, Hot& nxt,
+                      const BuildBufs& b, int* err, hipStream_t st);
+hipError_t dump_neighbor_counts(int dim, const GridView& g, int n, const Hot& h, int* cnt_by_id,
+                                double re, hipStream_t st);
+hipError_t dump_neighbor_lists(int dim, const GridView& g, int n, const Hot& h, const int* off_by_id,
+                               int* ids, double* dists, double re, hipStream_t st);
+
+// stages.hip
+struct Cold {
+    double* p;        // pressure (id order)
+    double* pmin;     // minimum pressure (id order)
+    double* ndr;      // number density ratio (id order)
+    int8_t* bc;       // boundary condition (id order)
+    double* pprev;    // last unclamped solution (id order), warm start
+    double* ghost_n;  // number density (re_p) of the last list a ghost had
+    double* z;        // 2D: frozen z of every particle (id order)
+    double* vz;       // 2D: frozen vz
+};
+
+struct StepScratch {
+    int8_t* bc_s;       // slot order
+    double* b_s;        // source term
+    double* diag_s;     // matrix diagonal
+    int* nnz_s;         // off-diagonal count per slot (Inner rows)
+    int* parent;        // connectivity forest
+    int8_t* surf;       // component holds a Surface particle
+    int* inner_flag;    // 1 for Inner slots
+    int* row_of;        // slot -> compact row (exclusive scan of inner_flag)
+    double* p_s;        // pressure, slot order
+    double* pmin_s;
+    double* vmax;       // Courant reduction target (one double)
+    int* coll_cnt;      // per id
+    int* coll_off;      // per id + 1
+    int* coll_pairs;    // 2 * cap
+    double* coll_dist;  // cap
+    int* coll_level;    // cap
+    int* coll_last;     // n, zero between steps
+    int coll_cap;
+    int* scan_tmp;
+};
+
+hipError_t launch_visc_move(int dim, const GridView& g, int n, const Consts& c, const Hot& cur,
+                            Hot& out, Cold& cold, Events& ev, hipStream_t st);
+hipError_t launch_collision(int dim, const GridView& g, int n, const Consts& c, Hot& h, StepScratch& w,
+                            int* err, hipStream_t st);
+hipError_t launch_bc_source(int dim, const GridView& g, int n, const Consts& c, const Hot& h, Cold& cold,
+                            StepScratch& w, hipStream_t st);
+hipError_t launch_diag_cc(int dim, const GridView& g, int n, const Consts& c, const Hot& h, StepScratch& w,
+                          Events& ev, hipStream_t st);
+hipError_t launch_pressure_out(int dim, const GridView& g, int n, const Consts& c, const Hot& h,
+                               const double* x_s, Cold& cold, StepScratch& w, hipStream_t st);
+hipError_t launch_grad_move(int dim, const GridView& g, int n, const Consts& c, const Hot& cur, Hot& out,
+                            Cold& cold, StepScratch& w, Events& ev, hipStream_t st);
+hipError_t launch_ndr(int dim, const GridView& g, int n, const Consts& c, const Hot& h, Cold& cold,
+                      hipStream_t st);
+hipError_t launch_courant(int dim, int n, const Hot& h, double* vmax, hipStream_t st);
+hipError_t launch_snapshot(int dim, int n, const Hot& h, const Cold& cold, double* pos_id, double* vel_id,
+                           hipStream_t st);
+
+// ppe.hip — matrix storage and CG
+struct Sell {
+    int rows;            // Inner rows
+    const int* rows_dev; // Inner rows, on the device
+    int nslices;
+    int* slice_off;      // nslices + 1 (in entries)
+    int* slice_w;        // nslices (max nnz of the slice)
+    int* col;            // entries
+    double* val64;       // scaled off-diagonal values (FP64)
+    float* val32;        // scaled off-diagonal values (FP32), mixed path
+    int* row_slot;       // rows -> slot
+    double* sdiag;       // 1/sqrt(diag) per row
+    double* dgl;         // diag per row
+    size_t cap;          // entry capacity
+    // debug CSR (raw values by slot), filled when capture is on
+    int* raw_off;        // n + 1 (by slot)
+    int* raw_col;        // slots
+    double* raw_val;
+    size_t raw_cap;
+};
+
+hipError_t assemble_sell(int dim, const GridView& g, int n, const Consts& c, const Hot& h,
+                         const StepScratch& w, Sell& A, int want_val32, int want_raw, int* err,
+                         int* host_rows, size_t* host_entries, hipStream_t st);
+
+struct CgWork;  // defined in ppe.hip
+CgWork* cg_create();
+void cg_destroy(CgWork* cw);
+hipError_t cg_reserve(CgWork* cw, int rows);
+// Solves A x = b for the Inner rows.  b_s / x_s in slot order; x0_id (id order,
+// may be NULL) gives the warm start.  Fills stats.
+hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double* b_s, const double* pprev_id,
+                    double* x_s, const mps_solver_options& o, mps_solver_stats& stats, hipStream_t st,
+                    std::string& msg);
+
+}  // namespace mps

@@ -1,0 +1,669 @@
+// stages.hip — the explicit and pressure-assembly stages of the MPS step on
+// the cell-sorted layout, FP64, in the reference's expression order.
+//
+// Floating-point contraction is off in this file: every sum, product and
+// quotient rounds as the reference's scalar C++ does (x86-64 SSE2, no FMA), so
+// the stages up to the pressure solve reproduce the reference bit for bit.
+#pragma clang fp contract(off)
+
+#include "mps_internal.hpp"
+
+namespace mps {
+
+static inline dim3 grid_for(int n) { return dim3((unsigned) ((n + kBlock - 1) / kBlock)); }
+
+__device__ __forceinline__ void push_event(int* counter, int* ids, int cap, int id) {
+    int k = atomicAdd(counter, 1);
+    if (k < cap) ids[k] = id;
+}
+
+// src/mps.cpp:643-667 (strict comparisons on every axis)
+__device__ __forceinline__ bool in_domain(const Consts& c, double x, double y, double z) {
+    return !(x < c.dmin[0] || c.dmax[0] < x || y < c.dmin[1] || c.dmax[1] < y || z < c.dmin[2] ||
+             c.dmax[2] < z);
+}
+
+// Number density over the current list within re_p, kept for a particle that
+// turns Ghost: the reference leaves a Ghost's list stale (src/mps.cpp:589-593)
+// and setNumberDensity later reads it (src/mps.cpp:630-640).
+template <int DIM>
+__device__ __forceinline__ double ndens(const GridView& g, const Hot& h, int s, double xi, double yi, double zi,
+                                        double re) {
+    double nd = 0.0;
+    for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, xi, yi, zi, re,
+                       [&](int, double dist, double, double, double) { nd += wfun(dist, re); });
+    return nd;
+}
+
+// ---------------- gravity + viscosity + explicit move (src/mps.cpp:165-215) ----------------
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_visc_move(GridView g, int n, Consts c, Hot cur, Hot out, Cold cold,
+                                                      Events ev, double avisc) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    double x = cur.px[s], y = cur.py[s], z = DIM == 3 ? cur.pz[s] : 0.0;
+    double vx = cur.vx[s], vy = cur.vy[s], vz = DIM == 3 ? cur.vz[s] : 0.0;
+    if (cur.type[s] == MPS_FLUID) {
+        double ax = c.g[0], ay = c.g[1], az = c.g[2];
+        double tx = 0.0, ty = 0.0, tz = 0.0;
+        const double re = c.re_v;
+        for_neighbors<DIM>(g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, re,
+                           [&](int q, double dist, double, double, double) {
+                               const double w = wfun(dist, re);
+                               tx += (cur.vx[q] - vx) * w;
+                               ty += (cur.vy[q] - vy) * w;
+                               if (DIM == 3) tz += (cur.vz[q] - vz) * w;
+                           });
+        tx *= avisc;
+        ty *= avisc;
+        tz *= avisc;
+        ax += tx;
+        ay += ty;
+        az += tz;
+        vx += ax * c.dt;
+        vy += ay * c.dt;
+        if (DIM == 3) vz += az * c.dt;
+        const double x0 = x, y0 = y, z0 = z;
+        x += vx * c.dt;
+        y += vy * c.dt;
+        if (DIM == 3) z += vz * c.dt;
+        if (!in_domain(c, x, y, DIM == 3 ? z : c.dmin[2])) {
+            cur.type[s] = MPS_GHOST;
+            const int id = cur.id[s];
+            push_event(ev.n_ghost, ev.ghost_ids, ev.cap, id);
+            cold.ghost_n[id] = ndens<DIM>(g, cur, s, x0, y0, z0, c.re_p);
+        }
+    }
+    out.px[s] = x;
+    out.py[s] = y;
+    out.vx[s] = vx;
+    out.vy[s] = vy;
+    if (DIM == 3) {
+        out.pz[s] = z;
+        out.vz[s] = vz;
+    }
+}
+
+hipError_t launch_visc_move(int dim, const GridView& g, int n, const Consts& c, const Hot& cur, Hot& out,
+                            Cold& cold, Events& ev, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const double avisc = (c.nu) * (2.0 * (double) c.dim) / (c.n0_v * c.lam_v);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_visc_move<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, cur, out, cold, ev, avisc);
+    else
+        hipLaunchKernelGGL(k_visc_move<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, cur, out, cold, ev, avisc);
+    return hipGetLastError();
+}
+
+// ---------------- collision (src/mps.cpp:217-250) ----------------
+//
+// The reference visits fluid particles in ascending id and, for each, its list
+// in order; a pair is skipped when j is fluid with j.id >= i.id, and acts when
+// the list distance (fixed at the build) is below the collision distance.
+// Candidate pairs are therefore fixed before any update.  They are listed in
+// the reference's order (per-id counts, scan, emit), then each pair gets a
+// level = 1 + the highest level of earlier pairs sharing a fluid particle;
+// walls never change, so they carry no dependency.  Pairs of one level touch
+// disjoint fluid particles and are resolved together; levels run in order.
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_coll_count(GridView g, int n, Consts c, Hot h, int* __restrict__ cnt) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int id = h.id[s];
+    int k = 0;
+    if (h.type[s] == MPS_FLUID) {
+        const double re = c.dc < c.re_max ? c.dc : c.re_max;
+        const double zi = DIM == 3 ? h.pz[s] : 0.0;
+        for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
+                           [&](int q, double, double, double, double) {
+                               if (!(h.type[q] == MPS_FLUID && h.id[q] >= id)) k++;
+                           });
+    }
+    cnt[id] = k;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_coll_emit(GridView g, int n, Consts c, Hot h, const int* __restrict__ cnt,
+                                                      const int* __restrict__ off, int* __restrict__ pairs,
+                                                      double* __restrict__ pdist, int cap, int* __restrict__ err) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int id = h.id[s];
+    if (cnt[id] == 0) return;
+    int o = off[id];
+    const double re = c.dc < c.re_max ? c.dc : c.re_max;
+    const double zi = DIM == 3 ? h.pz[s] : 0.0;
+    for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
+                       [&](int q, double dist, double, double, double) {
+                           if (h.type[q] == MPS_FLUID && h.id[q] >= id) return;
+                           if (o < cap) {
+                               pairs[2 * o] = s;
+                               pairs[2 * o + 1] = q;
+                               pdist[o] = dist;
+                           } else {
+                               atomicOr(err, kErrCollCap);
+                           }
+                           o++;
+                       });
+}
+
+constexpr int kResolveThreads = 1024;
+
+__global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts c, Hot h, const int* __restrict__ off,
+                                                                  const int* __restrict__ pairs,
+                                                                  const double* __restrict__ pdist,
+                                                                  int* __restrict__ level, int* __restrict__ last,
+                                                                  int cap) {
+    __shared__ int s_max;
+    int np = off[n];
+    if (np > cap) np = cap;
+    if (np == 0) return;
+    if (threadIdx.x == 0) {
+        int mx = 0;
+        for (int p = 0; p < np; p++) {
+            const int i = pairs[2 * p], j = pairs[2 * p + 1];
+            const bool jf = h.type[j] == MPS_FLUID;
+            int L = last[i];
+            if (jf && last[j] > L) L = last[j];
+            L += 1;
+            level[p] = L;
+            last[i] = L;
+            if (jf) last[j] = L;
+            if (L > mx) mx = L;
+        }
+        s_max = mx;
+    }
+    __syncthreads();
+    volatile double* px = h.px;
+    volatile double* py = h.py;
+    volatile double* pz = h.pz;
+    volatile double* vx = h.vx;
+    volatile double* vy = h.vy;
+    volatile double* vz = h.vz;
+    const bool d3 = c.dim == 3;
+    const double e = c.restitution, dc = c.dc;
+    for (int L = 1; L <= s_max; L++) {
+        for (int p = threadIdx.x; p < np; p += blockDim.x) {
+            if (level[p] != L) continue;
+            const int i = pairs[2 * p], j = pairs[2 * p + 1];
+            const double dist = pdist[p];
+            const bool jf = h.type[j] == MPS_FLUID;
+            // src/particle.cpp:19-29: 1/density for fluid, 0 otherwise
+            const double invMi = 1 / c.rho;
+            const double invMj = jf ? 1 / c.rho : 0;
+            const double reduced = 1.0 / (invMi + invMj);
+            double nx = px[j] - px[i], ny = py[j] - py[i], nz = d3 ? pz[j] - pz[i] : 0.0;
+            const double zz = (nx * nx + ny * ny) + nz * nz;
+            if (zz > 0.0) {  // Eigen normalized()
+                const double sq = sqrt(zz);
+                nx = nx / sq;
+                ny = ny / sq;
+                nz = nz / sq;
+            }
+            const double dvx = vx[j] - vx[i], dvy = vy[j] - vy[i], dvz = d3 ? vz[j] - vz[i] : 0.0;
+            const double rv = (dvx * nx + dvy * ny) + dvz * nz;
+            if (rv < 0.0) {
+                const double impulse = -(1.0 + e) * rv * reduced;
+                const double si = impulse * invMi, sj = impulse * invMj;
+                vx[i] = vx[i] - si * nx;
+                vy[i] = vy[i] - si * ny;
+                if (d3) vz[i] = vz[i] - si * nz;
+                const double depth = dc - dist;
+                const double pimp = depth * reduced;
+                const double pi_ = pimp * invMi, pj_ = pimp * invMj;
+                px[i] = px[i] - pi_ * nx;
+                py[i] = py[i] - pi_ * ny;
+                if (d3) pz[i] = pz[i] - pi_ * nz;
+                if (jf) {  // a wall's update adds zero: skipped
+                    vx[j] = vx[j] + sj * nx;
+                    vy[j] = vy[j] + sj * ny;
+                    if (d3) vz[j] = vz[j] + sj * nz;
+                    px[j] = px[j] + pj_ * nx;
+                    py[j] = py[j] + pj_ * ny;
+                    if (d3) pz[j] = pz[j] + pj_ * nz;
+                }
+            }
+        }
+        __threadfence();
+        __syncthreads();
+    }
+    for (int p = threadIdx.x; p < np; p += blockDim.x) {
+        last[pairs[2 * p]] = 0;
+        last[pairs[2 * p + 1]] = 0;
+    }
+}
+
+hipError_t launch_collision(int dim, const GridView& g, int n, const Consts& c, Hot& h, StepScratch& w, int* err,
+                            hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    if (dim == 2)
+        hipLaunchKernelGGL(k_coll_count<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, w.coll_cnt);
+    else
+        hipLaunchKernelGGL(k_coll_count<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, w.coll_cnt);
+    hipError_t e = scan_exclusive(w.coll_cnt, w.coll_off, n, w.scan_tmp, st);
+    if (e != hipSuccess) return e;
+    if (dim == 2)
+        hipLaunchKernelGGL(k_coll_emit<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, w.coll_cnt, w.coll_off,
+                           w.coll_pairs, w.coll_dist, w.coll_cap, err);
+    else
+        hipLaunchKernelGGL(k_coll_emit<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, w.coll_cnt, w.coll_off,
+                           w.coll_pairs, w.coll_dist, w.coll_cap, err);
+    hipLaunchKernelGGL(k_coll_resolve, dim3(1), dim3(kResolveThreads), 0, st, n, c, h, w.coll_off, w.coll_pairs,
+                       w.coll_dist, w.coll_level, w.coll_last, w.coll_cap);
+    return hipGetLastError();
+}
+
+// ---------------- boundary condition + source term (src/mps.cpp:262-320) ----------------
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_bc_source(GridView g, int n, Consts c, Hot h, Cold cold,
+                                                      int8_t* __restrict__ bc_s, double* __restrict__ b_s,
+                                                      double src_scale) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int t = h.type[s];
+    int8_t bc;
+    double b = 0.0;
+    if (t == MPS_GHOST || t == MPS_DUMMY_WALL) {
+        bc = MPS_BC_IGNORED;
+    } else {
+        const double rs = c.re_s, rp = c.re_p;
+        const double re = rs > rp ? rs : rp;
+        double ns = 0.0, np = 0.0;
+        const double zi = DIM == 3 ? h.pz[s] : 0.0;
+        for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
+                           [&](int, double dist, double, double, double) {
+                               if (dist < rs) ns += wfun(dist, rs);
+                               if (dist < rp) np += wfun(dist, rp);
+                           });
+        bc = ns < c.beta * c.n0_s ? MPS_BC_SURFACE : MPS_BC_INNER;
+        if (bc == MPS_BC_INNER && !c.hos_on) b = src_scale * (np - c.n0_p) / c.n0_p;
+    }
+    bc_s[s] = bc;
+    b_s[s] = b;
+    cold.bc[h.id[s]] = bc;
+}
+
+// Higher-order source term (src/mps.cpp:288-308); needs every slot's BC.
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_source_hos(GridView g, int n, Consts c, Hot h, Cold cold,
+                                                       const int8_t* __restrict__ bc_s, double* __restrict__ b_s,
+                                                       double hos_scale, double hos_coef) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    if (bc_s[s] != MPS_BC_INNER) return;
+    double b = hos_scale * (cold.ndr[h.id[s]] - 1.0);
+    const double re = c.re_p;
+    const double xi = h.px[s], yi = h.py[s], zi = DIM == 3 ? h.pz[s] : 0.0;
+    const double vxi = h.vx[s], vyi = h.vy[s], vzi = DIM == 3 ? h.vz[s] : 0.0;
+    for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, xi, yi, zi, c.re_max,
+                       [&](int q, double dist, double rx, double ry, double rz) {
+                           if (bc_s[q] == MPS_BC_IGNORED) return;
+                           const double ux = h.vx[q] - vxi, uy = h.vy[q] - vyi;
+                           const double uz = DIM == 3 ? h.vz[q] - vzi : 0.0;
+                           if (dist < re) b -= hos_coef * re * ((rx * ux + ry * uy) + rz * uz) / (dist * dist * dist);
+                       });
+    b_s[s] = b;
+}
+
+hipError_t launch_bc_source(int dim, const GridView& g, int n, const Consts& c, const Hot& h, Cold& cold,
+                            StepScratch& w, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const double dt = c.dt;
+    const double src_scale = c.relax * (1.0 / (dt * dt));
+    if (dim == 2)
+        hipLaunchKernelGGL(k_bc_source<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
+                           src_scale);
+    else
+        hipLaunchKernelGGL(k_bc_source<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
+                           src_scale);
+    if (c.hos_on) {
+        const double hos_scale = c.hos_gamma * (1.0 / (dt * dt));
+        const double hos_coef = (1.0 / (c.n0_p * dt));
+        if (dim == 2)
+            hipLaunchKernelGGL(k_source_hos<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
+                               hos_scale, hos_coef);
+        else
+            hipLaunchKernelGGL(k_source_hos<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
+                               hos_scale, hos_coef);
+    }
+    return hipGetLastError();
+}
+
+// ---------------- matrix diagonal + Dirichlet connectivity (src/mps.cpp:322-415) ----------------
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_diag(GridView g, int n, Consts c, Hot h, const int8_t* __restrict__ bc_s,
+                                                 double* __restrict__ diag, int* __restrict__ nnz, double coefA,
+                                                 double qc_add) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    double d = 0.0;
+    int k = 0;
+    if (bc_s[s] == MPS_BC_INNER) {
+        const double re = c.re_p;
+        const double zi = DIM == 3 ? h.pz[s] : 0.0;
+        for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
+                           [&](int q, double dist, double, double, double) {
+                               const int8_t bq = bc_s[q];
+                               if (bq == MPS_BC_IGNORED) return;
+                               const double a = coefA * wfun(dist, re) / c.rho;
+                               d += a;
+                               if (bq == MPS_BC_INNER) k++;
+                           });
+        if (c.qc_on) d += qc_add;
+    }
+    diag[s] = d;
+    nnz[s] = k;
+}
+
+// Connectivity: the reference runs a multi-source BFS from every Surface
+// particle over edges dist < re_p through any particle (src/mps.cpp:373-400).
+// The visited set is the union of connected components that hold a Surface
+// particle, found here with lock-free union-find (hook the larger root under
+// the smaller with atomicCAS, path halving on find).
+__device__ __forceinline__ int cc_find(volatile int* parent, int v) {
+    int curr = parent[v];
+    if (curr != v) {
+        int prev = v, next;
+        while (curr > (next = parent[curr])) {
+            parent[prev] = next;
+            prev = curr;
+            curr = next;
+        }
+    }
+    return curr;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_cc_init(GridView g, int n, Consts c, Hot h, int* __restrict__ parent) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int m = s;
+    if (h.type[s] != MPS_GHOST) {
+        const double zi = DIM == 3 ? h.pz[s] : 0.0;
+        for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, c.re_p,
+                           [&](int q, double, double, double, double) {
+                               if (q < m) m = q;
+                           });
+    }
+    parent[s] = m;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_cc_join(GridView g, int n, Consts c, Hot h, int* parent) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    if (h.type[s] == MPS_GHOST) return;
+    volatile int* vp = parent;
+    const double zi = DIM == 3 ? h.pz[s] : 0.0;
+    for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, c.re_p,
+                       [&](int q, double, double, double, double) {
+                           if (q <= s) return;
+                           int a = cc_find(vp, s);
+                           int b = cc_find(vp, q);
+                           while (a != b) {
+                               if (a < b) {
+                                   const int r = atomicCAS(&parent[b], b, a);
+                                   if (r == b) break;
+                                   b = r;
+                               } else {
+                                   const int r = atomicCAS(&parent[a], a, b);
+                                   if (r == a) break;
+                                   a = r;
+                               }
+                           }
+                       });
+}
+
+__global__ void __launch_bounds__(kBlock) k_cc_finalize(int n, int* parent, const int8_t* __restrict__ bc_s,
+                                                        int8_t* __restrict__ surf) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    volatile int* vp = parent;
+    int r = vp[s];
+    while (r != vp[r]) r = vp[r];
+    parent[s] = r;
+    if (bc_s[s] == MPS_BC_SURFACE) surf[r] = 1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cc_apply(int n, Hot h, const int* __restrict__ parent,
+                                                     const int8_t* __restrict__ bc_s, const int8_t* __restrict__ surf,
+                                                     double* __restrict__ diag, int* __restrict__ inner_flag,
+                                                     Events ev) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const bool inner = bc_s[s] == MPS_BC_INNER;
+    inner_flag[s] = inner ? 1 : 0;
+    if (inner && !surf[parent[s]]) {
+        diag[s] *= 2.0;
+        push_event(ev.n_unconn, ev.unconn_ids, ev.cap, h.id[s]);
+    }
+}
+
+hipError_t launch_diag_cc(int dim, const GridView& g, int n, const Consts& c, const Hot& h, StepScratch& w,
+                          Events& ev, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const double D = (double) c.dim;
+    const double coefA = (2.0 * D / (c.lam_p * c.n0_p));
+    const double qc_add = c.alpha / (c.dt * c.dt);
+    const dim3 gr = grid_for(n), bl(kBlock);
+    if (dim == 2) {
+        hipLaunchKernelGGL(k_diag<2>, gr, bl, 0, st, g, n, c, h, w.bc_s, w.diag_s, w.nnz_s, coefA, qc_add);
+        hipLaunchKernelGGL(k_cc_init<2>, gr, bl, 0, st, g, n, c, h, w.parent);
+        hipLaunchKernelGGL(k_cc_join<2>, gr, bl, 0, st, g, n, c, h, w.parent);
+    } else {
+        hipLaunchKernelGGL(k_diag<3>, gr, bl, 0, st, g, n, c, h, w.bc_s, w.diag_s, w.nnz_s, coefA, qc_add);
+        hipLaunchKernelGGL(k_cc_init<3>, gr, bl, 0, st, g, n, c, h, w.parent);
+        hipLaunchKernelGGL(k_cc_join<3>, gr, bl, 0, st, g, n, c, h, w.parent);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // surf[] is a byte array: clear it with the int fill over its rounded-up size
+    e = fill_int(reinterpret_cast<int*>(w.surf), 0, ((size_t) n + 3) / 4, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cc_finalize, gr, bl, 0, st, n, w.parent, w.bc_s, w.surf);
+    hipLaunchKernelGGL(k_cc_apply, gr, bl, 0, st, n, h, w.parent, w.bc_s, w.surf, w.diag_s, w.inner_flag, ev);
+    return hipGetLastError();
+}
+
+// ---------------- pressure out, minimum pressure (src/mps.cpp:466-503) ----------------
+
+__global__ void __launch_bounds__(kBlock) k_pressure_out(int n, Hot h, const int8_t* __restrict__ bc_s,
+                                                         const double* __restrict__ x_s, Cold cold,
+                                                         double* __restrict__ p_s) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int id = h.id[s];
+    const double x = bc_s[s] == MPS_BC_INNER ? x_s[s] : 0.0;  // empty rows solve to 0
+    cold.pprev[id] = x;
+    const double p = x < 0.0 ? 0.0 : x;  // removeNegativePressure
+    p_s[s] = p;
+    cold.p[id] = p;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_min_pressure(GridView g, int n, Consts c, Hot h,
+                                                         const int8_t* __restrict__ bc_s,
+                                                         const double* __restrict__ p_s, Cold cold,
+                                                         double* __restrict__ pmin_s) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int id = h.id[s];
+    if (bc_s[s] == MPS_BC_IGNORED) {
+        pmin_s[s] = cold.pmin[id];
+        return;
+    }
+    double pm = p_s[s];
+    const double zi = DIM == 3 ? h.pz[s] : 0.0;
+    for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, c.re_p,
+                       [&](int q, double, double, double, double) {
+                           if (h.type[q] == MPS_DUMMY_WALL) return;
+                           if (pm > p_s[q]) pm = p_s[q];
+                       });
+    pmin_s[s] = pm;
+    cold.pmin[id] = pm;
+}
+
+hipError_t launch_pressure_out(int dim, const GridView& g, int n, const Consts& c, const Hot& h, const double* x_s,
+                               Cold& cold, StepScratch& w, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pressure_out, grid_for(n), dim3(kBlock), 0, st, n, h, w.bc_s, x_s, cold, w.p_s);
+    if (dim == 2)
+        hipLaunchKernelGGL(k_min_pressure<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, w.bc_s, w.p_s, cold,
+                           w.pmin_s);
+    else
+        hipLaunchKernelGGL(k_min_pressure<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, w.bc_s, w.p_s, cold,
+                           w.pmin_s);
+    return hipGetLastError();
+}
+
+// ---------------- pressure gradient + implicit move (src/mps.cpp:505-545) ----------------
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_grad_move(GridView g, int n, Consts c, Hot cur, Hot out, Cold cold,
+                                                      const double* __restrict__ p_s,
+                                                      const double* __restrict__ pmin_s, Events ev, double gscale) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    double x = cur.px[s], y = cur.py[s], z = DIM == 3 ? cur.pz[s] : 0.0;
+    double vx = cur.vx[s], vy = cur.vy[s], vz = DIM == 3 ? cur.vz[s] : 0.0;
+    if (cur.type[s] == MPS_FLUID) {
+        const double re = c.re_p;
+        const double pmi = pmin_s[s];
+        double gx = 0.0, gy = 0.0, gz = 0.0;
+        for_neighbors<DIM>(g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, re,
+                           [&](int q, double dist, double dx, double dy, double dz) {
+                               if (cur.type[q] == MPS_DUMMY_WALL) return;
+                               const double dp = p_s[q] - pmi;
+                               const double w = wfun(dist, re);
+                               const double d2 = dist * dist;
+                               gx += dx * dp * w / d2;
+                               gy += dy * dp * w / d2;
+                               if (DIM == 3) gz += dz * dp * w / d2;
+                           });
+        gx *= gscale;
+        gy *= gscale;
+        gz *= gscale;
+        const double ax = -1.0 * gx / c.rho, ay = -1.0 * gy / c.rho, az = -1.0 * gz / c.rho;
+        vx += ax * c.dt;
+        vy += ay * c.dt;
+        if (DIM == 3) vz += az * c.dt;
+        const double x0 = x, y0 = y, z0 = z;
+        x += ax * c.dt * c.dt;
+        y += ay * c.dt * c.dt;
+        if (DIM == 3) z += az * c.dt * c.dt;
+        if (!in_domain(c, x, y, DIM == 3 ? z : c.dmin[2])) {
+            cur.type[s] = MPS_GHOST;
+            const int id = cur.id[s];
+            push_event(ev.n_ghost, ev.ghost_ids, ev.cap, id);
+            cold.ghost_n[id] = ndens<DIM>(g, cur, s, x0, y0, z0, c.re_p);
+        }
+    }
+    out.px[s] = x;
+    out.py[s] = y;
+    out.vx[s] = vx;
+    out.vy[s] = vy;
+    if (DIM == 3) {
+        out.pz[s] = z;
+        out.vz[s] = vz;
+    }
+}
+
+hipError_t launch_grad_move(int dim, const GridView& g, int n, const Consts& c, const Hot& cur, Hot& out,
+                            Cold& cold, StepScratch& w, Events& ev, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    const double gscale = (double) c.dim / c.n0_p;
+    if (dim == 2)
+        hipLaunchKernelGGL(k_grad_move<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, cur, out, cold, w.p_s, w.pmin_s,
+                           ev, gscale);
+    else
+        hipLaunchKernelGGL(k_grad_move<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, cur, out, cold, w.p_s, w.pmin_s,
+                           ev, gscale);
+    return hipGetLastError();
+}
+
+// ---------------- number density ratio (src/mps.cpp:627-641) ----------------
+
+template <int DIM>
+__global__ void __launch_bounds__(kBlock) k_ndr(GridView g, int n, Consts c, Hot h, Cold cold) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int id = h.id[s];
+    double nd;
+    if (h.type[s] != MPS_GHOST) {
+        const double zi = DIM == 3 ? h.pz[s] : 0.0;
+        nd = ndens<DIM>(g, h, s, h.px[s], h.py[s], zi, c.re_p);
+    } else {
+        nd = cold.ghost_n[id];
+    }
+    cold.ndr[id] = cold.bc[id] == MPS_BC_SURFACE ? 0.0 : nd / c.n0_p;
+}
+
+hipError_t launch_ndr(int dim, const GridView& g, int n, const Consts& c, const Hot& h, Cold& cold, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    if (dim == 2)
+        hipLaunchKernelGGL(k_ndr<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, cold);
+    else
+        hipLaunchKernelGGL(k_ndr<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, cold);
+    return hipGetLastError();
+}
+
+// ---------------- Courant number (src/mps.cpp:547-571) ----------------
+// max |v| over fluid; (|v| dt)/l0 is monotone in |v|, so the host applies it
+// to the maximum and gets the reference's value bit for bit.
+
+__global__ void __launch_bounds__(kBlock) k_vmax(int n, Hot h, int dim3d, unsigned long long* __restrict__ out) {
+    __shared__ double red[kBlock / 64];
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (s < n && h.type[s] == MPS_FLUID) {
+        const double a = h.vx[s], b = h.vy[s], cz = dim3d ? h.vz[s] : 0.0;
+        v = sqrt((a * a + b * b) + cz * cz);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double u = __shfl_down(v, o, 64);
+        v = u > v ? u : v;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int k = 1; k < kBlock / 64; k++) m = red[k] > m ? red[k] : m;
+        atomicMax(out, (unsigned long long) __double_as_longlong(m));
+    }
+}
+
+hipError_t launch_courant(int dim, int n, const Hot& h, double* vmax, hipStream_t st) {
+    hipError_t e = fill_double(vmax, 0.0, 1, st);
+    if (e != hipSuccess || n == 0) return e;
+    hipLaunchKernelGGL(k_vmax, grid_for(n), dim3(kBlock), 0, st, n, h, dim == 3 ? 1 : 0,
+                       reinterpret_cast<unsigned long long*>(vmax));
+    return hipGetLastError();
+}
+
+// ---------------- id-order snapshots ----------------
+
+__global__ void __launch_bounds__(kBlock) k_gather_id(int n, int dim, Hot h, Cold cold, double* __restrict__ pos,
+                                                      double* __restrict__ vel) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int id = h.id[s];
+    pos[3 * (size_t) id] = h.px[s];
+    pos[3 * (size_t) id + 1] = h.py[s];
+    pos[3 * (size_t) id + 2] = dim == 3 ? h.pz[s] : cold.z[id];
+    vel[3 * (size_t) id] = h.vx[s];
+    vel[3 * (size_t) id + 1] = h.vy[s];
+    vel[3 * (size_t) id + 2] = dim == 3 ? h.vz[s] : cold.vz[id];
+}
+
+hipError_t launch_snapshot(int dim, int n, const Hot& h, const Cold& cold, double* pos_id, double* vel_id,
+                           hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_id, grid_for(n), dim3(kBlock), 0, st, n, dim, h, cold, pos_id, vel_id);
+    return hipGetLastError();
+}
+
+}  // namespace mps

@@ -1,0 +1,241 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 CPU oracle.
+
+Bit-exact: bucket linked lists, neighbour ids and distances, boundary
+conditions, the source term, the matrix (diagonal and off-diagonals) and the
+state entering the pressure stage.  Within stated FP64 tolerances: the
+pressure solution (two different CG summation orders, both run to near
+machine precision) and the state after the step.
+"""
+import numpy as np
+import pytest
+
+from mps_amd import capi, configs
+from oracle_lib import Oracle
+
+pytestmark = pytest.mark.gpu
+
+# tolerance of the GPU CG in parity tests: ||b-Ax|| <= 1e-14 ||b||; the oracle
+# runs Eigen's rule (machine epsilon on the recursive residual)
+GPU_TOL = 1e-14
+# pressure parity: relative 2-norm error of the solution
+P_RTOL = 1e-9
+
+
+def _jitter(settings, scene, seed, amp=0.2, vamp=0.05):
+    rng = np.random.default_rng(seed)
+    f = scene.type == configs.FLUID
+    d = settings.dim if hasattr(settings, "dim") else 2
+    scene.pos[f, :d] += rng.uniform(-amp, amp, size=(int(f.sum()), d)) * settings.particle_distance
+    scene.vel[f, :d] = rng.normal(0, vamp, size=(int(f.sum()), d))
+    return scene
+
+
+def _pair(settings, scene, tol=GPU_TOL):
+    o = Oracle(settings, scene)
+    g = capi.Simulation(settings, scene)
+    g.set_solver(precision=capi.MPS_PREC_FP64, tolerance=tol, warm_start=False)
+    return o, g
+
+
+def _cases():
+    out = []
+    s = configs.Settings()  # tutorials/dambreak/settings.yml
+    out.append(("dambreak_tutorial_jitter", s, _jitter(s, configs.tutorial_dambreak(), 1)))
+    s2, sc2, _ = configs.workload("dambreak2d_584")
+    out.append(("dambreak2d_584_jitter", s2, _jitter(s2, sc2, 2)))
+    return out
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("name,s,sc", CASES, ids=[c[0] for c in CASES])
+def test_bucket_bit_exact(name, s, sc):
+    o, g = _pair(s, sc)
+    o.set_neighbors()
+    ob = o.bucket()
+    gb = g.bucket()
+    np.testing.assert_array_equal(gb["dims"], ob["dims"])
+    assert gb["length"] == ob["length"]
+    np.testing.assert_array_equal(gb["first"], ob["first"])
+    np.testing.assert_array_equal(gb["last"], ob["last"])
+    np.testing.assert_array_equal(gb["next"], ob["next"])
+
+
+@pytest.mark.parametrize("name,s,sc", CASES, ids=[c[0] for c in CASES])
+def test_neighbors_bit_exact(name, s, sc):
+    o, g = _pair(s, sc)
+    o.set_neighbors()
+    oo, oi, od = o.neighbors()
+    go, gi, gd = g.neighbors()
+    np.testing.assert_array_equal(go, oo)
+    np.testing.assert_array_equal(gi, oi)
+    assert (gd.view(np.uint64) == od.view(np.uint64)).all()  # distances bit for bit
+
+
+def _step_both(s, sc, steps=1, export_last=True):
+    o, g = _pair(s, sc)
+    o.load_initial_state()
+    g.load_initial_state()
+    g.set_snapshot(True)
+    co = cg = None
+    for k in range(steps):
+        ex = export_last and k == steps - 1
+        co = o.step(ex)
+        cg = g.step(ex)
+    return o, g, co, cg
+
+
+@pytest.mark.parametrize("name,s,sc", CASES, ids=[c[0] for c in CASES])
+def test_initial_number_density_bit_exact(name, s, sc):
+    o, g = _pair(s, sc)
+    o.load_initial_state()
+    g.load_initial_state()
+    assert (g.state()["ndr"] == o.state()["ndr"]).all()
+
+
+@pytest.mark.parametrize("name,s,sc", CASES, ids=[c[0] for c in CASES])
+def test_one_step_pre_pressure_state_and_system_bit_exact(name, s, sc):
+    o, g, co, cg = _step_both(s, sc)
+    opos, ovel = o.snapshot()
+    gpos, gvel = g.snapshot()
+    assert (gpos == opos).all() and (gvel == ovel).all()
+    os_, gs = o.state(), g.state()
+    np.testing.assert_array_equal(gs["bc"], os_["bc"])
+    op, gp = o.pressure_system(), g.pressure_system()
+    assert (gp["b"] == op["b"]).all()
+    assert (gp["diag"] == op["diag"]).all()
+    np.testing.assert_array_equal(gp["row_ptr"], op["row_ptr"])
+    np.testing.assert_array_equal(gp["col"], op["col"])
+    assert (gp["val"] == op["val"]).all()
+    # the solutions agree to the conditioning of A times the solver tolerances
+    nx = np.linalg.norm(op["x"])
+    assert nx > 0
+    assert np.linalg.norm(gp["x"] - op["x"]) <= P_RTOL * nx
+    st = g.solver_stats()
+    assert st["rel_residual"] <= 1e-13 and st["rows"] == int((os_["bc"] == configs.BC_INNER).sum())
+
+
+@pytest.mark.parametrize("name,s,sc", CASES, ids=[c[0] for c in CASES])
+def test_one_step_final_state(name, s, sc):
+    o, g, co, cg = _step_both(s, sc)
+    os_, gs = o.state(), g.state()
+    np.testing.assert_array_equal(gs["type"], os_["type"])
+    pscale = np.abs(os_["p"]).max()
+    assert np.abs(gs["p"] - os_["p"]).max() <= P_RTOL * pscale
+    vscale = np.abs(os_["vel"]).max()
+    assert np.abs(gs["vel"] - os_["vel"]).max() <= 1e-9 * vscale
+    assert np.abs(gs["pos"] - os_["pos"]).max() <= 1e-15
+    assert np.abs(gs["ndr"] - os_["ndr"]).max() <= 1e-14
+    assert cg == pytest.approx(co, rel=1e-9)
+
+
+def test_dambreak_584_fifty_steps():
+    s, sc, _ = configs.workload("dambreak2d_584")
+    o, g, co, cg = _step_both(s, sc, steps=50)
+    os_, gs = o.state(), g.state()
+    np.testing.assert_array_equal(gs["type"], os_["type"])
+    np.testing.assert_array_equal(gs["bc"], os_["bc"])
+    assert np.abs(gs["pos"] - os_["pos"]).max() <= 1e-12
+    vs = np.abs(os_["vel"]).max()
+    assert np.abs(gs["vel"] - os_["vel"]).max() <= 1e-8 * vs
+    ps = np.abs(os_["p"]).max()
+    assert np.abs(gs["p"] - os_["p"]).max() <= 1e-8 * ps
+
+
+def _small(types, pos, vel, **kw):
+    s = configs.Settings(**kw)
+    return s, configs.Scene(np.array(types, np.int8), np.array(pos, float), np.array(vel, float))
+
+
+COLLISION_CASES = {
+    "head_on": _small([0, 0], [[0.10, 0.1, 0.0], [0.104, 0.1, 0.0]], [[0.5, 0, 0], [-0.5, 0, 0]],
+                      gravity=(0.0, 0.0, 0.0), kinematic_viscosity=0.0),
+    "wall": _small([0, 1], [[0.10, 0.1, 0.0], [0.103, 0.1, 0.0]], [[0.3, 0, 0], [0, 0, 0]],
+                   gravity=(0.0, 0.0, 0.0), kinematic_viscosity=0.0),
+    "chain": _small([0, 0, 0], [[0.10, 0.1, 0.0], [0.104, 0.1, 0.0], [0.108, 0.1, 0.0]],
+                    [[0, 0, 0], [-0.2, 0, 0], [-0.6, 0, 0]], gravity=(0.0, 0.0, 0.0), kinematic_viscosity=0.0),
+    "out_of_domain": _small([0], [[0.399, 0.1, 0.0]], [[5.0, 0, 0]]),
+}
+
+
+@pytest.mark.parametrize("name", sorted(COLLISION_CASES))
+def test_small_scenes_bit_exact(name):
+    s, sc = COLLISION_CASES[name]
+    o, g, co, cg = _step_both(s, sc, steps=3)
+    os_, gs = o.state(), g.state()
+    assert (gs["pos"] == os_["pos"]).all() and (gs["vel"] == os_["vel"]).all()
+    np.testing.assert_array_equal(gs["type"], os_["type"])
+    assert cg == co
+    gc, gg, gu = g.debug_events()
+    oc = o.debug_counts()
+    np.testing.assert_array_equal(gc, oc)
+    np.testing.assert_array_equal(gg, np.sort(o.event_ids(0)))
+
+
+def test_dense_collisions_match_sequential_order():
+    # many overlapping fluid pairs: the level-scheduled GPU resolution must
+    # equal the reference's sequential loop bit for bit
+    s = configs.Settings(gravity=(0.0, 0.0, 0.0), kinematic_viscosity=0.0)
+    sc = configs.tutorial_dambreak()
+    rng = np.random.default_rng(7)
+    f = np.flatnonzero(sc.type == configs.FLUID)
+    sc.pos[f, :2] += rng.uniform(-0.35, 0.35, size=(f.size, 2)) * s.particle_distance
+    sc.vel[f, :2] = rng.normal(0, 0.3, size=(f.size, 2))
+    o, g = _pair(s, sc)
+    o.load_initial_state()
+    g.load_initial_state()
+    g.set_snapshot(True)
+    o.step()
+    g.step()
+    opos, ovel = o.snapshot()
+    gpos, gvel = g.snapshot()
+    assert (gpos == opos).all() and (gvel == ovel).all()
+
+
+def test_enclosed_fluid_unconnected_events():
+    l0 = 0.01
+    full = configs.lattice_2d(l0, (0.1, 0.1), (0.0, 0.1), (0.0, 0.1), 2, 2)
+    top = full.pos[full.pos[:, 1] < 0].copy()
+    top_t = full.type[full.pos[:, 1] < 0].copy()
+    top[:, 1] = 0.1 - top[:, 1]
+    sc = configs.Scene(np.concatenate([full.type, top_t]), np.concatenate([full.pos, top]),
+                       np.zeros((full.n + top.shape[0], 3)))
+    s = configs.Settings(gravity=(0.0, 0.0, 0.0), domain_x=(-0.1, 0.2), domain_y=(-0.1, 0.2))
+    o, g, co, cg = _step_both(s, sc)
+    gc, gg, gu = g.debug_events()
+    np.testing.assert_array_equal(gu, np.sort(o.event_ids(1)))
+    assert gc[1] == o.debug_counts()[1] > 0
+    assert (g.pressure_system()["diag"] == o.pressure_system()["diag"]).all()
+
+
+def test_3d_one_step():
+    s = configs.dambreak_settings(0.02, 1e-3, 1.0, dim=3)
+    sc = configs.lattice_3d(0.02, (0.3, 0.3, 0.1), (0.0, 0.1), (0.0, 0.2), (0.0, 0.1), 1, 2)
+    s.domain_x, s.domain_y, s.domain_z = (-0.1, 0.4), (-0.1, 0.4), (-0.1, 0.2)
+    sc = _jitter(s, sc, 5, amp=0.1)
+    o, g = _pair(s, sc)
+    o.set_neighbors()
+    oo, oi, od = o.neighbors()
+    go, gi, gd = g.neighbors()
+    np.testing.assert_array_equal(go, oo)
+    np.testing.assert_array_equal(gi, oi)
+    assert (gd == od).all()
+    o2, g2, co, cg = _step_both(s, sc)
+    op, gp = o2.pressure_system(), g2.pressure_system()
+    assert (gp["b"] == op["b"]).all() and (gp["diag"] == op["diag"]).all() and (gp["val"] == op["val"]).all()
+    assert np.linalg.norm(gp["x"] - op["x"]) <= P_RTOL * np.linalg.norm(op["x"])
+
+
+def test_graph_and_stream_paths_identical():
+    s, sc, _ = configs.workload("dambreak2d_584")
+    res = []
+    for use_graph in (True, False):
+        g = capi.Simulation(s, sc)
+        g.set_solver(tolerance=1e-12, use_graph=use_graph, check_interval=8)
+        g.load_initial_state()
+        for _ in range(5):
+            g.step()
+        res.append(g.state())
+    for k in ("pos", "vel", "p"):
+        assert (res[0][k] == res[1][k]).all()

@@ -1,0 +1,180 @@
+"""Benchmark: particle-steps/s of the MPS step on the 2M-particle 2D dam break,
+mixed-precision vs FP64 pressure solve at an equal 1e-9 relative residual.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+One GPU.  The timed region is K full MPS steps per path, inputs resident in
+HBM; every step's true FP64 relative residual ||b-Ax||/||b|| is checked
+against the tolerance.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from mps_amd import capi, configs  # noqa: E402
+
+METRIC = "particle-steps/s @2M 2D dambreak, mixed vs FP64 PPE at equal 1e-9 residual"
+HBM_PEAK_GBS = 8000.0
+TOL = 1e-9
+
+
+def run_path(settings, scene, precision, steps, warmup, check_interval, reps, use_graph=True):
+    sim = capi.Simulation(settings, scene)
+    sim.set_solver(precision=precision, tolerance=TOL, warm_start=True, check_interval=check_interval,
+                   inner_tolerance=1e-4, use_graph=use_graph)
+    sim.load_initial_state()
+    for _ in range(warmup):
+        sim.step()
+    its, res, refs = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()  # returns after the step's stream has drained
+        st = sim.solver_stats()
+        its.append(st["iterations"])
+        res.append(st["rel_residual"])
+        refs.append(st["refinements"])
+    t1 = time.perf_counter()
+    kern = sim.time_pressure_kernels(reps)
+    state = sim.state()
+    rows = sim.solver_stats()["rows"]
+    sim.close()
+    return dict(seconds=t1 - t0, iterations=its, residuals=res, refinements=refs, kernels=kern, state=state,
+                rows=rows)
+
+
+def cpu_baseline(settings, scene, n_fluid, threads):
+    """FP64 CPU oracle (the reference's algorithm restated in C, OpenMP on the
+    stage loops and the SpMV) for one step of the same workload, CG to the
+    same 1e-9 tolerance, single solve."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    from oracle_lib import Oracle
+
+    o = Oracle(settings, scene, tol=TOL, duplicate=False)
+    o.load_initial_state()
+    t0 = time.perf_counter()
+    o.step()
+    dt = time.perf_counter() - t0
+    it, rr = o.cg_stats()
+    o.close()
+    return dict(value=n_fluid / dt, unit="particle-steps/s", cores=threads, kind="port",
+                sample=f"1 step of dambreak2d_2M from t=0 (N={scene.n}), FP64 oracle, CG to {TOL:g} relative "
+                       f"residual in {it} iterations, single solve; {dt:.1f} s on {threads} threads of "
+                       f"{platform.processor() or platform.machine()}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="dambreak2d_2M")
+    ap.add_argument("--check-interval", type=int, default=32)
+    ap.add_argument("--reps", type=int, default=30, help="launches per kernel for the event timing")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="enqueue CG chunks directly (for profilers)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {a.gpus} is not supported (the MPS step here is a single-GPU "
+              f"path; multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    settings, scene, _ = configs.workload(a.workload)
+    n_total = scene.n
+    n_fluid = int((scene.type == configs.FLUID).sum())
+    print(f"[bench] workload {a.workload}: N={n_total} fluid={n_fluid} steps={a.steps} warmup={a.warmup}",
+          file=sys.stderr, flush=True)
+
+    out = {}
+    for name, prec in (("fp64", capi.MPS_PREC_FP64), ("mixed", capi.MPS_PREC_MIXED)):
+        r = run_path(settings, scene, prec, a.steps, a.warmup, a.check_interval, a.reps, not a.no_graph)
+        worst = max(r["residuals"]) if r["residuals"] else 0.0
+        if not worst <= TOL:
+            print(f"[bench] {name}: residual {worst:.3e} above {TOL:g}", file=sys.stderr)
+            sys.exit(1)
+        out[name] = r
+        print(f"[bench] {name}: {1e3 * r['seconds'] / a.steps:.2f} ms/step, mean CG iterations "
+              f"{np.mean(r['iterations']):.1f}, worst residual {worst:.2e}", file=sys.stderr, flush=True)
+
+    mixed, fp64 = out["mixed"], out["fp64"]
+    ms_mixed = 1e3 * mixed["seconds"] / a.steps
+    ms_fp64 = 1e3 * fp64["seconds"] / a.steps
+    value = n_fluid * a.steps / mixed["seconds"]
+    value_fp64 = n_fluid * a.steps / fp64["seconds"]
+    # error of the mixed path against the FP64 path after the same steps
+    err = {}
+    for k in ("p", "vel", "pos"):
+        d = np.abs(mixed["state"][k] - fp64["state"][k]).max()
+        s = np.abs(fp64["state"][k]).max()
+        err[k] = float(d / s) if s > 0 else float(d)
+    # dominant kernel: the FP32 SpMV of the mixed path's inner CG
+    ms, by = mixed["kernels"]["spmv32"]
+    achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_mixed,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": a.workload,
+            "n_particles": n_total,
+            "n_fluid": n_fluid,
+            "dt": settings.dt,
+            "tolerance": TOL,
+            "warm_start": True,
+            "per_particle_value_basis": "n_fluid",
+        },
+        "fp64": {
+            "value": value_fp64,
+            "ms_per_step": ms_fp64,
+            "mean_cg_iterations": float(np.mean(fp64["iterations"])),
+            "worst_rel_residual": max(fp64["residuals"]),
+        },
+        "mixed": {
+            "mean_inner_iterations": float(np.mean(mixed["iterations"])),
+            "mean_refinements": float(np.mean(mixed["refinements"])),
+            "worst_rel_residual": max(mixed["residuals"]),
+            "rel_error_vs_fp64": err,
+        },
+        "speedup_mixed_vs_fp64": value / value_fp64,
+        "value_n_total": n_total * a.steps / mixed["seconds"],
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_cg_spmv<float>",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_launch": by,
+            "avg_launch_ms": ms,
+            "rows": mixed["rows"],
+        },
+        "kernels_ms": {k: v[0] for k, v in mixed["kernels"].items()},
+    }
+    if not a.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        res["cpu_baseline"] = cpu_baseline(settings, scene, n_fluid, threads)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

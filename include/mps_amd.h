@@ -185,6 +185,14 @@ int mps_get_pressure_system(const mps_handle* h, double* b, double* diag,
 int mps_set_snapshot(mps_handle* h, int32_t on);
 int mps_get_snapshot(const mps_handle* h, double* pos, double* vel);
 
+/* Times the pressure-solver kernels on the system of the last step, with HIP
+ * events on the handle's stream, averaged over reps launches each (no
+ * counterpart in the reference; a measurement hook).  out_ms / out_bytes:
+ *   [0] FP64 SpMV, [1] FP64 vector update, [2] FP32 SpMV, [3] FP32 update,
+ * bytes = algorithmic bytes of one launch (matrix entries, column indices and
+ * vectors each read or written once).  Leaves the solver scratch undefined. */
+int mps_time_pressure_kernels(mps_handle* h, int32_t reps, double out_ms[4], int64_t out_bytes[4]);
+
 #ifdef __cplusplus
 }
 #endif

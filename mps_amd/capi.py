@@ -27,6 +27,7 @@ EXPORTED = [
     "mps_load_initial_state", "mps_step", "mps_run_steps", "mps_download", "mps_download_min_pressure",
     "mps_upload", "mps_debug_events", "mps_clear_debug_events", "mps_bucket_shape", "mps_dump_bucket",
     "mps_dump_neighbors", "mps_get_pressure_system", "mps_set_snapshot", "mps_get_snapshot",
+    "mps_time_pressure_kernels",
 ]
 
 
@@ -97,6 +98,7 @@ def lib():
     L.mps_get_pressure_system.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.mps_set_snapshot.argtypes = [vp, i32]
     L.mps_get_snapshot.argtypes = [vp, vp, vp]
+    L.mps_time_pressure_kernels.argtypes = [vp, i32, vp, vp]
     for name in EXPORTED:
         if name not in ("mps_last_error",):
             getattr(L, name).restype = i32
@@ -231,6 +233,15 @@ class Simulation:
         vel = np.zeros((self.n, 3))
         self._check(self.L.mps_get_snapshot(self.h, _p(pos), _p(vel)))
         return pos, vel
+
+    def time_pressure_kernels(self, reps=50):
+        """Average launch time (ms) and algorithmic bytes of the CG kernels on
+        the last step's system: keys spmv64, update64, spmv32, update32."""
+        ms = np.zeros(4)
+        by = np.zeros(4, np.int64)
+        self._check(self.L.mps_time_pressure_kernels(self.h, reps, _p(ms), _p(by)))
+        names = ["spmv64", "update64", "spmv32", "update32"]
+        return {k: (float(ms[i]), int(by[i])) for i, k in enumerate(names)}
 
     def pressure_system(self):
         n = self.n

@@ -873,4 +873,18 @@ int mps_get_snapshot(const mps_handle* hc, double* pos, double* vel) {
     return MPS_OK;
 }
 
+int mps_time_pressure_kernels(mps_handle* h, int32_t reps, double out_ms[4], int64_t out_bytes[4]) {
+    if (!h || !out_ms || !out_bytes || reps < 1) return MPS_E_INVALID;
+    // off-diagonal count of the system: scan of the per-slot counts
+    HIPCHK(h, scan_exclusive(h->w.nnz_s, h->w.coll_off, h->n, h->bb.scan_tmp, h->st));
+    int nnz = 0;
+    HIPCHK(h, hipMemcpyAsync(h->pinned_i + 8, h->w.coll_off + h->n, sizeof(int), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    nnz = h->pinned_i[8];
+    long long bytes[4];
+    HIPCHK(h, cg_time_kernels(h->cg, h->A, reps, nnz, h->st, out_ms, bytes));
+    for (int k = 0; k < 4; k++) out_bytes[k] = bytes[k];
+    return MPS_OK;
+}
+
 }  // extern "C"

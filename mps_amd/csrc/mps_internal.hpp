@@ -270,5 +270,7 @@ hipError_t cg_reserve(CgWork* cw, int rows);
 hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double* b_s, const double* pprev_id,
                     double* x_s, const mps_solver_options& o, mps_solver_stats& stats, hipStream_t st,
                     std::string& msg);
+hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, hipStream_t st, double ms[4],
+                           long long bytes[4]);
 
 }  // namespace mps

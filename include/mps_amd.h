@@ -96,6 +96,7 @@ typedef struct mps_solver_stats {
     int32_t converged;         /* 1 if the true relative residual met the tolerance */
     double rhs_norm;           /* ||b||_2 */
     double rel_residual;       /* ||b-Ax||_2 / ||b||_2, true FP64 residual */
+    double initial_residual;   /* ||b-Ax0||_2 / ||b||_2 of the starting guess */
 } mps_solver_stats;
 
 typedef struct mps_handle mps_handle;

@@ -58,6 +58,7 @@ class SolverStats(ctypes.Structure):
         ("converged", ctypes.c_int32),
         ("rhs_norm", ctypes.c_double),
         ("rel_residual", ctypes.c_double),
+        ("initial_residual", ctypes.c_double),
     ]
 
     def as_dict(self):

@@ -206,6 +206,7 @@ static int alloc_all(mps_handle* h) {
     HIPCHK(h, dalloc(&h->A.sdiag, n));
     HIPCHK(h, dalloc(&h->A.dgl, n));
     HIPCHK(h, dalloc(&h->A.raw_off, n + 1));
+    HIPCHK(h, dalloc(&h->A.maxdelta, 1));
     h->ev_cap = (int) std::max<size_t>(n, 1024);
     HIPCHK(h, dalloc(&h->dev_counters, 4));
     HIPCHK(h, dalloc(&h->ev.ghost_ids, (size_t) h->ev_cap));
@@ -243,7 +244,7 @@ static void free_all(mps_handle* h) {
                     h->w.vmax,      h->w.coll_cnt,    h->w.coll_off, h->w.coll_pairs, h->w.coll_dist, h->w.coll_level,
                     h->w.coll_last, h->x_s,           h->dbg_id,     h->tmp3a,        h->tmp3b,       h->tmp8,
                     h->A.slice_off, h->A.slice_w,     h->A.row_slot, h->A.sdiag,      h->A.dgl,       h->A.raw_off,
-                    h->A.col,       h->A.val64,       h->A.val32,    h->A.raw_col,    h->A.raw_val,   h->dev_counters,
+                    h->A.col,       h->A.dcol,        h->A.maxdelta, h->A.val64,       h->A.val32,    h->A.raw_col,    h->A.raw_val,   h->dev_counters,
                     h->ev.ghost_ids, h->ev.unconn_ids, h->snap_pos,  h->snap_vel};
     for (void* p : ptrs) hipFree(p);
     if (h->pinned_i) hipHostFree(h->pinned_i);

@@ -243,7 +243,10 @@ struct Sell {
     int nslices;
     int* slice_off;      // nslices + 1 (in entries)
     int* slice_w;        // nslices (max nnz of the slice)
-    int* col;            // entries
+    int* col;            // entries: column (compact row index)
+    short* dcol;         // entries: column - row, when use16
+    int use16;           // every |column - row| fits 16 bits
+    int* maxdelta;       // device: max |column - row| of the last assembly
     double* val64;       // scaled off-diagonal values (FP64)
     float* val32;        // scaled off-diagonal values (FP32), mixed path
     int* row_slot;       // rows -> slot

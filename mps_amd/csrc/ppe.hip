@@ -78,6 +78,7 @@ __global__ void __launch_bounds__(kBlock) k_fill_sell(GridView g, Consts c, Hot 
                                                       const int8_t* __restrict__ bc_s,
                                                       const double* __restrict__ diag_s,
                                                       const int* __restrict__ slice_pos, int* __restrict__ col,
+                                                      short* __restrict__ dcol, int* __restrict__ maxdelta,
                                                       double* __restrict__ val64, float* __restrict__ val32,
                                                       double* __restrict__ sdiag, double* __restrict__ dgl,
                                                       const int* __restrict__ raw_off, int* __restrict__ raw_col,
@@ -93,6 +94,7 @@ __global__ void __launch_bounds__(kBlock) k_fill_sell(GridView g, Consts c, Hot 
     const size_t base = (size_t) slice_pos[sl] * 64;
     const int width = slice_pos[sl + 1] - slice_pos[sl];
     int k = 0;
+    int md = 0;
     int ro = raw_off ? raw_off[s] : 0;
     const double re = c.re_p;
     const double zi = DIM == 3 ? h.pz[s] : 0.0;
@@ -105,7 +107,12 @@ __global__ void __launch_bounds__(kBlock) k_fill_sell(GridView g, Consts c, Hot 
                            const double sj = 1.0 / sqrt(diag_s[q]);
                            const double sv = v * (si * sj);
                            const size_t idx = base + (size_t) k * 64 + lane;
-                           col[idx] = row_of[q];
+                           const int cq = row_of[q];
+                           const int dl = cq - r;
+                           const int ad = dl < 0 ? -dl : dl;
+                           md = ad > md ? ad : md;
+                           col[idx] = cq;
+                           dcol[idx] = (short) dl;
                            val64[idx] = sv;
                            if (val32) val32[idx] = (float) sv;
                            if (raw_col) {
@@ -118,9 +125,11 @@ __global__ void __launch_bounds__(kBlock) k_fill_sell(GridView g, Consts c, Hot 
     for (; k < width; k++) {
         const size_t idx = base + (size_t) k * 64 + lane;
         col[idx] = r;
+        dcol[idx] = 0;
         val64[idx] = 0.0;
         if (val32) val32[idx] = 0.0f;
     }
+    if (md > 0) atomicMax(maxdelta, md);
 }
 
 hipError_t assemble_sell(int dim, const GridView& g, int n, const Consts& c, const Hot& h, const StepScratch& w,
@@ -156,13 +165,16 @@ hipError_t assemble_sell(int dim, const GridView& g, int n, const Consts& c, con
     if (entries > A.cap) {
         size_t cap = entries + entries / 4 + 64 * 64;
         hipFree(A.col);
+        hipFree(A.dcol);
         hipFree(A.val64);
         hipFree(A.val32);
         A.col = nullptr;
+        A.dcol = nullptr;
         A.val64 = nullptr;
         A.val32 = nullptr;
         A.cap = 0;
         if ((e = hipMalloc(&A.col, cap * sizeof(int))) != hipSuccess) return e;
+        if ((e = hipMalloc(&A.dcol, cap * sizeof(short))) != hipSuccess) return e;
         if ((e = hipMalloc(&A.val64, cap * sizeof(double))) != hipSuccess) return e;
         if ((e = hipMalloc(&A.val32, cap * sizeof(float))) != hipSuccess) return e;
         A.cap = cap;
@@ -186,20 +198,27 @@ hipError_t assemble_sell(int dim, const GridView& g, int n, const Consts& c, con
             A.raw_cap = cap;
         }
     }
+    A.use16 = 0;
     if (rows == 0) return hipSuccess;
     const double D = (double) c.dim;
     const double coefA = (2.0 * D / (c.lam_p * c.n0_p));
     const int* rows_p = w.row_of + n;
+    if ((e = fill_int(A.maxdelta, 0, 1, st)) != hipSuccess) return e;
     if (dim == 2)
         hipLaunchKernelGGL(k_fill_sell<2>, grid_for(rows), dim3(kBlock), 0, st, g, c, h, rows_p, A.row_slot, w.row_of,
-                           w.bc_s, w.diag_s, A.slice_off, A.col, A.val64, want_val32 ? A.val32 : nullptr, A.sdiag,
+                           w.bc_s, w.diag_s, A.slice_off, A.col, A.dcol, A.maxdelta, A.val64, want_val32 ? A.val32 : nullptr, A.sdiag,
                            A.dgl, want_raw ? A.raw_off : nullptr, want_raw ? A.raw_col : nullptr,
                            want_raw ? A.raw_val : nullptr, coefA);
     else
         hipLaunchKernelGGL(k_fill_sell<3>, grid_for(rows), dim3(kBlock), 0, st, g, c, h, rows_p, A.row_slot, w.row_of,
-                           w.bc_s, w.diag_s, A.slice_off, A.col, A.val64, want_val32 ? A.val32 : nullptr, A.sdiag,
+                           w.bc_s, w.diag_s, A.slice_off, A.col, A.dcol, A.maxdelta, A.val64, want_val32 ? A.val32 : nullptr, A.sdiag,
                            A.dgl, want_raw ? A.raw_off : nullptr, want_raw ? A.raw_col : nullptr,
                            want_raw ? A.raw_val : nullptr, coefA);
+    // 16-bit column offsets when every row's neighbours lie within +-32767 rows
+    int md = 0;
+    if ((e = hipMemcpyAsync(&md, A.maxdelta, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    A.use16 = md <= 32767 ? 1 : 0;
     return hipGetLastError();
 }
 
@@ -207,10 +226,17 @@ hipError_t assemble_sell(int dim, const GridView& g, int n, const Consts& c, con
 //
 // Every CG kernel launches the same grid of nb <= kMaxBlocks blocks; block b
 // owns a contiguous range of rows (a multiple of 64, so whole SELL slices)
-// and writes its partial sums to part[b * NV + k].  The next kernel starts by
-// reducing all nb partials in one fixed order, redundantly in every block:
-// the kernel boundary makes the partials visible on every XCD, no fence or
-// atomic is needed, and every block derives the same scalars bit for bit.
+// and writes its partial sums to part[b * NV + k].  A kernel that needs the
+// sums starts by reducing all nb partials in one fixed order, redundantly in
+// every block: the kernel boundary makes the partials visible on every XCD,
+// no fence or atomic is needed, and every block derives the same scalars bit
+// for bit, so results are deterministic.
+//
+// Iteration (Chronopoulos-Gear form of CG, one reduction point):
+//   apply : w = Ahat r; partials of r.r, w.r and the stopping norm
+//   step  : beta = g/g_old, alpha = g/(d - beta g/alpha_old);
+//           p = r + beta p; s = w + beta s; y += alpha p; r -= alpha s
+// apply gathers only r, once per stored entry; step is elementwise.
 
 constexpr int kMaxBlocks = 1024;
 
@@ -222,42 +248,53 @@ static int cg_blocks(int rows) {
 
 struct CgScalars {
     double gam[2];     // r.r by iteration parity
+    double alp[2];     // alpha by iteration parity
     double thr;        // stop when rho2 <= thr
-    double bnorm2;     // ||b||^2
     double rho2;       // latest residual^2 (unscaled sum a_ii r_i^2, or r.r for inner solves)
-    double init_rho2;  // residual^2 at the start of an inner solve
-    double red[2];     // results of k_reduce_pairs
-    int iter;
+    double red[3];     // results of k_reduce_parts
+    int iter;          // apply launches run (= completed updates once done)
     int max_iter;
     int done;
     int converged;
-    int first;
     int frozen;        // timing mode: kernels leave every scalar as it is
+    int pad;
 };
 
-// Fixed-order sum of nv interleaved partials (part[b*nv + k], b < nb) in every
+// Fixed-order sum of NV interleaved partials (part[b*NV + k], b < nb) in every
 // block; result broadcast to all threads.
 template <int NV>
 __device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nb, double (&out)[NV],
                                                 double* red) {
+    double acc[NV];
+    for (int k = 0; k < NV; k++) acc[k] = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+        for (int k = 0; k < NV; k++) acc[k] += part[(size_t) NV * b + k];
     for (int k = 0; k < NV; k++) {
-        double acc = 0.0;
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += part[(size_t) NV * b + k];
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-        __syncthreads();
-        double t = 0.0;
-        for (int w = 0; w < (int) (blockDim.x >> 6); w++) t += red[w];
-        out[k] = t;
-        __syncthreads();
+        double a = acc[k];
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+        if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * NV + k] = a;
     }
+    __syncthreads();
+    for (int k = 0; k < NV; k++) {
+        double t = 0.0;
+        for (int w = 0; w < (int) (blockDim.x >> 6); w++) t += red[w * NV + k];
+        out[k] = t;
+    }
+    __syncthreads();
 }
 
 template <int NV>
 __device__ __forceinline__ void write_partials(const double (&v)[NV], double* __restrict__ part, double* red) {
     for (int k = 0; k < NV; k++) {
-        double t = block_sum(v[k], red);
-        if (threadIdx.x == 0) part[(size_t) NV * blockIdx.x + k] = t;
+        double a = v[k];
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_down(a, o, 64);
+        if ((threadIdx.x & 63) == 0) red[(threadIdx.x >> 6) * NV + k] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double t = 0.0;
+        for (int w = 0; w < (int) (blockDim.x >> 6); w++) t += red[w * NV + threadIdx.x];
+        part[(size_t) NV * blockIdx.x + threadIdx.x] = t;
     }
 }
 
@@ -270,132 +307,146 @@ __device__ __forceinline__ void block_rows(int rows, int& r0, int& r1) {
     r1 = r0 + per < rows ? r0 + per : rows;
 }
 
-template <typename T>
-__device__ __forceinline__ T pval(const T* __restrict__ r, const T* __restrict__ pold, T beta, int j, bool first) {
-    return first ? r[j] : r[j] + beta * pold[j];
-}
-
-// spmv: p = r + beta p_old (own row and each neighbour's, on the fly),
-// q = Ahat p, partial p.q.  Prologue: r.r and rho2 of the previous update.
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_cg_spmv(const int* __restrict__ rows_p, const int* __restrict__ slice_pos,
-                                                    const int* __restrict__ col, const T* __restrict__ val,
-                                                    const T* __restrict__ r, const T* __restrict__ pold,
-                                                    T* __restrict__ pnew, T* __restrict__ q,
-                                                    const double* __restrict__ part_u, double* __restrict__ part_s,
-                                                    CgScalars* sc, int par) {
-#pragma clang fp contract(fast)
-    __shared__ double red[kBlock / 64];
-    // block-uniform view of the done flag: block 0 of this very launch may
-    // raise it while other blocks start
+__device__ __forceinline__ int block_uniform_done(CgScalars* sc) {
     __shared__ int s_done;
     if (threadIdx.x == 0) s_done = *(volatile int*) &sc->done;
     __syncthreads();
-    if (s_done) return;
-    double gr[2];
-    reduce_partials<2>(part_u, gridDim.x, gr, red);
-    const double g = gr[0], rho = gr[1];
+    return s_done;
+}
+
+// Column of SELL entry idx of row `row`: absolute (int) or 16-bit offset.
+template <typename IDX>
+__device__ __forceinline__ int col_of(const IDX* __restrict__ c, size_t idx, int row) {
+    if constexpr (sizeof(IDX) == 2)
+        return row + (int) c[idx];
+    else
+        return (int) c[idx];
+}
+
+// acc + sum_k val[k] x[col[k]] over the row's SELL entries, gathers issued in
+// batches of 4 for memory-level parallelism.
+template <typename T, typename IDX>
+__device__ __forceinline__ T sell_row(const int* __restrict__ slice_pos, const IDX* __restrict__ col,
+                                      const T* __restrict__ val, const T* __restrict__ x, int row, T acc) {
+    const int sl = row >> 6, lane = row & 63;
+    const int s0 = slice_pos[sl], w = slice_pos[sl + 1] - s0;
+    const size_t base = (size_t) s0 * 64 + lane;
+    int k = 0;
+    for (; k + 4 <= w; k += 4) {
+        const size_t i0 = base + (size_t) k * 64;
+        const int c0 = col_of(col, i0, row), c1 = col_of(col, i0 + 64, row);
+        const int c2 = col_of(col, i0 + 128, row), c3 = col_of(col, i0 + 192, row);
+        const T v0 = val[i0], v1 = val[i0 + 64], v2 = val[i0 + 128], v3 = val[i0 + 192];
+        const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        acc += v0 * x0;
+        acc += v1 * x1;
+        acc += v2 * x2;
+        acc += v3 * x3;
+    }
+    for (; k < w; k++) {
+        const size_t i0 = base + (size_t) k * 64;
+        acc += val[i0] * x[col_of(col, i0, row)];
+    }
+    return acc;
+}
+
+// apply: w = Ahat r (unit diagonal); partials r.r, w.r, stopping norm.
+template <typename T, typename IDX, bool WEIGHTED>
+__global__ void __launch_bounds__(kBlock) k_cg_apply(const int* __restrict__ rows_p, const int* __restrict__ slice_pos,
+                                                     const IDX* __restrict__ col, const T* __restrict__ val,
+                                                     const T* __restrict__ r, T* __restrict__ w,
+                                                     const double* __restrict__ dgl, double* __restrict__ part,
+                                                     CgScalars* sc) {
+#pragma clang fp contract(fast)
+    __shared__ double red[3 * kBlock / 64];
+    if (block_uniform_done(sc)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !sc->frozen) sc->iter += 1;
+    int r0, r1;
+    block_rows(*rows_p, r0, r1);
+    double g = 0.0, d = 0.0, rw = 0.0;
+    for (int row = r0 + (int) threadIdx.x; row < r1; row += kBlock) {
+        const T rr = r[row];
+        const T acc = sell_row<T, IDX>(slice_pos, col, val, r, row, rr);
+        w[row] = acc;
+        const double r2 = (double) rr * (double) rr;
+        g += r2;
+        d += (double) acc * (double) rr;
+        rw += WEIGHTED ? dgl[row] * r2 : r2;
+    }
+    double v[3] = {g, d, rw};
+    write_partials<3>(v, part, red);
+}
+
+// step: convergence test on r, then the vector recurrences.
+template <typename T, bool WEIGHTED>
+__global__ void __launch_bounds__(kBlock) k_cg_step(const int* __restrict__ rows_p, T* __restrict__ y,
+                                                    T* __restrict__ r, T* __restrict__ p, T* __restrict__ s,
+                                                    const T* __restrict__ w, const double* __restrict__ part,
+                                                    CgScalars* sc, int par) {
+#pragma clang fp contract(fast)
+    __shared__ double red[3 * kBlock / 64];
+    if (block_uniform_done(sc)) return;
+    double sums[3];
+    reduce_partials<3>(part, gridDim.x, sums, red);
+    const double g = sums[0], dd = sums[1], rho = sums[2];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    const bool first = sc->first != 0;
-    if (!sc->frozen) {
+    const int it = sc->iter;  // this iteration's apply has run: it = k + 1
+    const bool first = it == 1;
+    T alpha, beta;
+    if (sc->frozen) {
+        alpha = (T) 0.0;  // y and r stay, p and s stay bounded
+        beta = (T) 0.5;
+    } else {
         const bool conv = rho <= sc->thr;
-        if (conv || sc->iter >= sc->max_iter || !(rho == rho)) {
+        if (conv || it > sc->max_iter || !(rho == rho)) {
             if (lead) {
                 sc->done = 1;
                 sc->converged = conv ? 1 : 0;
                 sc->rho2 = rho;
+                sc->iter = it - 1;
             }
             return;
         }
+        const double b = first ? 0.0 : g / sc->gam[par ^ 1];
+        const double a = first ? g / dd : g / (dd - b * g / sc->alp[par ^ 1]);
         if (lead) {
             sc->gam[par] = g;
+            sc->alp[par] = a;
             sc->rho2 = rho;
         }
+        alpha = (T) a;
+        beta = (T) b;
     }
-    const T beta = (T) (first ? 0.0 : (sc->frozen ? 0.5 : g / sc->gam[par ^ 1]));
     int r0, r1;
     block_rows(*rows_p, r0, r1);
-    double pq = 0.0;
     for (int row = r0 + (int) threadIdx.x; row < r1; row += kBlock) {
-        const T pr = pval(r, pold, beta, row, first);
-        T acc = pr;
-        const int sl = row >> 6, lane = row & 63;
-        const int s0 = slice_pos[sl], w = slice_pos[sl + 1] - s0;
-        const size_t base = (size_t) s0 * 64 + lane;
-        for (int k = 0; k < w; k++) {
-            const size_t idx = base + (size_t) k * 64;
-            acc += val[idx] * pval(r, pold, beta, col[idx], first);
-        }
-        pnew[row] = pr;
-        q[row] = acc;
-        pq += (double) pr * (double) acc;
-    }
-    double v[1] = {pq};
-    write_partials<1>(v, part_s, red);
-}
-
-// update: y += alpha p, r -= alpha q; partials r.r and rho2 (sum a_ii r^2 when
-// WEIGHTED).  Prologue: p.q of the spmv.
-template <typename T, bool WEIGHTED>
-__global__ void __launch_bounds__(kBlock) k_cg_update(const int* __restrict__ rows_p, T* __restrict__ y,
-                                                      T* __restrict__ r, const T* __restrict__ p,
-                                                      const T* __restrict__ q, const double* __restrict__ dgl,
-                                                      const double* __restrict__ part_s, double* __restrict__ part_u,
-                                                      CgScalars* sc, int par) {
-#pragma clang fp contract(fast)
-    __shared__ double red[kBlock / 64];
-    __shared__ int s_done;
-    if (threadIdx.x == 0) s_done = *(volatile int*) &sc->done;
-    __syncthreads();
-    if (s_done) return;
-    double d[1];
-    reduce_partials<1>(part_s, gridDim.x, d, red);
-    const T alpha = (T) (sc->frozen ? 0.0 : sc->gam[par] / d[0]);
-    int r0, r1;
-    block_rows(*rows_p, r0, r1);
-    double rr = 0.0, rw = 0.0;
-    for (int row = r0 + (int) threadIdx.x; row < r1; row += kBlock) {
-        y[row] += alpha * p[row];
-        const T rn = r[row] - alpha * q[row];
-        r[row] = rn;
-        const double r2 = (double) rn * (double) rn;
-        rr += r2;
-        rw += WEIGHTED ? dgl[row] * r2 : r2;
-    }
-    double v[2] = {rr, rw};
-    write_partials<2>(v, part_u, red);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && !sc->frozen) {
-        sc->iter += 1;
-        sc->first = 0;
+        const T rr = r[row], ww = w[row];
+        const T pp = first ? rr : rr + beta * p[row];
+        const T ss = first ? ww : ww + beta * s[row];
+        p[row] = pp;
+        s[row] = ss;
+        y[row] += alpha * pp;
+        r[row] = rr - alpha * ss;
     }
 }
 
 // One block: sums nv-interleaved partials into sc->red[0..nv-1].
-__global__ void __launch_bounds__(kBlock) k_reduce_pairs(const double* __restrict__ part, int nb, int nv,
-                                                         CgScalars* sc) {
-    __shared__ double red[kBlock / 64];
-    double o2[2];
-    if (nv == 2) {
-        reduce_partials<2>(part, nb, o2, red);
-    } else {
-        double o1[1];
-        reduce_partials<1>(part, nb, o1, red);
-        o2[0] = o1[0];
-        o2[1] = 0.0;
-    }
-    if (threadIdx.x == 0) {
-        sc->red[0] = o2[0];
-        sc->red[1] = o2[1];
-    }
+template <int NV>
+__global__ void __launch_bounds__(kBlock) k_reduce_parts(const double* __restrict__ part, int nb, CgScalars* sc) {
+    __shared__ double red[3 * kBlock / 64];
+    double o[NV];
+    reduce_partials<NV>(part, nb, o, red);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NV; k++) sc->red[k] = o[k];
 }
 
-// FP64 init: yhat = x0 sqrt(d), bhat = S b; partial ||b||^2 (and 0).
+// FP64 init: yhat = x0 sqrt(d), bhat = S b; partial ||b||^2.
 __global__ void __launch_bounds__(kBlock) k_cg_init(const int* __restrict__ rows_p, const int* __restrict__ row_slot,
                                                     const int32_t* __restrict__ id, const double* __restrict__ b_s,
                                                     const double* __restrict__ pprev, const double* __restrict__ sdiag,
                                                     double* __restrict__ y, double* __restrict__ bh,
                                                     double* __restrict__ part) {
-    __shared__ double red[kBlock / 64];
+    __shared__ double red[3 * kBlock / 64];
     int r0, r1;
     block_rows(*rows_p, r0, r1);
     double bb = 0.0;
@@ -407,34 +458,26 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const int* __restrict__ rows
         y[row] = pprev ? pprev[id[s]] / si : 0.0;
         bb += b * b;
     }
-    double v[2] = {bb, 0.0};
-    write_partials<2>(v, part, red);
+    double v[1] = {bb};
+    write_partials<1>(v, part, red);
 }
 
-// FP64 residual r = bhat - Ahat y (unit diagonal); optionally zeroes p;
-// partials r.r and sum d r^2.
+// FP64 residual r = bhat - Ahat y (unit diagonal); partials r.r, sum d r^2.
+template <typename IDX>
 __global__ void __launch_bounds__(kBlock) k_residual64(const int* __restrict__ rows_p, const int* __restrict__ slice_pos,
-                                                       const int* __restrict__ col, const double* __restrict__ val,
+                                                       const IDX* __restrict__ col, const double* __restrict__ val,
                                                        const double* __restrict__ y, const double* __restrict__ bh,
                                                        const double* __restrict__ dgl, double* __restrict__ r,
-                                                       double* __restrict__ p0, double* __restrict__ part) {
+                                                       double* __restrict__ part) {
 #pragma clang fp contract(fast)
-    __shared__ double red[kBlock / 64];
+    __shared__ double red[3 * kBlock / 64];
     int r0, r1;
     block_rows(*rows_p, r0, r1);
     double rr = 0.0, rw = 0.0;
     for (int row = r0 + (int) threadIdx.x; row < r1; row += kBlock) {
-        double acc = y[row];
-        const int sl = row >> 6, lane = row & 63;
-        const int s0 = slice_pos[sl], w = slice_pos[sl + 1] - s0;
-        const size_t base = (size_t) s0 * 64 + lane;
-        for (int k = 0; k < w; k++) {
-            const size_t idx = base + (size_t) k * 64;
-            acc += val[idx] * y[col[idx]];
-        }
+        const double acc = sell_row<double, IDX>(slice_pos, col, val, y, row, y[row]);
         const double rv = bh[row] - acc;
         r[row] = rv;
-        if (p0) p0[row] = 0.0;
         rr += rv * rv;
         rw += dgl[row] * (rv * rv);
     }
@@ -442,11 +485,11 @@ __global__ void __launch_bounds__(kBlock) k_residual64(const int* __restrict__ r
     write_partials<2>(v, part, red);
 }
 
-// Inner solve setup: r32 = r64 (scaled residual), e = 0, p = 0; partials r.r twice.
+// Inner solve setup: r32 = r64 (scaled residual), e = 0; partial r.r.
 __global__ void __launch_bounds__(kBlock) k_inner_init(const int* __restrict__ rows_p, const double* __restrict__ r64,
                                                        float* __restrict__ r32, float* __restrict__ e32,
-                                                       float* __restrict__ p32, double* __restrict__ part) {
-    __shared__ double red[kBlock / 64];
+                                                       double* __restrict__ part) {
+    __shared__ double red[3 * kBlock / 64];
     int r0, r1;
     block_rows(*rows_p, r0, r1);
     double rr = 0.0;
@@ -454,11 +497,10 @@ __global__ void __launch_bounds__(kBlock) k_inner_init(const int* __restrict__ r
         const float rv = (float) r64[row];
         r32[row] = rv;
         e32[row] = 0.0f;
-        p32[row] = 0.0f;
         rr += (double) rv * (double) rv;
     }
-    double v[2] = {rr, rr};
-    write_partials<2>(v, part, red);
+    double v[1] = {rr};
+    write_partials<1>(v, part, red);
 }
 
 __global__ void __launch_bounds__(kBlock) k_add_correction(const int* __restrict__ rows_p, double* __restrict__ y,
@@ -482,26 +524,28 @@ struct CgWork {
     double* y = nullptr;      // FP64 solution (scaled)
     double* bh = nullptr;     // scaled rhs
     double* r64 = nullptr;    // FP64 residual (scaled)
-    double* p64[2] = {nullptr, nullptr};
-    double* q64 = nullptr;
+    double* p64 = nullptr;
+    double* s64 = nullptr;
+    double* w64 = nullptr;
     float* r32 = nullptr;
-    float* p32[2] = {nullptr, nullptr};
-    float* q32 = nullptr;
+    float* p32 = nullptr;
+    float* s32 = nullptr;
+    float* w32 = nullptr;
     float* e32 = nullptr;     // inner correction
-    double* part_s = nullptr; // spmv partials (1 per block)
-    double* part_u = nullptr; // update / residual partials (2 per block)
+    double* part = nullptr;   // block partials (3 per block)
     CgScalars* sc = nullptr;  // device scalars
     CgScalars* host = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // chunk graphs: FP64 CG, FP32 inner CG
     int graph_chunk[2] = {0, 0};
     const void* graph_key[2] = {nullptr, nullptr};
+    int graph_idx16[2] = {-1, -1};
 };
 
 CgWork* cg_create() { return new CgWork(); }
 
 static void cg_free_vectors(CgWork* cw) {
-    void* ptrs[] = {cw->y, cw->bh, cw->r64, cw->p64[0], cw->p64[1], cw->q64, cw->r32, cw->p32[0], cw->p32[1], cw->q32,
-                    cw->e32, cw->part_s, cw->part_u};
+    void* ptrs[] = {cw->y, cw->bh, cw->r64, cw->p64, cw->s64, cw->w64, cw->r32, cw->p32, cw->s32, cw->w32, cw->e32,
+                    cw->part};
     for (void* p : ptrs) hipFree(p);
 }
 
@@ -522,13 +566,12 @@ hipError_t cg_reserve(CgWork* cw, int rows) {
     hipError_t e;
     const size_t d = (size_t) cap * sizeof(double), f = (size_t) cap * sizeof(float);
     if ((e = hipMalloc(&cw->y, d)) || (e = hipMalloc(&cw->bh, d)) || (e = hipMalloc(&cw->r64, d)) ||
-        (e = hipMalloc(&cw->p64[0], d)) || (e = hipMalloc(&cw->p64[1], d)) || (e = hipMalloc(&cw->q64, d)) ||
-        (e = hipMalloc(&cw->r32, f)) || (e = hipMalloc(&cw->p32[0], f)) || (e = hipMalloc(&cw->p32[1], f)) ||
-        (e = hipMalloc(&cw->q32, f)) || (e = hipMalloc(&cw->e32, f)))
+        (e = hipMalloc(&cw->p64, d)) || (e = hipMalloc(&cw->s64, d)) || (e = hipMalloc(&cw->w64, d)) ||
+        (e = hipMalloc(&cw->r32, f)) || (e = hipMalloc(&cw->p32, f)) || (e = hipMalloc(&cw->s32, f)) ||
+        (e = hipMalloc(&cw->w32, f)) || (e = hipMalloc(&cw->e32, f)))
         return e;
     cw->nb = cg_blocks(cap);
-    if ((e = hipMalloc(&cw->part_s, sizeof(double) * (size_t) cw->nb))) return e;
-    if ((e = hipMalloc(&cw->part_u, sizeof(double) * 2 * (size_t) cw->nb))) return e;
+    if ((e = hipMalloc(&cw->part, sizeof(double) * 3 * (size_t) cw->nb))) return e;
     if (!cw->sc) {
         if ((e = hipMalloc(&cw->sc, sizeof(CgScalars)))) return e;
         if ((e = hipHostMalloc(&cw->host, sizeof(CgScalars), hipHostMallocDefault))) return e;
@@ -541,46 +584,46 @@ hipError_t cg_reserve(CgWork* cw, int rows) {
     return hipSuccess;
 }
 
-// Enqueue `chunk` CG iterations (two kernels each); chunk is even, so the p
-// buffer and scalar parities line up from one chunk to the next.
-template <typename T, bool WEIGHTED>
-static void enqueue_chunk(CgWork* cw, const Sell& A, const T* val, T* y, T* r, T* const* pbuf, T* q, int chunk,
-                          hipStream_t st) {
+// Enqueue `chunk` CG iterations (two kernels each); chunk is even, so the
+// scalar parities line up from one chunk to the next.
+template <typename T, typename IDX, bool WEIGHTED>
+static void enqueue_chunk(CgWork* cw, const Sell& A, const IDX* col, const T* val, T* y, T* r, T* p, T* s, T* w,
+                          int chunk, hipStream_t st) {
     const dim3 gr(cw->nb), bl(kBlock);
     for (int k = 0; k < chunk; k++) {
-        const int par = k & 1;
-        T* pold = pbuf[par];
-        T* pnew = pbuf[par ^ 1];
-        hipLaunchKernelGGL((k_cg_spmv<T>), gr, bl, 0, st, A.rows_dev, A.slice_off, A.col, val, r, pold, pnew, q,
-                           cw->part_u, cw->part_s, cw->sc, par);
-        hipLaunchKernelGGL((k_cg_update<T, WEIGHTED>), gr, bl, 0, st, A.rows_dev, y, r, pnew, q, A.dgl, cw->part_s,
-                           cw->part_u, cw->sc, par);
+        hipLaunchKernelGGL((k_cg_apply<T, IDX, WEIGHTED>), gr, bl, 0, st, A.rows_dev, A.slice_off, col, val, r, w,
+                           A.dgl, cw->part, cw->sc);
+        hipLaunchKernelGGL((k_cg_step<T, WEIGHTED>), gr, bl, 0, st, A.rows_dev, y, r, p, s, w, cw->part, cw->sc,
+                           k & 1);
     }
 }
 
-template <typename T, bool WEIGHTED>
-static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const T* val, T* y, T* r, T* const* pbuf, T* q,
-                              const mps_solver_options& o, hipStream_t st) {
+template <typename T, typename IDX, bool WEIGHTED>
+static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* col, const T* val, T* y, T* r, T* p,
+                              T* s, T* w, const mps_solver_options& o, hipStream_t st) {
     const int chunk = std::max(2, (o.check_interval + 1) & ~1);
+    const int idx16 = sizeof(IDX) == 2 ? 1 : 0;
     hipError_t e;
     for (;;) {
         if (o.use_graph) {
-            if (!cw->graph[which] || cw->graph_chunk[which] != chunk || cw->graph_key[which] != (const void*) val) {
+            if (!cw->graph[which] || cw->graph_chunk[which] != chunk || cw->graph_key[which] != (const void*) val ||
+                cw->graph_idx16[which] != idx16) {
                 if (cw->graph[which]) hipGraphExecDestroy(cw->graph[which]);
                 cw->graph[which] = nullptr;
                 hipGraph_t gph;
                 if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal))) return e;
-                enqueue_chunk<T, WEIGHTED>(cw, A, val, y, r, pbuf, q, chunk, st);
+                enqueue_chunk<T, IDX, WEIGHTED>(cw, A, col, val, y, r, p, s, w, chunk, st);
                 if ((e = hipStreamEndCapture(st, &gph))) return e;
                 e = hipGraphInstantiate(&cw->graph[which], gph, nullptr, nullptr, 0);
                 hipGraphDestroy(gph);
                 if (e) return e;
                 cw->graph_chunk[which] = chunk;
                 cw->graph_key[which] = val;
+                cw->graph_idx16[which] = idx16;
             }
             if ((e = hipGraphLaunch(cw->graph[which], st))) return e;
         } else {
-            enqueue_chunk<T, WEIGHTED>(cw, A, val, y, r, pbuf, q, chunk, st);
+            enqueue_chunk<T, IDX, WEIGHTED>(cw, A, col, val, y, r, p, s, w, chunk, st);
         }
         if ((e = hipMemcpyAsync(cw->host, cw->sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st))) return e;
         if ((e = hipStreamSynchronize(st))) return e;
@@ -600,9 +643,29 @@ static hipError_t read_scalars(CgWork* cw, hipStream_t st) {
 }
 
 // Reduces nv-interleaved partials into sc->red and reads the scalars back.
-static hipError_t reduce_and_read(CgWork* cw, const double* part, int nv, hipStream_t st) {
-    hipLaunchKernelGGL(k_reduce_pairs, dim3(1), dim3(kBlock), 0, st, part, cw->nb, nv, cw->sc);
+static hipError_t reduce_and_read(CgWork* cw, int nv, hipStream_t st) {
+    if (nv == 1)
+        hipLaunchKernelGGL(k_reduce_parts<1>, dim3(1), dim3(kBlock), 0, st, cw->part, cw->nb, cw->sc);
+    else
+        hipLaunchKernelGGL(k_reduce_parts<2>, dim3(1), dim3(kBlock), 0, st, cw->part, cw->nb, cw->sc);
     return read_scalars(cw, st);
+}
+
+static void launch_residual(CgWork* cw, const Sell& A, hipStream_t st) {
+    const dim3 gr(cw->nb), bl(kBlock);
+    if (A.use16)
+        hipLaunchKernelGGL(k_residual64<short>, gr, bl, 0, st, A.rows_dev, A.slice_off, A.dcol, A.val64, cw->y, cw->bh,
+                           A.dgl, cw->r64, cw->part);
+    else
+        hipLaunchKernelGGL(k_residual64<int>, gr, bl, 0, st, A.rows_dev, A.slice_off, A.col, A.val64, cw->y, cw->bh,
+                           A.dgl, cw->r64, cw->part);
+}
+
+template <typename T, bool WEIGHTED>
+static hipError_t run_cg(CgWork* cw, int which, const Sell& A, const T* val, T* y, T* r, T* p, T* s, T* w,
+                         const mps_solver_options& o, hipStream_t st) {
+    if (A.use16) return run_cg_loop<T, short, WEIGHTED>(cw, which, A, A.dcol, val, y, r, p, s, w, o, st);
+    return run_cg_loop<T, int, WEIGHTED>(cw, which, A, A.col, val, y, r, p, s, w, o, st);
 }
 
 hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double* b_s, const double* pprev_id,
@@ -620,11 +683,9 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
     const int max_it = o.max_iterations > 0 ? o.max_iterations : 2 * rows;
     const double tol = o.tolerance;
 
-    CgScalars s0{};
-    if ((e = set_scalars(cw, s0, st))) return e;
     hipLaunchKernelGGL(k_cg_init, gr, bl, 0, st, A.rows_dev, A.row_slot, h.id, b_s, o.warm_start ? pprev_id : nullptr,
-                       A.sdiag, cw->y, cw->bh, cw->part_u);
-    if ((e = reduce_and_read(cw, cw->part_u, 2, st))) return e;
+                       A.sdiag, cw->y, cw->bh, cw->part);
+    if ((e = reduce_and_read(cw, 1, st))) return e;
     const double bn2 = cw->host->red[0];
     stats.rhs_norm = sqrt(bn2);
     if (bn2 == 0.0) {
@@ -635,23 +696,21 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
         return hipGetLastError();
     }
     const double thr = tol * tol * bn2;
-    hipLaunchKernelGGL(k_residual64, gr, bl, 0, st, A.rows_dev, A.slice_off, A.col, A.val64, cw->y, cw->bh, A.dgl,
-                       cw->r64, cw->p64[0], cw->part_u);
-    if ((e = reduce_and_read(cw, cw->part_u, 2, st))) return e;
+    launch_residual(cw, A, st);
+    if ((e = reduce_and_read(cw, 2, st))) return e;
     double rho2 = cw->host->red[1];
+    stats.initial_residual = sqrt(rho2 / bn2);
     int total_it = 0, refinements = 0;
     bool conv = rho2 <= thr;
 
     if (o.precision == MPS_PREC_FP64) {
         if (!conv) {
-            // part_u already holds the initial r.r and rho2 for the first spmv
             CgScalars s{};
             s.thr = thr;
-            s.bnorm2 = bn2;
             s.max_iter = max_it;
-            s.first = 1;
             if ((e = set_scalars(cw, s, st))) return e;
-            if ((e = run_cg_loop<double, true>(cw, 0, A, A.val64, cw->y, cw->r64, cw->p64, cw->q64, o, st))) return e;
+            if ((e = run_cg<double, true>(cw, 0, A, A.val64, cw->y, cw->r64, cw->p64, cw->s64, cw->w64, o, st)))
+                return e;
             total_it = cw->host->iter;
             conv = cw->host->converged != 0;
         }
@@ -659,9 +718,8 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
         const int max_ref = o.max_refinements > 0 ? o.max_refinements : 20;
         const double eta = o.inner_tolerance > 0 ? o.inner_tolerance : 1e-4;
         while (!conv && refinements < max_ref && total_it < max_it) {
-            hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, A.rows_dev, cw->r64, cw->r32, cw->e32, cw->p32[0],
-                               cw->part_u);
-            if ((e = reduce_and_read(cw, cw->part_u, 2, st))) return e;
+            hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, A.rows_dev, cw->r64, cw->r32, cw->e32, cw->part);
+            if ((e = reduce_and_read(cw, 1, st))) return e;
             const double init = cw->host->red[0];
             // reduce the inner residual by eta, or by what the outer loop
             // still needs when that is less
@@ -670,29 +728,25 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
             if (need > red2) red2 = need * 0.25;
             CgScalars s{};
             s.thr = init * red2;
-            s.init_rho2 = init;
             s.max_iter = max_it - total_it;
-            s.first = 1;
             s.done = init == 0.0 ? 1 : 0;
             if ((e = set_scalars(cw, s, st))) return e;
             if (!s.done) {
-                if ((e = run_cg_loop<float, false>(cw, 1, A, A.val32, cw->e32, cw->r32, cw->p32, cw->q32, o, st)))
+                if ((e = run_cg<float, false>(cw, 1, A, A.val32, cw->e32, cw->r32, cw->p32, cw->s32, cw->w32, o, st)))
                     return e;
                 total_it += cw->host->iter;
             }
             hipLaunchKernelGGL(k_add_correction, gr, bl, 0, st, A.rows_dev, cw->y, cw->e32);
-            hipLaunchKernelGGL(k_residual64, gr, bl, 0, st, A.rows_dev, A.slice_off, A.col, A.val64, cw->y, cw->bh,
-                               A.dgl, cw->r64, nullptr, cw->part_u);
-            if ((e = reduce_and_read(cw, cw->part_u, 2, st))) return e;
+            launch_residual(cw, A, st);
+            if ((e = reduce_and_read(cw, 2, st))) return e;
             rho2 = cw->host->red[1];
             conv = rho2 <= thr;
             refinements++;
         }
     }
     // true FP64 residual of the final iterate
-    hipLaunchKernelGGL(k_residual64, gr, bl, 0, st, A.rows_dev, A.slice_off, A.col, A.val64, cw->y, cw->bh, A.dgl,
-                       cw->r64, nullptr, cw->part_u);
-    if ((e = reduce_and_read(cw, cw->part_u, 2, st))) return e;
+    launch_residual(cw, A, st);
+    if ((e = reduce_and_read(cw, 2, st))) return e;
     stats.iterations = total_it;
     stats.refinements = refinements;
     stats.rel_residual = sqrt(cw->host->red[1] / bn2);
@@ -700,6 +754,29 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
     hipLaunchKernelGGL(k_cg_finish, gr, bl, 0, st, A.rows_dev, A.row_slot, cw->y, A.sdiag, x_s);
     if (!conv) msg = "pressure solve stopped above its tolerance";
     return hipGetLastError();
+}
+
+template <typename IDX>
+static void time_launch(CgWork* cw, const Sell& A, const IDX* col, int kind, hipStream_t st) {
+    const dim3 gr(cw->nb), bl(kBlock);
+    switch (kind) {
+        case 0:
+            hipLaunchKernelGGL((k_cg_apply<double, IDX, true>), gr, bl, 0, st, A.rows_dev, A.slice_off, col, A.val64,
+                               cw->r64, cw->w64, A.dgl, cw->part, cw->sc);
+            break;
+        case 1:
+            hipLaunchKernelGGL((k_cg_step<double, true>), gr, bl, 0, st, A.rows_dev, cw->y, cw->r64, cw->p64, cw->s64,
+                               cw->w64, cw->part, cw->sc, 0);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_cg_apply<float, IDX, false>), gr, bl, 0, st, A.rows_dev, A.slice_off, col, A.val32,
+                               cw->r32, cw->w32, A.dgl, cw->part, cw->sc);
+            break;
+        default:
+            hipLaunchKernelGGL((k_cg_step<float, false>), gr, bl, 0, st, A.rows_dev, cw->e32, cw->r32, cw->p32,
+                               cw->s32, cw->w32, cw->part, cw->sc, 0);
+            break;
+    }
 }
 
 hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, hipStream_t st, double ms[4],
@@ -713,17 +790,21 @@ hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, h
     hipError_t e = cg_reserve(cw, rows);
     if (e) return e;
     const dim3 gr(cw->nb), bl(kBlock);
-    // finite vector contents: r = 1e-3, p = 0, y = 0, and their FP32 copies
-    if ((e = fill_double(cw->r64, 1e-3, (size_t) rows, st))) return e;
-    if ((e = fill_double(cw->p64[0], 0.0, (size_t) rows, st))) return e;
-    if ((e = fill_double(cw->y, 0.0, (size_t) rows, st))) return e;
-    hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, A.rows_dev, cw->r64, cw->r32, cw->e32, cw->p32[0], cw->part_u);
-    if ((e = fill_double(cw->part_s, 1.0, (size_t) cw->nb, st))) return e;
+    // finite vector contents
+    const size_t R = (size_t) rows;
+    if ((e = fill_double(cw->r64, 1e-3, R, st)) || (e = fill_double(cw->p64, 0.0, R, st)) ||
+        (e = fill_double(cw->s64, 0.0, R, st)) || (e = fill_double(cw->w64, 0.0, R, st)) ||
+        (e = fill_double(cw->y, 0.0, R, st)))
+        return e;
+    hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, A.rows_dev, cw->r64, cw->r32, cw->e32, cw->part);
+    hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, A.rows_dev, cw->p64, cw->p32, cw->s32, cw->part);
+    hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, A.rows_dev, cw->p64, cw->w32, cw->e32, cw->part);
+    if ((e = fill_double(cw->part, 1.0, 3 * (size_t) cw->nb, st))) return e;
     CgScalars s{};
-    s.frozen = 1;  // alpha = 0: y and r stay as they are; p = r + p/2 stays bounded
+    s.frozen = 1;
+    s.iter = 2;
     s.max_iter = 0x7fffffff;
     s.thr = -1.0;
-    s.gam[0] = s.gam[1] = 1.0;
     if ((e = set_scalars(cw, s, st))) return e;
     hipEvent_t a, b;
     if ((e = hipEventCreate(&a))) return e;
@@ -732,24 +813,10 @@ hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, h
         // one untimed launch, then reps timed launches
         for (int rep = -1; rep < reps; rep++) {
             if (rep == 0) hipEventRecord(a, st);
-            switch (kind) {
-                case 0:
-                    hipLaunchKernelGGL((k_cg_spmv<double>), gr, bl, 0, st, A.rows_dev, A.slice_off, A.col, A.val64,
-                                       cw->r64, cw->p64[0], cw->p64[1], cw->q64, cw->part_u, cw->part_s, cw->sc, 0);
-                    break;
-                case 1:
-                    hipLaunchKernelGGL((k_cg_update<double, true>), gr, bl, 0, st, A.rows_dev, cw->y, cw->r64,
-                                       cw->p64[1], cw->q64, A.dgl, cw->part_s, cw->part_u, cw->sc, 0);
-                    break;
-                case 2:
-                    hipLaunchKernelGGL((k_cg_spmv<float>), gr, bl, 0, st, A.rows_dev, A.slice_off, A.col, A.val32,
-                                       cw->r32, cw->p32[0], cw->p32[1], cw->q32, cw->part_u, cw->part_s, cw->sc, 0);
-                    break;
-                default:
-                    hipLaunchKernelGGL((k_cg_update<float, false>), gr, bl, 0, st, A.rows_dev, cw->e32, cw->r32,
-                                       cw->p32[1], cw->q32, A.dgl, cw->part_s, cw->part_u, cw->sc, 0);
-                    break;
-            }
+            if (A.use16)
+                time_launch<short>(cw, A, A.dcol, kind, st);
+            else
+                time_launch<int>(cw, A, A.col, kind, st);
         }
         hipEventRecord(b, st);
         if ((e = hipEventSynchronize(b))) break;
@@ -760,11 +827,14 @@ hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, h
     hipEventDestroy(a);
     hipEventDestroy(b);
     if (e) return e;
-    const long long R = rows;
-    bytes[0] = R * 4 * 8 + nnz * (4 + 8);
-    bytes[1] = R * (6 * 8 + 8);
-    bytes[2] = R * 4 * 4 + nnz * (4 + 4);
-    bytes[3] = R * (6 * 4);
+    const long long Rl = rows;
+    const long long ib = A.use16 ? 2 : 4;
+    // apply: matrix entries + r read + w written (+ d when weighted)
+    bytes[0] = nnz * (ib + 8) + Rl * (8 + 8 + 8);
+    // step: r, w, p, s, y read; p, s, y, r written
+    bytes[1] = Rl * 9 * 8;
+    bytes[2] = nnz * (ib + 4) + Rl * (4 + 4);
+    bytes[3] = Rl * 9 * 4;
     return hipGetLastError();
 }
 

@@ -120,6 +120,14 @@ def main():
     # dominant kernel: the FP32 SpMV of the mixed path's inner CG
     ms, by = mixed["kernels"]["spmv32"]
     achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    # HBM bytes per launch from the committed, copy-calibrated FETCH_SIZE /
+    # WRITE_SIZE runs of the same kernel, scaled to this run's row count
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "pmc_cg_apply_dambreak2d_2M.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            t = json.load(f)
+        traffic = (t["fetch_bytes"] + t["write_bytes"]) * mixed["rows"] / t["rows"]
     res = {
         "metric": METRIC,
         "value": value,
@@ -163,7 +171,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_launch": by,
             "avg_launch_ms": ms,
             "rows": mixed["rows"],

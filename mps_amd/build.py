@@ -19,6 +19,14 @@ ROOT = os.path.dirname(HERE)
 INCLUDE = os.path.join(ROOT, "include")
 
 SOURCES = ["grid.hip", "stages.hip", "ppe.hip", "capi.hip"]
+HOST = os.path.join(HERE, "host")
+HOST_SOURCES = ["particle.cpp", "logger.cpp", "settings.cpp", "exporter.cpp", "bucket.cpp", "mps.cpp",
+                "simulation.cpp", "hooks.cpp"]
+HOST_LIB = os.path.join(HERE, "libmps_host.so")
+CLI = os.path.join(HERE, "mps")
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter", "-ffp-contract=off",
+              f"-I{INCLUDE}"]
 HEADERS = [os.path.join(CSRC, "mps_internal.hpp"), os.path.join(INCLUDE, "mps_amd.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -69,9 +77,29 @@ def build(force: bool = False, verbose: bool = False) -> str:
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"link failed:\n$ {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    _build_host(force)
     if verbose:
         print(LIB)
     return LIB
+
+
+def _build_host(force):
+    """The C++ class surface (MPS, Bucket, Particle, Settings, Exporter,
+    Simulation) as libmps_host.so over libmps_amd.so, plus the `mps` CLI."""
+    srcs = [os.path.join(HOST, f) for f in HOST_SOURCES]
+    hdrs = [os.path.join(HOST, f) for f in os.listdir(HOST) if f.endswith(".hpp")] + HEADERS
+    link = ["-L" + HERE, "-lmps_amd", "-Wl,-rpath,$ORIGIN"]
+    if force or _stale(HOST_LIB, srcs + hdrs + [LIB]):
+        cmd = [CXX] + HOST_FLAGS + ["-shared", "-o", HOST_LIB] + srcs + link
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host build failed:\n$ {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    main = os.path.join(HOST, "main.cpp")
+    if force or _stale(CLI, [main, HOST_LIB] + hdrs):
+        cmd = [CXX] + HOST_FLAGS + ["-o", CLI, main, "-L" + HERE, "-lmps_host", "-lmps_amd", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"cli build failed:\n$ {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
 
 
 if __name__ == "__main__":

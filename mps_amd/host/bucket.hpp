@@ -1,0 +1,23 @@
+// bucket.hpp — the reference's uniform bucket grid (src/bucket.hpp:8-19).
+// Geometry as Bucket::Bucket computes it; storeParticles fills the linked
+// lists from the GPU counting sort (libmps_amd.so), bit-identical to the
+// reference's serial insertion.
+#pragma once
+
+#include <vector>
+
+#include "particle.hpp"
+#include "settings.hpp"
+
+class Bucket {
+public:
+    int num = 0, numX = 0, numY = 0, numZ = 0;
+    double length = 0;
+    Domain domain;
+    std::vector<int> next, first, last;
+
+    Bucket() = default;
+    Bucket(const double& reMax, const Domain& domain, const int& particleSize);
+    // Throws std::runtime_error if the GPU library fails.
+    void storeParticles(std::vector<Particle>& particles);
+};

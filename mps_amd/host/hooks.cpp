@@ -1,0 +1,64 @@
+// hooks.cpp — extern "C" entry points of libmps_host.so that exercise the
+// host-side formats without a GPU (used by the CPU test suite).
+#include <cstring>
+
+#include "exporter.hpp"
+#include "settings.hpp"
+
+extern "C" {
+
+// Settings::load of a YAML file into the C-ABI POD; returns 0 on success.
+int mps_host_load_settings(const char* yaml, mps_settings* out, char* input_csv, int cap) {
+    Settings s;
+    if (!s.load(yaml)) return 1;
+    *out = s.toC();
+    if (input_csv && cap > 0) {
+        std::strncpy(input_csv, s.inputCsvPath.string().c_str(), (size_t) cap - 1);
+        input_csv[cap - 1] = '\0';
+    }
+    return 0;
+}
+
+// Reads a particle CSV and writes it back through the CSV and VTU writers.
+int mps_host_convert_csv(const char* in_csv, double density, const char* out_csv, const char* out_vtu) {
+    std::vector<Particle> ps;
+    double t = 0.0;
+    if (!importCsv(in_csv, density, ps, t)) return 1;
+    Exporter ex;
+    if (out_csv) ex.toCsv(out_csv, ps, t);
+    if (out_vtu) ex.toVtu(out_vtu, ps, t);
+    return 0;
+}
+
+}
+
+#include "bucket.hpp"
+
+extern "C" {
+
+// Bucket(reMax, domain, N).storeParticles(particles of a CSV): dims4 gets
+// num/numX/numY/numZ; first/last (num) and next (N) are filled when not NULL.
+int mps_host_bucket_from_csv(const char* csv, double re_max, const double* dmin, const double* dmax, int* dims4,
+                             int* first, int* last, int* next) {
+    std::vector<Particle> ps;
+    double t = 0.0;
+    if (!importCsv(csv, 1000.0, ps, t)) return 1;
+    Domain d(dmin[0], dmax[0], dmin[1], dmax[1], dmin[2], dmax[2]);
+    Bucket b(re_max, d, (int) ps.size());
+    dims4[0] = b.num;
+    dims4[1] = b.numX;
+    dims4[2] = b.numY;
+    dims4[3] = b.numZ;
+    if (!first) return 0;
+    try {
+        b.storeParticles(ps);
+    } catch (const std::exception&) {
+        return 2;
+    }
+    std::copy(b.first.begin(), b.first.end(), first);
+    std::copy(b.last.begin(), b.last.end(), last);
+    std::copy(b.next.begin(), b.next.end(), next);
+    return 0;
+}
+
+}

@@ -186,6 +186,17 @@ int mps_get_pressure_system(const mps_handle* h, double* b, double* diag,
 int mps_set_snapshot(mps_handle* h, int32_t on);
 int mps_get_snapshot(const mps_handle* h, double* pos, double* vel);
 
+/* Launch block sizes of the per-particle kernel families (no counterpart in
+ * the reference; the GPU path's tuning), in this order: bucket build,
+ * viscosity+move, collision, BC+source, diagonal+connectivity, matrix fill,
+ * pressure out+minimum pressure, gradient+move, number density.  Multiples of
+ * 64 in [64, 512]; defaults 256.  Results do not depend on them.
+ * mps_autotune runs 4 x steps_per_candidate real steps (the state advances
+ * exactly as it would untuned), times each family at 64/128/256/512 threads
+ * with HIP events and keeps the fastest per family. */
+int mps_set_block_sizes(mps_handle* h, const int32_t block_sizes[9]);
+int mps_autotune(mps_handle* h, int32_t steps_per_candidate, int32_t block_sizes_out[9]);
+
 /* Times the pressure-solver kernels on the system of the last step, with HIP
  * events on the handle's stream, averaged over reps launches each (no
  * counterpart in the reference; a measurement hook).  out_ms / out_bytes:

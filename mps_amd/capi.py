@@ -27,7 +27,7 @@ EXPORTED = [
     "mps_load_initial_state", "mps_step", "mps_run_steps", "mps_download", "mps_download_min_pressure",
     "mps_upload", "mps_debug_events", "mps_clear_debug_events", "mps_bucket_shape", "mps_dump_bucket",
     "mps_dump_neighbors", "mps_get_pressure_system", "mps_set_snapshot", "mps_get_snapshot",
-    "mps_time_pressure_kernels",
+    "mps_time_pressure_kernels", "mps_set_block_sizes", "mps_autotune",
 ]
 
 
@@ -100,6 +100,8 @@ def lib():
     L.mps_set_snapshot.argtypes = [vp, i32]
     L.mps_get_snapshot.argtypes = [vp, vp, vp]
     L.mps_time_pressure_kernels.argtypes = [vp, i32, vp, vp]
+    L.mps_set_block_sizes.argtypes = [vp, vp]
+    L.mps_autotune.argtypes = [vp, i32, vp]
     for name in EXPORTED:
         if name not in ("mps_last_error",):
             getattr(L, name).restype = i32
@@ -234,6 +236,17 @@ class Simulation:
         vel = np.zeros((self.n, 3))
         self._check(self.L.mps_get_snapshot(self.h, _p(pos), _p(vel)))
         return pos, vel
+
+    def autotune(self, steps_per_candidate=2):
+        """Runs 4 x steps_per_candidate steps, keeps the fastest block size per
+        kernel family; returns the 9 chosen sizes."""
+        out = np.zeros(9, np.int32)
+        self._check(self.L.mps_autotune(self.h, steps_per_candidate, _p(out)))
+        return out
+
+    def set_block_sizes(self, sizes):
+        a = np.ascontiguousarray(sizes, np.int32)
+        self._check(self.L.mps_set_block_sizes(self.h, _p(a)))
 
     def time_pressure_kernels(self, reps=50):
         """Average launch time (ms) and algorithmic bytes of the CG kernels on

@@ -59,7 +59,29 @@ struct mps_handle {
     int want_snap = 0;
     int have_system = 0;
     int built_once = 0;
+    // launch block sizes per kernel family (see Fam); results do not depend
+    // on them, the autotuner only changes speed
+    int bs[9] = {256, 256, 256, 256, 256, 256, 256, 256, 256};
+    bool timing = false;
+    hipEvent_t tev[16][2] = {};
+    int tfam[16] = {};
+    int nsites = 0;
+    double tacc[9] = {};
 };
+
+// Kernel families the autotuner sizes independently.
+enum Fam { F_BUILD, F_VISC, F_COLL, F_BCSRC, F_DIAGCC, F_FILL, F_POUT, F_GRAD, F_NDR, F_COUNT };
+
+static void tbeg(mps_handle* h, int fam) {
+    if (!h->timing || h->nsites >= 16) return;
+    h->tfam[h->nsites] = fam;
+    hipEventRecord(h->tev[h->nsites][0], h->st);
+}
+static void tend(mps_handle* h) {
+    if (!h->timing || h->nsites >= 16) return;
+    hipEventRecord(h->tev[h->nsites][1], h->st);
+    h->nsites++;
+}
 
 static std::string g_create_error = "no error";
 
@@ -231,6 +253,9 @@ static int alloc_all(mps_handle* h) {
 }
 
 static void free_all(mps_handle* h) {
+    for (int k = 0; k < 16; k++)
+        for (int j = 0; j < 2; j++)
+            if (h->tev[k][j]) hipEventDestroy(h->tev[k][j]);
     for (int k = 0; k < 2; k++) {
         for (int f = 0; f < 6; f++) hipFree(h->P[k][f]);
         hipFree(h->ID[k]);
@@ -258,7 +283,9 @@ static void free_all(mps_handle* h) {
 static int do_build(mps_handle* h) {
     Hot c = cur(h);
     Hot nx = hot_view(h, 1 - h->pv, 1 - h->it);
-    HIPCHK(h, build_grid(h->dim, h->g, h->n, c, nx, h->bb, h->ev.err, h->st));
+    tbeg(h, F_BUILD);
+    HIPCHK(h, build_grid(h->dim, h->g, h->n, c, nx, h->bb, h->ev.err, h->st, h->bs[F_BUILD]));
+    tend(h);
     h->pv ^= 1;
     h->it ^= 1;
     h->built_once = 1;
@@ -306,7 +333,9 @@ static int number_density(mps_handle* h) {
     int rc = do_build(h);
     if (rc) return rc;
     GridView g = gview(h);
-    HIPCHK(h, launch_ndr(h->dim, g, h->n, h->c, cur(h), h->cold, h->st));
+    tbeg(h, F_NDR);
+    HIPCHK(h, launch_ndr(h->dim, g, h->n, h->c, cur(h), h->cold, h->st, h->bs[F_NDR]));
+    tend(h);
     return MPS_OK;
 }
 
@@ -319,7 +348,9 @@ static int step_core(mps_handle* h, int is_export, double* vmax_out) {
         GridView g = gview(h);
         Hot c = cur(h);
         Hot o = hot_view(h, 1 - h->pv, h->it);
-        HIPCHK(h, launch_visc_move(h->dim, g, n, h->c, c, o, h->cold, h->ev, h->st));
+        tbeg(h, F_VISC);
+        HIPCHK(h, launch_visc_move(h->dim, g, n, h->c, c, o, h->cold, h->ev, h->st, h->bs[F_VISC]));
+        tend(h);
         h->pv ^= 1;
     }
     // setNeighbors #2, collision
@@ -327,7 +358,9 @@ static int step_core(mps_handle* h, int is_export, double* vmax_out) {
     {
         GridView g = gview(h);
         Hot c = cur(h);
-        HIPCHK(h, launch_collision(h->dim, g, n, h->c, c, h->w, h->ev.err, h->st));
+        tbeg(h, F_COLL);
+        HIPCHK(h, launch_collision(h->dim, g, n, h->c, c, h->w, h->ev.err, h->st, h->bs[F_COLL]));
+        tend(h);
     }
     // setNeighbors #3, pressure
     if ((rc = do_build(h))) return rc;
@@ -335,20 +368,30 @@ static int step_core(mps_handle* h, int is_export, double* vmax_out) {
     Hot c = cur(h);
     if (h->want_snap) HIPCHK(h, launch_snapshot(h->dim, n, c, h->cold, h->snap_pos, h->snap_vel, h->st));
     if (h->want_raw) HIPCHK(h, hipMemcpyAsync(h->dbg_id, c.id, sizeof(int32_t) * (size_t) n, hipMemcpyDeviceToDevice, h->st));
-    HIPCHK(h, launch_bc_source(h->dim, g, n, h->c, c, h->cold, h->w, h->st));
-    HIPCHK(h, launch_diag_cc(h->dim, g, n, h->c, c, h->w, h->ev, h->st));
+    tbeg(h, F_BCSRC);
+    HIPCHK(h, launch_bc_source(h->dim, g, n, h->c, c, h->cold, h->w, h->st, h->bs[F_BCSRC]));
+    tend(h);
+    tbeg(h, F_DIAGCC);
+    HIPCHK(h, launch_diag_cc(h->dim, g, n, h->c, c, h->w, h->ev, h->st, h->bs[F_DIAGCC]));
+    tend(h);
     int rows = 0;
     size_t entries = 0;
+    tbeg(h, F_FILL);
     HIPCHK(h, assemble_sell(h->dim, g, n, h->c, c, h->w, h->A, h->opt.precision == MPS_PREC_MIXED, h->want_raw,
-                            h->ev.err, &rows, &entries, h->st));
+                            h->ev.err, &rows, &entries, h->st, h->bs[F_FILL]));
+    tend(h);
     std::string msg;
     HIPCHK(h, cg_solve(h->cg, h->A, c, n, h->w.b_s, h->cold.pprev, h->x_s, h->opt, h->stats, h->st, msg));
-    HIPCHK(h, launch_pressure_out(h->dim, g, n, h->c, c, h->x_s, h->cold, h->w, h->st));
+    tbeg(h, F_POUT);
+    HIPCHK(h, launch_pressure_out(h->dim, g, n, h->c, c, h->x_s, h->cold, h->w, h->st, h->bs[F_POUT]));
+    tend(h);
     h->have_system = h->want_raw;
     // pressure gradient + move
     {
         Hot o = hot_view(h, 1 - h->pv, h->it);
-        HIPCHK(h, launch_grad_move(h->dim, g, n, h->c, c, o, h->cold, h->w, h->ev, h->st));
+        tbeg(h, F_GRAD);
+        HIPCHK(h, launch_grad_move(h->dim, g, n, h->c, c, o, h->cold, h->w, h->ev, h->st, h->bs[F_GRAD]));
+        tend(h);
         h->pv ^= 1;
     }
     if (is_export || h->c.hos_on) {
@@ -871,6 +914,62 @@ int mps_get_snapshot(const mps_handle* hc, double* pos, double* vel) {
     const size_t N = (size_t) h->n;
     if (pos) HIPCHK(h, hipMemcpy(pos, h->snap_pos, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
     if (vel) HIPCHK(h, hipMemcpy(vel, h->snap_vel, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+    return MPS_OK;
+}
+
+// Project-written autotuner: for each candidate block size, run real steps
+// with every per-particle kernel family at that size and HIP events around
+// each family; then give each family its fastest size.  The state advances
+// by candidates x steps_per_candidate steps exactly as untuned steps would:
+// no result depends on these block sizes.
+int mps_autotune(mps_handle* h, int32_t steps_per_candidate, int32_t block_sizes_out[9]) {
+    if (!h || steps_per_candidate < 1) return MPS_E_INVALID;
+    const int cand[4] = {64, 128, 256, 512};
+    double best_t[F_COUNT];
+    int best_b[F_COUNT];
+    for (int f = 0; f < F_COUNT; f++) {
+        best_t[f] = 1e300;
+        best_b[f] = 256;
+    }
+    for (int k = 0; k < 16; k++)
+        for (int j = 0; j < 2; j++)
+            if (!h->tev[k][j]) HIPCHK(h, hipEventCreate(&h->tev[k][j]));
+    for (int ci = 0; ci < 4; ci++) {
+        for (int f = 0; f < F_COUNT; f++) h->bs[f] = cand[ci];
+        double acc[F_COUNT] = {};
+        for (int k = 0; k < steps_per_candidate; k++) {
+            h->timing = true;
+            h->nsites = 0;
+            int rc = step_core(h, 0, h->pinned_d);
+            h->timing = false;
+            if (rc) return rc;
+            if ((rc = drain(h))) return rc;
+            courant_of(h, h->pinned_d[0]);
+            for (int sidx = 0; sidx < h->nsites; sidx++) {
+                float ms = 0.f;
+                HIPCHK(h, hipEventElapsedTime(&ms, h->tev[sidx][0], h->tev[sidx][1]));
+                acc[h->tfam[sidx]] += ms;
+            }
+        }
+        for (int f = 0; f < F_COUNT; f++)
+            if (acc[f] > 0 && acc[f] < best_t[f]) {
+                best_t[f] = acc[f];
+                best_b[f] = cand[ci];
+            }
+    }
+    for (int f = 0; f < F_COUNT; f++) {
+        h->bs[f] = best_b[f];
+        if (block_sizes_out) block_sizes_out[f] = best_b[f];
+    }
+    return MPS_OK;
+}
+
+int mps_set_block_sizes(mps_handle* h, const int32_t block_sizes[9]) {
+    if (!h || !block_sizes) return MPS_E_INVALID;
+    for (int f = 0; f < F_COUNT; f++)
+        if (block_sizes[f] < 64 || block_sizes[f] > kMaxTuneBlock || block_sizes[f] % 64)
+            return fail(h, MPS_E_INVALID, "block sizes must be multiples of 64 in [64, 512]");
+    for (int f = 0; f < F_COUNT; f++) h->bs[f] = block_sizes[f];
     return MPS_OK;
 }
 

@@ -158,7 +158,7 @@ hipError_t fill_double(double* p, double v, size_t n, hipStream_t st) {
 // ---------------- build ----------------
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_key_count(GridView g, int n, Hot cur, int* __restrict__ cnt,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_key_count(GridView g, int n, Hot cur, int* __restrict__ cnt,
                                                       int* __restrict__ err) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(kBlock) k_key_count(GridView g, int n, Hot cur
     atomicAdd(&cnt[key], 1);
 }
 
-__global__ void __launch_bounds__(kBlock) k_bin(int n, Hot cur, int* __restrict__ cnt, const int* __restrict__ cs,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_bin(int n, Hot cur, int* __restrict__ cnt, const int* __restrict__ cs,
                                                 int* __restrict__ bin_id, int* __restrict__ dst) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(kBlock) k_bin(int n, Hot cur, int* __restrict_
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_rank_permute(GridView g, int n, Hot cur, Hot nxt,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_rank_permute(GridView g, int n, Hot cur, Hot nxt,
                                                          const int* __restrict__ bin_id,
                                                          const int* __restrict__ dst,
                                                          int* __restrict__ slot_of) {
@@ -223,8 +223,8 @@ __global__ void __launch_bounds__(kBlock) k_rank_permute(GridView g, int n, Hot 
 }
 
 hipError_t build_grid(int dim, const GridView& g, int n, const Hot& cur, Hot& nxt, const BuildBufs& b,
-                      int* err, hipStream_t st) {
-    const dim3 grid((n + kBlock - 1) / kBlock), blk(kBlock);
+                      int* err, hipStream_t st, int bs) {
+    const dim3 grid((n + bs - 1) / bs), blk(bs);
     if (n == 0) return scan_exclusive(b.cnt, b.cs, g.ncell + 1, b.scan_tmp, st);
     if (dim == 2)
         hipLaunchKernelGGL(k_key_count<2>, grid, blk, 0, st, g, n, cur, b.cnt, err);
@@ -246,7 +246,7 @@ hipError_t build_grid(int dim, const GridView& g, int n, const Hot& cur, Hot& nx
 // ---------------- neighbour dumps (test entry points) ----------------
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_nb_count(GridView g, int n, Hot h, int* __restrict__ cnt_by_id,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_nb_count(GridView g, int n, Hot h, int* __restrict__ cnt_by_id,
                                                      double re) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(kBlock) k_nb_count(GridView g, int n, Hot h, i
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_nb_list(GridView g, int n, Hot h, const int* __restrict__ off_by_id,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_nb_list(GridView g, int n, Hot h, const int* __restrict__ off_by_id,
                                                     int* __restrict__ ids, double* __restrict__ dists, double re) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;

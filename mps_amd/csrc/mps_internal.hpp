@@ -22,6 +22,7 @@
 namespace mps {
 
 constexpr int kBlock = 256;
+constexpr int kMaxTuneBlock = 512;  // largest block the autotuner may pick for per-particle kernels
 
 struct GridView {
     double xmin, ymin, zmin;
@@ -178,7 +179,7 @@ struct BuildBufs {
     int* slot_of;    // n: id -> slot
 };
 hipError_t build_grid(int dim, const GridView& g, int n, const Hot& cur, Hot& nxt,
-                      const BuildBufs& b, int* err, hipStream_t st);
+                      const BuildBufs& b, int* err, hipStream_t st, int bs);
 hipError_t dump_neighbor_counts(int dim, const GridView& g, int n, const Hot& h, int* cnt_by_id,
                                 double re, hipStream_t st);
 hipError_t dump_neighbor_lists(int dim, const GridView& g, int n, const Hot& h, const int* off_by_id,
@@ -219,19 +220,19 @@ struct StepScratch {
 };
 
 hipError_t launch_visc_move(int dim, const GridView& g, int n, const Consts& c, const Hot& cur,
-                            Hot& out, Cold& cold, Events& ev, hipStream_t st);
+                            Hot& out, Cold& cold, Events& ev, hipStream_t st, int bs);
 hipError_t launch_collision(int dim, const GridView& g, int n, const Consts& c, Hot& h, StepScratch& w,
-                            int* err, hipStream_t st);
+                            int* err, hipStream_t st, int bs);
 hipError_t launch_bc_source(int dim, const GridView& g, int n, const Consts& c, const Hot& h, Cold& cold,
-                            StepScratch& w, hipStream_t st);
+                            StepScratch& w, hipStream_t st, int bs);
 hipError_t launch_diag_cc(int dim, const GridView& g, int n, const Consts& c, const Hot& h, StepScratch& w,
-                          Events& ev, hipStream_t st);
+                          Events& ev, hipStream_t st, int bs);
 hipError_t launch_pressure_out(int dim, const GridView& g, int n, const Consts& c, const Hot& h,
-                               const double* x_s, Cold& cold, StepScratch& w, hipStream_t st);
+                               const double* x_s, Cold& cold, StepScratch& w, hipStream_t st, int bs);
 hipError_t launch_grad_move(int dim, const GridView& g, int n, const Consts& c, const Hot& cur, Hot& out,
-                            Cold& cold, StepScratch& w, Events& ev, hipStream_t st);
+                            Cold& cold, StepScratch& w, Events& ev, hipStream_t st, int bs);
 hipError_t launch_ndr(int dim, const GridView& g, int n, const Consts& c, const Hot& h, Cold& cold,
-                      hipStream_t st);
+                      hipStream_t st, int bs);
 hipError_t launch_courant(int dim, int n, const Hot& h, double* vmax, hipStream_t st);
 hipError_t launch_snapshot(int dim, int n, const Hot& h, const Cold& cold, double* pos_id, double* vel_id,
                            hipStream_t st);
@@ -262,7 +263,7 @@ struct Sell {
 
 hipError_t assemble_sell(int dim, const GridView& g, int n, const Consts& c, const Hot& h,
                          const StepScratch& w, Sell& A, int want_val32, int want_raw, int* err,
-                         int* host_rows, size_t* host_entries, hipStream_t st);
+                         int* host_rows, size_t* host_entries, hipStream_t st, int bs);
 
 struct CgWork;  // defined in ppe.hip
 CgWork* cg_create();

@@ -62,7 +62,7 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 
 // ---------------- assembly ----------------
 
-__global__ void __launch_bounds__(kBlock) k_rows(int n, const int* __restrict__ inner_flag,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_rows(int n, const int* __restrict__ inner_flag,
                                                  const int* __restrict__ row_of, const int* __restrict__ nnz_s,
                                                  int* __restrict__ row_slot, int* __restrict__ slice_w) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(kBlock) k_rows(int n, const int* __restrict__ 
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_fill_sell(GridView g, Consts c, Hot h, const int* __restrict__ rows_p,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_fill_sell(GridView g, Consts c, Hot h, const int* __restrict__ rows_p,
                                                       const int* __restrict__ row_slot, const int* __restrict__ row_of,
                                                       const int8_t* __restrict__ bc_s,
                                                       const double* __restrict__ diag_s,
@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(kBlock) k_fill_sell(GridView g, Consts c, Hot 
 
 hipError_t assemble_sell(int dim, const GridView& g, int n, const Consts& c, const Hot& h, const StepScratch& w,
                          Sell& A, int want_val32, int want_raw, int* err, int* host_rows, size_t* host_entries,
-                         hipStream_t st) {
+                         hipStream_t st, int bs) {
     (void) err;
     hipError_t e;
     const int nsl_max = (n + 63) / 64;
@@ -144,7 +144,7 @@ hipError_t assemble_sell(int dim, const GridView& g, int n, const Consts& c, con
     e = fill_int(A.slice_w, 0, (size_t) nsl_max + 1, st);
     if (e != hipSuccess) return e;
     if (n > 0)
-        hipLaunchKernelGGL(k_rows, grid_for(n), dim3(kBlock), 0, st, n, w.inner_flag, w.row_of, w.nnz_s, A.row_slot,
+        hipLaunchKernelGGL(k_rows, dim3((n + bs - 1) / bs), dim3(bs), 0, st, n, w.inner_flag, w.row_of, w.nnz_s, A.row_slot,
                            A.slice_w);
     e = scan_exclusive(A.slice_w, A.slice_off, nsl_max, w.scan_tmp, st);
     if (e != hipSuccess) return e;
@@ -205,12 +205,12 @@ hipError_t assemble_sell(int dim, const GridView& g, int n, const Consts& c, con
     const int* rows_p = w.row_of + n;
     if ((e = fill_int(A.maxdelta, 0, 1, st)) != hipSuccess) return e;
     if (dim == 2)
-        hipLaunchKernelGGL(k_fill_sell<2>, grid_for(rows), dim3(kBlock), 0, st, g, c, h, rows_p, A.row_slot, w.row_of,
+        hipLaunchKernelGGL(k_fill_sell<2>, dim3((rows + bs - 1) / bs), dim3(bs), 0, st, g, c, h, rows_p, A.row_slot, w.row_of,
                            w.bc_s, w.diag_s, A.slice_off, A.col, A.dcol, A.maxdelta, A.val64, want_val32 ? A.val32 : nullptr, A.sdiag,
                            A.dgl, want_raw ? A.raw_off : nullptr, want_raw ? A.raw_col : nullptr,
                            want_raw ? A.raw_val : nullptr, coefA);
     else
-        hipLaunchKernelGGL(k_fill_sell<3>, grid_for(rows), dim3(kBlock), 0, st, g, c, h, rows_p, A.row_slot, w.row_of,
+        hipLaunchKernelGGL(k_fill_sell<3>, dim3((rows + bs - 1) / bs), dim3(bs), 0, st, g, c, h, rows_p, A.row_slot, w.row_of,
                            w.bc_s, w.diag_s, A.slice_off, A.col, A.dcol, A.maxdelta, A.val64, want_val32 ? A.val32 : nullptr, A.sdiag,
                            A.dgl, want_raw ? A.raw_off : nullptr, want_raw ? A.raw_col : nullptr,
                            want_raw ? A.raw_val : nullptr, coefA);
@@ -539,6 +539,7 @@ struct CgWork {
     int graph_chunk[2] = {0, 0};
     const void* graph_key[2] = {nullptr, nullptr};
     int graph_idx16[2] = {-1, -1};
+    int graph_nb[2] = {0, 0};
 };
 
 CgWork* cg_create() { return new CgWork(); }
@@ -570,8 +571,8 @@ hipError_t cg_reserve(CgWork* cw, int rows) {
         (e = hipMalloc(&cw->r32, f)) || (e = hipMalloc(&cw->p32, f)) || (e = hipMalloc(&cw->s32, f)) ||
         (e = hipMalloc(&cw->w32, f)) || (e = hipMalloc(&cw->e32, f)))
         return e;
-    cw->nb = cg_blocks(cap);
-    if ((e = hipMalloc(&cw->part, sizeof(double) * 3 * (size_t) cw->nb))) return e;
+    cw->nb = cg_blocks(rows);
+    if ((e = hipMalloc(&cw->part, sizeof(double) * 3 * (size_t) kMaxBlocks))) return e;
     if (!cw->sc) {
         if ((e = hipMalloc(&cw->sc, sizeof(CgScalars)))) return e;
         if ((e = hipHostMalloc(&cw->host, sizeof(CgScalars), hipHostMallocDefault))) return e;
@@ -607,7 +608,7 @@ static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* c
     for (;;) {
         if (o.use_graph) {
             if (!cw->graph[which] || cw->graph_chunk[which] != chunk || cw->graph_key[which] != (const void*) val ||
-                cw->graph_idx16[which] != idx16) {
+                cw->graph_idx16[which] != idx16 || cw->graph_nb[which] != cw->nb) {
                 if (cw->graph[which]) hipGraphExecDestroy(cw->graph[which]);
                 cw->graph[which] = nullptr;
                 hipGraph_t gph;
@@ -620,6 +621,7 @@ static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* c
                 cw->graph_chunk[which] = chunk;
                 cw->graph_key[which] = val;
                 cw->graph_idx16[which] = idx16;
+                cw->graph_nb[which] = cw->nb;
             }
             if ((e = hipGraphLaunch(cw->graph[which], st))) return e;
         } else {
@@ -679,6 +681,9 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
     if (rows == 0) return hipSuccess;
     hipError_t e = cg_reserve(cw, rows);
     if (e) return e;
+    // the block grid (and with it the order of every reduction) depends on
+    // the live row count only, so equal systems give equal bits
+    cw->nb = cg_blocks(rows);
     const dim3 gr(cw->nb), bl(kBlock);
     const int max_it = o.max_iterations > 0 ? o.max_iterations : 2 * rows;
     const double tol = o.tolerance;
@@ -789,6 +794,7 @@ hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, h
     if (rows == 0 || reps <= 0) return hipSuccess;
     hipError_t e = cg_reserve(cw, rows);
     if (e) return e;
+    cw->nb = cg_blocks(rows);
     const dim3 gr(cw->nb), bl(kBlock);
     // finite vector contents
     const size_t R = (size_t) rows;

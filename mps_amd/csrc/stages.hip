@@ -10,7 +10,7 @@
 
 namespace mps {
 
-static inline dim3 grid_for(int n) { return dim3((unsigned) ((n + kBlock - 1) / kBlock)); }
+static inline dim3 grid_for(int n, int bs) { return dim3((unsigned) ((n + bs - 1) / bs)); }
 
 __device__ __forceinline__ void push_event(int* counter, int* ids, int cap, int id) {
     int k = atomicAdd(counter, 1);
@@ -38,7 +38,7 @@ __device__ __forceinline__ double ndens(const GridView& g, const Hot& h, int s, 
 // ---------------- gravity + viscosity + explicit move (src/mps.cpp:165-215) ----------------
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_visc_move(GridView g, int n, Consts c, Hot cur, Hot out, Cold cold,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_visc_move(GridView g, int n, Consts c, Hot cur, Hot out, Cold cold,
                                                       Events ev, double avisc) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -86,13 +86,13 @@ __global__ void __launch_bounds__(kBlock) k_visc_move(GridView g, int n, Consts 
 }
 
 hipError_t launch_visc_move(int dim, const GridView& g, int n, const Consts& c, const Hot& cur, Hot& out,
-                            Cold& cold, Events& ev, hipStream_t st) {
+                            Cold& cold, Events& ev, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
     const double avisc = (c.nu) * (2.0 * (double) c.dim) / (c.n0_v * c.lam_v);
     if (dim == 2)
-        hipLaunchKernelGGL(k_visc_move<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, cur, out, cold, ev, avisc);
+        hipLaunchKernelGGL(k_visc_move<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, cur, out, cold, ev, avisc);
     else
-        hipLaunchKernelGGL(k_visc_move<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, cur, out, cold, ev, avisc);
+        hipLaunchKernelGGL(k_visc_move<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, cur, out, cold, ev, avisc);
     return hipGetLastError();
 }
 
@@ -108,7 +108,7 @@ hipError_t launch_visc_move(int dim, const GridView& g, int n, const Consts& c, 
 // disjoint fluid particles and are resolved together; levels run in order.
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_coll_count(GridView g, int n, Consts c, Hot h, int* __restrict__ cnt) {
+__global__ void __launch_bounds__(kMaxTuneBlock) k_coll_count(GridView g, int n, Consts c, Hot h, int* __restrict__ cnt) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int id = h.id[s];
@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(kBlock) k_coll_count(GridView g, int n, Consts
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_coll_emit(GridView g, int n, Consts c, Hot h, const int* __restrict__ cnt,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_coll_emit(GridView g, int n, Consts c, Hot h, const int* __restrict__ cnt,
                                                       const int* __restrict__ off, int* __restrict__ pairs,
                                                       double* __restrict__ pdist, int cap, int* __restrict__ err) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -236,19 +236,19 @@ __global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts 
 }
 
 hipError_t launch_collision(int dim, const GridView& g, int n, const Consts& c, Hot& h, StepScratch& w, int* err,
-                            hipStream_t st) {
+                            hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
     if (dim == 2)
-        hipLaunchKernelGGL(k_coll_count<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, w.coll_cnt);
+        hipLaunchKernelGGL(k_coll_count<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, w.coll_cnt);
     else
-        hipLaunchKernelGGL(k_coll_count<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, w.coll_cnt);
+        hipLaunchKernelGGL(k_coll_count<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, w.coll_cnt);
     hipError_t e = scan_exclusive(w.coll_cnt, w.coll_off, n, w.scan_tmp, st);
     if (e != hipSuccess) return e;
     if (dim == 2)
-        hipLaunchKernelGGL(k_coll_emit<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, w.coll_cnt, w.coll_off,
+        hipLaunchKernelGGL(k_coll_emit<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, w.coll_cnt, w.coll_off,
                            w.coll_pairs, w.coll_dist, w.coll_cap, err);
     else
-        hipLaunchKernelGGL(k_coll_emit<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, w.coll_cnt, w.coll_off,
+        hipLaunchKernelGGL(k_coll_emit<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, w.coll_cnt, w.coll_off,
                            w.coll_pairs, w.coll_dist, w.coll_cap, err);
     hipLaunchKernelGGL(k_coll_resolve, dim3(1), dim3(kResolveThreads), 0, st, n, c, h, w.coll_off, w.coll_pairs,
                        w.coll_dist, w.coll_level, w.coll_last, w.coll_cap);
@@ -258,7 +258,7 @@ hipError_t launch_collision(int dim, const GridView& g, int n, const Consts& c, 
 // ---------------- boundary condition + source term (src/mps.cpp:262-320) ----------------
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_bc_source(GridView g, int n, Consts c, Hot h, Cold cold,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g, int n, Consts c, Hot h, Cold cold,
                                                       int8_t* __restrict__ bc_s, double* __restrict__ b_s,
                                                       double src_scale) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(kBlock) k_bc_source(GridView g, int n, Consts 
 
 // Higher-order source term (src/mps.cpp:288-308); needs every slot's BC.
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_source_hos(GridView g, int n, Consts c, Hot h, Cold cold,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_source_hos(GridView g, int n, Consts c, Hot h, Cold cold,
                                                        const int8_t* __restrict__ bc_s, double* __restrict__ b_s,
                                                        double hos_scale, double hos_coef) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -309,24 +309,24 @@ __global__ void __launch_bounds__(kBlock) k_source_hos(GridView g, int n, Consts
 }
 
 hipError_t launch_bc_source(int dim, const GridView& g, int n, const Consts& c, const Hot& h, Cold& cold,
-                            StepScratch& w, hipStream_t st) {
+                            StepScratch& w, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
     const double dt = c.dt;
     const double src_scale = c.relax * (1.0 / (dt * dt));
     if (dim == 2)
-        hipLaunchKernelGGL(k_bc_source<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
+        hipLaunchKernelGGL(k_bc_source<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
                            src_scale);
     else
-        hipLaunchKernelGGL(k_bc_source<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
+        hipLaunchKernelGGL(k_bc_source<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
                            src_scale);
     if (c.hos_on) {
         const double hos_scale = c.hos_gamma * (1.0 / (dt * dt));
         const double hos_coef = (1.0 / (c.n0_p * dt));
         if (dim == 2)
-            hipLaunchKernelGGL(k_source_hos<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
+            hipLaunchKernelGGL(k_source_hos<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
                                hos_scale, hos_coef);
         else
-            hipLaunchKernelGGL(k_source_hos<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
+            hipLaunchKernelGGL(k_source_hos<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
                                hos_scale, hos_coef);
     }
     return hipGetLastError();
@@ -335,7 +335,7 @@ hipError_t launch_bc_source(int dim, const GridView& g, int n, const Consts& c, 
 // ---------------- matrix diagonal + Dirichlet connectivity (src/mps.cpp:322-415) ----------------
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_diag(GridView g, int n, Consts c, Hot h, const int8_t* __restrict__ bc_s,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_diag(GridView g, int n, Consts c, Hot h, const int8_t* __restrict__ bc_s,
                                                  double* __restrict__ diag, int* __restrict__ nnz, double coefA,
                                                  double qc_add) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -378,7 +378,7 @@ __device__ __forceinline__ int cc_find(volatile int* parent, int v) {
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_cc_init(GridView g, int n, Consts c, Hot h, int* __restrict__ parent) {
+__global__ void __launch_bounds__(kMaxTuneBlock) k_cc_init(GridView g, int n, Consts c, Hot h, int* __restrict__ parent) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     int m = s;
@@ -393,7 +393,7 @@ __global__ void __launch_bounds__(kBlock) k_cc_init(GridView g, int n, Consts c,
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_cc_join(GridView g, int n, Consts c, Hot h, int* parent) {
+__global__ void __launch_bounds__(kMaxTuneBlock) k_cc_join(GridView g, int n, Consts c, Hot h, int* parent) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     if (h.type[s] == MPS_GHOST) return;
@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(kBlock) k_cc_join(GridView g, int n, Consts c,
                        });
 }
 
-__global__ void __launch_bounds__(kBlock) k_cc_finalize(int n, int* parent, const int8_t* __restrict__ bc_s,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_cc_finalize(int n, int* parent, const int8_t* __restrict__ bc_s,
                                                         int8_t* __restrict__ surf) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(kBlock) k_cc_finalize(int n, int* parent, cons
     if (bc_s[s] == MPS_BC_SURFACE) surf[r] = 1;
 }
 
-__global__ void __launch_bounds__(kBlock) k_cc_apply(int n, Hot h, const int* __restrict__ parent,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_cc_apply(int n, Hot h, const int* __restrict__ parent,
                                                      const int8_t* __restrict__ bc_s, const int8_t* __restrict__ surf,
                                                      double* __restrict__ diag, int* __restrict__ inner_flag,
                                                      Events ev) {
@@ -444,12 +444,12 @@ __global__ void __launch_bounds__(kBlock) k_cc_apply(int n, Hot h, const int* __
 }
 
 hipError_t launch_diag_cc(int dim, const GridView& g, int n, const Consts& c, const Hot& h, StepScratch& w,
-                          Events& ev, hipStream_t st) {
+                          Events& ev, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
     const double D = (double) c.dim;
     const double coefA = (2.0 * D / (c.lam_p * c.n0_p));
     const double qc_add = c.alpha / (c.dt * c.dt);
-    const dim3 gr = grid_for(n), bl(kBlock);
+    const dim3 gr = grid_for(n, bs), bl(bs);
     if (dim == 2) {
         hipLaunchKernelGGL(k_diag<2>, gr, bl, 0, st, g, n, c, h, w.bc_s, w.diag_s, w.nnz_s, coefA, qc_add);
         hipLaunchKernelGGL(k_cc_init<2>, gr, bl, 0, st, g, n, c, h, w.parent);
@@ -471,7 +471,7 @@ hipError_t launch_diag_cc(int dim, const GridView& g, int n, const Consts& c, co
 
 // ---------------- pressure out, minimum pressure (src/mps.cpp:466-503) ----------------
 
-__global__ void __launch_bounds__(kBlock) k_pressure_out(int n, Hot h, const int8_t* __restrict__ bc_s,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_pressure_out(int n, Hot h, const int8_t* __restrict__ bc_s,
                                                          const double* __restrict__ x_s, Cold cold,
                                                          double* __restrict__ p_s) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -485,7 +485,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_out(int n, Hot h, const int
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_min_pressure(GridView g, int n, Consts c, Hot h,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_min_pressure(GridView g, int n, Consts c, Hot h,
                                                          const int8_t* __restrict__ bc_s,
                                                          const double* __restrict__ p_s, Cold cold,
                                                          double* __restrict__ pmin_s) {
@@ -508,14 +508,14 @@ __global__ void __launch_bounds__(kBlock) k_min_pressure(GridView g, int n, Cons
 }
 
 hipError_t launch_pressure_out(int dim, const GridView& g, int n, const Consts& c, const Hot& h, const double* x_s,
-                               Cold& cold, StepScratch& w, hipStream_t st) {
+                               Cold& cold, StepScratch& w, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pressure_out, grid_for(n), dim3(kBlock), 0, st, n, h, w.bc_s, x_s, cold, w.p_s);
+    hipLaunchKernelGGL(k_pressure_out, grid_for(n, bs), dim3(bs), 0, st, n, h, w.bc_s, x_s, cold, w.p_s);
     if (dim == 2)
-        hipLaunchKernelGGL(k_min_pressure<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, w.bc_s, w.p_s, cold,
+        hipLaunchKernelGGL(k_min_pressure<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, w.bc_s, w.p_s, cold,
                            w.pmin_s);
     else
-        hipLaunchKernelGGL(k_min_pressure<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, w.bc_s, w.p_s, cold,
+        hipLaunchKernelGGL(k_min_pressure<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, w.bc_s, w.p_s, cold,
                            w.pmin_s);
     return hipGetLastError();
 }
@@ -523,7 +523,7 @@ hipError_t launch_pressure_out(int dim, const GridView& g, int n, const Consts& 
 // ---------------- pressure gradient + implicit move (src/mps.cpp:505-545) ----------------
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_grad_move(GridView g, int n, Consts c, Hot cur, Hot out, Cold cold,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g, int n, Consts c, Hot cur, Hot out, Cold cold,
                                                       const double* __restrict__ p_s,
                                                       const double* __restrict__ pmin_s, Events ev, double gscale) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -573,14 +573,14 @@ __global__ void __launch_bounds__(kBlock) k_grad_move(GridView g, int n, Consts 
 }
 
 hipError_t launch_grad_move(int dim, const GridView& g, int n, const Consts& c, const Hot& cur, Hot& out,
-                            Cold& cold, StepScratch& w, Events& ev, hipStream_t st) {
+                            Cold& cold, StepScratch& w, Events& ev, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
     const double gscale = (double) c.dim / c.n0_p;
     if (dim == 2)
-        hipLaunchKernelGGL(k_grad_move<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, cur, out, cold, w.p_s, w.pmin_s,
+        hipLaunchKernelGGL(k_grad_move<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, cur, out, cold, w.p_s, w.pmin_s,
                            ev, gscale);
     else
-        hipLaunchKernelGGL(k_grad_move<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, cur, out, cold, w.p_s, w.pmin_s,
+        hipLaunchKernelGGL(k_grad_move<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, cur, out, cold, w.p_s, w.pmin_s,
                            ev, gscale);
     return hipGetLastError();
 }
@@ -588,7 +588,7 @@ hipError_t launch_grad_move(int dim, const GridView& g, int n, const Consts& c, 
 // ---------------- number density ratio (src/mps.cpp:627-641) ----------------
 
 template <int DIM>
-__global__ void __launch_bounds__(kBlock) k_ndr(GridView g, int n, Consts c, Hot h, Cold cold) {
+__global__ void __launch_bounds__(kMaxTuneBlock) k_ndr(GridView g, int n, Consts c, Hot h, Cold cold) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int id = h.id[s];
@@ -602,12 +602,13 @@ __global__ void __launch_bounds__(kBlock) k_ndr(GridView g, int n, Consts c, Hot
     cold.ndr[id] = cold.bc[id] == MPS_BC_SURFACE ? 0.0 : nd / c.n0_p;
 }
 
-hipError_t launch_ndr(int dim, const GridView& g, int n, const Consts& c, const Hot& h, Cold& cold, hipStream_t st) {
+hipError_t launch_ndr(int dim, const GridView& g, int n, const Consts& c, const Hot& h, Cold& cold, hipStream_t st,
+                      int bs) {
     if (n == 0) return hipSuccess;
     if (dim == 2)
-        hipLaunchKernelGGL(k_ndr<2>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, cold);
+        hipLaunchKernelGGL(k_ndr<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, cold);
     else
-        hipLaunchKernelGGL(k_ndr<3>, grid_for(n), dim3(kBlock), 0, st, g, n, c, h, cold);
+        hipLaunchKernelGGL(k_ndr<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, cold);
     return hipGetLastError();
 }
 
@@ -639,14 +640,14 @@ __global__ void __launch_bounds__(kBlock) k_vmax(int n, Hot h, int dim3d, unsign
 hipError_t launch_courant(int dim, int n, const Hot& h, double* vmax, hipStream_t st) {
     hipError_t e = fill_double(vmax, 0.0, 1, st);
     if (e != hipSuccess || n == 0) return e;
-    hipLaunchKernelGGL(k_vmax, grid_for(n), dim3(kBlock), 0, st, n, h, dim == 3 ? 1 : 0,
+    hipLaunchKernelGGL(k_vmax, grid_for(n, kBlock), dim3(kBlock), 0, st, n, h, dim == 3 ? 1 : 0,
                        reinterpret_cast<unsigned long long*>(vmax));
     return hipGetLastError();
 }
 
 // ---------------- id-order snapshots ----------------
 
-__global__ void __launch_bounds__(kBlock) k_gather_id(int n, int dim, Hot h, Cold cold, double* __restrict__ pos,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_gather_id(int n, int dim, Hot h, Cold cold, double* __restrict__ pos,
                                                       double* __restrict__ vel) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -662,7 +663,7 @@ __global__ void __launch_bounds__(kBlock) k_gather_id(int n, int dim, Hot h, Col
 hipError_t launch_snapshot(int dim, int n, const Hot& h, const Cold& cold, double* pos_id, double* vel_id,
                            hipStream_t st) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather_id, grid_for(n), dim3(kBlock), 0, st, n, dim, h, cold, pos_id, vel_id);
+    hipLaunchKernelGGL(k_gather_id, grid_for(n, kBlock), dim3(kBlock), 0, st, n, dim, h, cold, pos_id, vel_id);
     return hipGetLastError();
 }
 

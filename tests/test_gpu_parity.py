@@ -239,3 +239,24 @@ def test_graph_and_stream_paths_identical():
         res.append(g.state())
     for k in ("pos", "vel", "p"):
         assert (res[0][k] == res[1][k]).all()
+
+
+def test_autotuner_changes_speed_not_results():
+    s, sc, _ = configs.workload("dambreak2d_584")
+    sc = _jitter(s, sc, 11)
+    runs = []
+    for mode in ("default", "tuned", "odd"):
+        g = capi.Simulation(s, sc)
+        g.set_solver(tolerance=1e-12)
+        g.load_initial_state()
+        if mode == "tuned":
+            sizes = g.autotune(2)  # 8 real steps
+            assert all(int(x) in (64, 128, 256, 512) for x in sizes)
+        else:
+            if mode == "odd":
+                g.set_block_sizes([64, 128, 512, 64, 512, 128, 64, 512, 128])
+            for _ in range(8):
+                g.step()
+        runs.append(g.state())
+    for k in ("pos", "vel", "p", "ndr", "bc", "type"):
+        assert (runs[0][k] == runs[1][k]).all() and (runs[0][k] == runs[2][k]).all(), k

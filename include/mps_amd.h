@@ -117,6 +117,24 @@ int mps_create(const mps_settings* s, int32_t n, const int32_t* id,
                mps_handle** out);
 int mps_destroy(mps_handle* h);
 
+/* Ensembles (no counterpart in the reference, which runs one scene per
+ * process): nscene independent scenes with their own settings (dt, l0,
+ * viscosity, domain, ...) in one handle, stepped together.  Particles are
+ * packed scene after scene, counts[k] per scene, with local ids 0..n_k-1 in
+ * order; every other entry point addresses them by global index (scene
+ * offset + local id).  Each scene keeps its own bucket grid (a key range of
+ * the shared cell table) and its own CG scalars and convergence test, so a
+ * scene's results are bit-identical to the same scene run alone.  All scenes
+ * share the solver options and must share dim. */
+int mps_create_ensemble(int32_t nscene, const mps_settings* s, const int32_t* counts,
+                        const int32_t* id, const int8_t* type, const double* pos,
+                        const double* vel, mps_handle** out);
+int mps_scene_count(const mps_handle* h);
+/* Per-scene statistics of the last step: solver stats and Courant number
+ * (either pointer may be NULL). mps_step's Courant is their maximum, and
+ * mps_get_solver_stats sums iterations and takes the worst residual. */
+int mps_ensemble_stats(const mps_handle* h, mps_solver_stats* stats, double* courant);
+
 int mps_set_solver(mps_handle* h, const mps_solver_options* o);
 int mps_get_solver_stats(const mps_handle* h, mps_solver_stats* out);
 

@@ -28,6 +28,7 @@ EXPORTED = [
     "mps_upload", "mps_debug_events", "mps_clear_debug_events", "mps_bucket_shape", "mps_dump_bucket",
     "mps_dump_neighbors", "mps_get_pressure_system", "mps_set_snapshot", "mps_get_snapshot",
     "mps_time_pressure_kernels", "mps_set_block_sizes", "mps_autotune",
+    "mps_create_ensemble", "mps_scene_count", "mps_ensemble_stats",
 ]
 
 
@@ -102,6 +103,9 @@ def lib():
     L.mps_time_pressure_kernels.argtypes = [vp, i32, vp, vp]
     L.mps_set_block_sizes.argtypes = [vp, vp]
     L.mps_autotune.argtypes = [vp, i32, vp]
+    L.mps_create_ensemble.argtypes = [i32, vp, vp, vp, vp, vp, vp, ctypes.POINTER(vp)]
+    L.mps_scene_count.argtypes = [vp]
+    L.mps_ensemble_stats.argtypes = [vp, vp, vp]
     for name in EXPORTED:
         if name not in ("mps_last_error",):
             getattr(L, name).restype = i32
@@ -114,21 +118,54 @@ def _p(a):
 
 
 class Simulation:
-    """One particle system on the GPU (the C ABI handle)."""
+    """One particle system on the GPU (the C ABI handle).  With a list of
+    (settings, scene) pairs it is an ensemble: independent scenes stepped
+    together in one handle, particles packed scene after scene."""
 
-    def __init__(self, settings, scene: Scene):
+    def __init__(self, settings, scene: Scene = None):
         self.L = lib()
-        self.n = scene.n
-        self._cs = settings.to_c() if hasattr(settings, "to_c") else settings
-        ids = np.arange(self.n, dtype=np.int32)
-        t = np.ascontiguousarray(scene.type, np.int8)
-        pos = np.ascontiguousarray(scene.pos, np.float64)
-        vel = np.ascontiguousarray(scene.vel, np.float64)
-        h = ctypes.c_void_p()
-        rc = self.L.mps_create(ctypes.byref(self._cs), self.n, _p(ids), _p(t), _p(pos), _p(vel), ctypes.byref(h))
+        if scene is None:
+            members = list(settings)
+            cs = (MpsSettings * len(members))(*[m[0].to_c() for m in members])
+            scenes = [m[1] for m in members]
+            counts = np.array([sc.n for sc in scenes], np.int32)
+            ids = np.concatenate([np.arange(sc.n, dtype=np.int32) for sc in scenes])
+            t = np.ascontiguousarray(np.concatenate([sc.type for sc in scenes]), np.int8)
+            pos = np.ascontiguousarray(np.concatenate([sc.pos for sc in scenes]), np.float64)
+            vel = np.ascontiguousarray(np.concatenate([sc.vel for sc in scenes]), np.float64)
+            self.n = int(counts.sum())
+            self.offsets = np.concatenate([[0], np.cumsum(counts)])
+            self._cs = cs
+            h = ctypes.c_void_p()
+            rc = self.L.mps_create_ensemble(len(members), ctypes.cast(cs, ctypes.c_void_p), _p(counts), _p(ids),
+                                            _p(t), _p(pos), _p(vel), ctypes.byref(h))
+        else:
+            self.n = scene.n
+            self.offsets = np.array([0, self.n])
+            self._cs = settings.to_c() if hasattr(settings, "to_c") else settings
+            ids = np.arange(self.n, dtype=np.int32)
+            t = np.ascontiguousarray(scene.type, np.int8)
+            pos = np.ascontiguousarray(scene.pos, np.float64)
+            vel = np.ascontiguousarray(scene.vel, np.float64)
+            h = ctypes.c_void_p()
+            rc = self.L.mps_create(ctypes.byref(self._cs), self.n, _p(ids), _p(t), _p(pos), _p(vel), ctypes.byref(h))
         if rc:
             raise MpsError(rc, self.L.mps_last_error(None).decode())
         self.h = h
+        self.nscene = len(self.offsets) - 1
+
+    def scene_stats(self):
+        """Per-scene solver statistics and Courant numbers of the last step."""
+        st = (SolverStats * self.nscene)()
+        cn = np.zeros(self.nscene)
+        self._check(self.L.mps_ensemble_stats(self.h, ctypes.cast(st, ctypes.c_void_p), _p(cn)))
+        return [x.as_dict() for x in st], cn
+
+    def scene_state(self, k, state=None):
+        """The fields of scene k (local ids) out of a state() dict."""
+        state = self.state() if state is None else state
+        a, b = int(self.offsets[k]), int(self.offsets[k + 1])
+        return {key: v[a:b] for key, v in state.items()}
 
     def _check(self, rc):
         if rc:

@@ -308,3 +308,57 @@ def read_csv(path: str):
     p = data[:, col["Pressure (Pa)"]] if "Pressure (Pa)" in col else np.zeros(ids.shape[0])
     ndr = data[:, col["Number Density Ratio"]] if "Number Density Ratio" in col else np.zeros(ids.shape[0])
     return t0, Scene(typ, pos.astype(np.float64), vel.astype(np.float64)), p, ndr
+
+
+# ---- ensembles of independent scenes ----
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014): a documented 64-bit generator, so
+    an ensemble drawn from a seed is reproducible from any language."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.state = seed & self.MASK
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        """Uniform in [0, 1) from the top 53 bits."""
+        return (self.next_u64() >> 11) * (1.0 / (1 << 53))
+
+
+def ensemble_members(seed: int = 2024, count: int = 64, l0_choices=(0.004, 0.002, 0.001)):
+    """Independent 2D scenes: half dam breaks (tank 1.0 x 0.6, column against
+    the left wall), half hydrostatic tanks (the column fills the tank's width,
+    tank 0.1 m taller than the column).  Column width U[0.15, 0.35] m, height
+    U[0.3, 0.6] m (rounded to whole particle rows), l0 drawn from l0_choices,
+    nu log-uniform in [1e-6, 1e-3], dt = 0.2 l0 / 3.5 m/s.  Returns a list of
+    (settings, scene, kind)."""
+    rng = SplitMix64(seed)
+    out = []
+    for k in range(count):
+        kind = "dambreak" if k % 2 == 0 else "hydrostatic"
+        w = 0.15 + 0.2 * rng.uniform()
+        hgt = 0.3 + 0.3 * rng.uniform()
+        l0 = l0_choices[min(int(rng.uniform() * len(l0_choices)), len(l0_choices) - 1)]
+        nu = 10.0 ** (-6.0 + 3.0 * rng.uniform())
+        nx = max(2, int(round(w / l0)))
+        ny = max(2, int(round(hgt / l0)))
+        w, hgt = nx * l0, ny * l0
+        dt = 0.2 * l0 / 3.5
+        if kind == "dambreak":
+            tank = (1.0, 0.6)
+        else:
+            tank = (w, hgt + 0.1)
+        sc = lattice_2d(l0, tank, (0.0, w), (0.0, hgt), 2, 2)
+        s = Settings(dim=2, particle_distance=l0, dt=dt, finish_time=400 * dt, output_interval=1.0,
+                     cfl_condition=0.3, domain_x=(-0.1, tank[0] + 0.1), domain_y=(-0.1, tank[1] + 0.1),
+                     kinematic_viscosity=nu, ratio_pressure=2.1, ratio_viscosity=3.1, ratio_surface=2.1)
+        out.append((s, sc, kind))
+    return out

@@ -18,14 +18,32 @@
 
 using namespace mps;
 
-struct mps_handle {
-    std::string err;
+// One independent scene of a handle (a handle from mps_create holds one).
+struct SceneInfo {
     mps_settings s{};
     Consts c{};
     GridView g{};
+    int iz0 = 1;  // 2D: the common z layer of the reference's key
+    int id0 = 0;  // first particle id of the scene
+    int n = 0;
+};
+
+struct mps_handle {
+    std::string err;
+    mps_settings s{};  // scene 0
+    Consts c{};        // scene 0 (kernel arguments of single-scene handles)
+    GridView g{};      // scene 0, ghost key of the whole key space
     int n = 0;
     int dim = 2;
-    int iz0 = 1;  // 2D: the common z layer of the reference's key
+    int iz0 = 1;
+    int nscene = 1;
+    std::vector<SceneInfo> scenes;
+    GridView* d_g = nullptr;   // device scene table (ensembles)
+    Consts* d_c = nullptr;
+    int16_t* d_sid = nullptr;
+    int* d_key0 = nullptr;     // nscene + 1 first keys
+    std::vector<mps_solver_stats> sstats;
+    std::vector<double> scourant;
     hipStream_t st = nullptr;
     // hot sets
     double* P[2][6] = {};  // px py pz vx vy vz
@@ -46,7 +64,6 @@ struct mps_handle {
     Sell A{};
     CgWork* cg = nullptr;
     mps_solver_options opt{};
-    mps_solver_stats stats{};
     Events ev{};
     int* dev_counters = nullptr;  // [0] ghost, [1] unconnected, [2] err
     int ev_cap = 0;
@@ -122,6 +139,11 @@ static Hot hot_view(mps_handle* h, int pv, int it) {
 
 static Hot cur(mps_handle* h) { return hot_view(h, h->pv, h->it); }
 
+static SceneTab tab(const mps_handle* h) {
+    if (h->nscene > 1) return SceneTab{h->d_g, h->d_c, h->d_sid};
+    return SceneTab{nullptr, nullptr, nullptr};
+}
+
 static void ref_values(int dim, double l0, double re, double* n0_out, double* lam_out) {
     // RefValues (src/mps.cpp:132-163), host-side, FP64 in the reference's order
     int iz0 = dim == 2 ? 0 : -4, iz1 = dim == 2 ? 1 : 5;
@@ -143,34 +165,81 @@ static void ref_values(int dim, double l0, double re, double* n0_out, double* la
     *lam_out = lam;
 }
 
-static int gridview_build(mps_handle* h) {
-    const mps_settings& s = h->s;
-    GridView& g = h->g;
+// Bucket::Bucket (src/bucket.cpp:10-22) for one scene.
+static bool scene_grid(const mps_settings& s, int dim, GridView& g, std::string& err) {
     g.xmin = s.domain_min[0];
     g.ymin = s.domain_min[1];
     g.zmin = s.domain_min[2];
     g.len = s.re_max;
-    // Bucket::Bucket (src/bucket.cpp:10-22)
     const double lx = s.domain_max[0] - s.domain_min[0];
     const double ly = s.domain_max[1] - s.domain_min[1];
     const double lz = s.domain_max[2] - s.domain_min[2];
     const double fx = lx / g.len, fy = ly / g.len, fz = lz / g.len;
-    if (!(fx >= 0 && fy >= 0 && fz >= 0) || fx > 1e8 || fy > 1e8 || fz > 1e8)
-        return fail(h, MPS_E_INVALID, "domain range is not a valid bucket grid");
+    if (!(fx >= 0 && fy >= 0 && fz >= 0) || fx > 1e8 || fy > 1e8 || fz > 1e8) {
+        err = "domain range is not a valid bucket grid";
+        return false;
+    }
     g.nx = (int) fx + 3;
     g.ny = (int) fy + 3;
     g.nz = (int) fz + 3;
-    const double cells = (double) g.nx * g.ny * (h->dim == 2 ? 1 : g.nz);
-    if (cells > 1.0e9) return fail(h, MPS_E_INVALID, "bucket grid too large");
-    g.flat = (h->dim == 2 || g.nz == 3) ? 1 : 0;
+    const double cells = (double) g.nx * g.ny * (dim == 2 ? 1 : g.nz);
+    if (cells > 1.0e9) {
+        err = "bucket grid too large";
+        return false;
+    }
+    g.flat = (dim == 2 || g.nz == 3) ? 1 : 0;
     g.ncell = g.flat ? g.nx * g.ny : g.nx * g.ny * g.nz;
+    g.key0 = 0;
+    g.ghost_key = g.ncell;
     g.cs = nullptr;
-    return MPS_OK;
+    return true;
+}
+
+// Settings-derived constants (src/settings.cpp:43-61) and RefValues
+// (src/mps.cpp:21-35) of one scene.
+static void scene_consts(const mps_settings& s, Consts& c) {
+    c = Consts{};
+    c.dim = s.dim;
+    c.dt = s.dt;
+    c.l0 = s.particle_distance;
+    c.rho = s.density;
+    c.nu = s.kinematic_viscosity;
+    for (int k = 0; k < 3; k++) {
+        c.g[k] = s.gravity[k];
+        c.dmin[k] = s.domain_min[k];
+        c.dmax[k] = s.domain_max[k];
+    }
+    c.re_p = s.re_pressure;
+    c.re_v = s.re_viscosity;
+    c.re_s = s.re_surface;
+    c.re_max = s.re_max;
+    ref_values(s.dim, s.particle_distance, s.re_pressure, &c.n0_p, &c.lam_p);
+    ref_values(s.dim, s.particle_distance, s.re_viscosity, &c.n0_v, &c.lam_v);
+    ref_values(s.dim, s.particle_distance, s.re_surface, &c.n0_s, &c.lam_s);
+    c.beta = s.surface_threshold;
+    c.relax = s.relaxation;
+    c.qc_on = s.qc_on;
+    c.alpha = s.qc_on ? s.compressibility : 0.0;
+    c.hos_on = s.hos_on;
+    c.hos_gamma = s.hos_gamma;
+    c.dc = s.collision_distance;
+    c.restitution = s.restitution;
+    c.cfl = s.cfl_condition;
+    const double dt = c.dt;
+    const double D = (double) c.dim;
+    c.avisc = (c.nu) * (2.0 * D) / (c.n0_v * c.lam_v);
+    c.src_scale = c.relax * (1.0 / (dt * dt));
+    c.hos_scale = c.hos_gamma * (1.0 / (dt * dt));
+    c.hos_coef = (1.0 / (c.n0_p * dt));
+    c.coefA = (2.0 * D / (c.lam_p * c.n0_p));
+    c.qc_add = c.alpha / (dt * dt);
+    c.gscale = D / c.n0_p;
 }
 
 static int alloc_all(mps_handle* h) {
     const size_t n = (size_t) h->n;
     const int d3 = h->dim == 3;
+    const size_t ns = (size_t) h->nscene;
     for (int k = 0; k < 2; k++) {
         for (int f = 0; f < 6; f++) {
             if (!d3 && (f == 2 || f == 5)) continue;
@@ -180,7 +249,7 @@ static int alloc_all(mps_handle* h) {
         HIPCHK(h, dalloc(&h->TY[k], n));
         HIPCHK(h, dalloc(&h->KEY[k], n));
     }
-    const size_t nc = (size_t) h->g.ncell;
+    const size_t nc = (size_t) h->g.ghost_key;  // every scene's cells
     HIPCHK(h, dalloc(&h->bb.cnt, nc + 1));
     HIPCHK(h, dalloc(&h->bb.cs, nc + 2));
     HIPCHK(h, dalloc(&h->bb.bin_id, n));
@@ -207,7 +276,13 @@ static int alloc_all(mps_handle* h) {
     HIPCHK(h, dalloc(&w.row_of, n + 1));
     HIPCHK(h, dalloc(&w.p_s, n));
     HIPCHK(h, dalloc(&w.pmin_s, n));
-    HIPCHK(h, dalloc(&w.vmax, 1));
+    HIPCHK(h, dalloc(&w.vmax, ns));
+    HIPCHK(h, dalloc(&h->d_key0, ns + 1));
+    if (ns > 1) {
+        HIPCHK(h, dalloc(&h->d_g, ns));
+        HIPCHK(h, dalloc(&h->d_c, ns));
+        HIPCHK(h, dalloc(&h->d_sid, n));
+    }
     HIPCHK(h, dalloc(&w.coll_cnt, n));
     HIPCHK(h, dalloc(&w.coll_off, n + 1));
     w.coll_cap = (int) std::min<size_t>(std::max<size_t>(4 * n, 4096), (size_t) 1 << 26);
@@ -238,7 +313,7 @@ static int alloc_all(mps_handle* h) {
     h->ev.err = h->dev_counters + 2;
     h->ev.cap = h->ev_cap;
     HIPCHK(h, hipHostMalloc((void**) &h->pinned_i, 16 * sizeof(int), hipHostMallocDefault));
-    HIPCHK(h, hipHostMalloc((void**) &h->pinned_d, 16 * sizeof(double), hipHostMallocDefault));
+    HIPCHK(h, hipHostMalloc((void**) &h->pinned_d, (16 + ns) * sizeof(double), hipHostMallocDefault));
     HIPCHK(h, fill_int(h->bb.cnt, 0, nc + 1, h->st));
     HIPCHK(h, fill_int(h->w.coll_last, 0, n, h->st));
     HIPCHK(h, fill_int(h->dev_counters, 0, 4, h->st));
@@ -270,7 +345,8 @@ static void free_all(mps_handle* h) {
                     h->w.coll_last, h->x_s,           h->dbg_id,     h->tmp3a,        h->tmp3b,       h->tmp8,
                     h->A.slice_off, h->A.slice_w,     h->A.row_slot, h->A.sdiag,      h->A.dgl,       h->A.raw_off,
                     h->A.col,       h->A.dcol,        h->A.maxdelta, h->A.val64,       h->A.val32,    h->A.raw_col,    h->A.raw_val,   h->dev_counters,
-                    h->ev.ghost_ids, h->ev.unconn_ids, h->snap_pos,  h->snap_vel};
+                    h->ev.ghost_ids, h->ev.unconn_ids, h->snap_pos,  h->snap_vel,     h->d_g,         h->d_c,
+                    h->d_sid,       h->d_key0};
     for (void* p : ptrs) hipFree(p);
     if (h->pinned_i) hipHostFree(h->pinned_i);
     if (h->pinned_d) hipHostFree(h->pinned_d);
@@ -284,7 +360,7 @@ static int do_build(mps_handle* h) {
     Hot c = cur(h);
     Hot nx = hot_view(h, 1 - h->pv, 1 - h->it);
     tbeg(h, F_BUILD);
-    HIPCHK(h, build_grid(h->dim, h->g, h->n, c, nx, h->bb, h->ev.err, h->st, h->bs[F_BUILD]));
+    HIPCHK(h, build_grid(h->dim, h->g, tab(h), h->n, c, nx, h->bb, h->ev.err, h->st, h->bs[F_BUILD]));
     tend(h);
     h->pv ^= 1;
     h->it ^= 1;
@@ -334,7 +410,7 @@ static int number_density(mps_handle* h) {
     if (rc) return rc;
     GridView g = gview(h);
     tbeg(h, F_NDR);
-    HIPCHK(h, launch_ndr(h->dim, g, h->n, h->c, cur(h), h->cold, h->st, h->bs[F_NDR]));
+    HIPCHK(h, launch_ndr(h->dim, g, tab(h), h->n, h->c, cur(h), h->cold, h->st, h->bs[F_NDR]));
     tend(h);
     return MPS_OK;
 }
@@ -349,7 +425,7 @@ static int step_core(mps_handle* h, int is_export, double* vmax_out) {
         Hot c = cur(h);
         Hot o = hot_view(h, 1 - h->pv, h->it);
         tbeg(h, F_VISC);
-        HIPCHK(h, launch_visc_move(h->dim, g, n, h->c, c, o, h->cold, h->ev, h->st, h->bs[F_VISC]));
+        HIPCHK(h, launch_visc_move(h->dim, g, tab(h), n, h->c, c, o, h->cold, h->ev, h->st, h->bs[F_VISC]));
         tend(h);
         h->pv ^= 1;
     }
@@ -359,7 +435,7 @@ static int step_core(mps_handle* h, int is_export, double* vmax_out) {
         GridView g = gview(h);
         Hot c = cur(h);
         tbeg(h, F_COLL);
-        HIPCHK(h, launch_collision(h->dim, g, n, h->c, c, h->w, h->ev.err, h->st, h->bs[F_COLL]));
+        HIPCHK(h, launch_collision(h->dim, g, tab(h), n, h->c, c, h->w, h->ev.err, h->st, h->bs[F_COLL]));
         tend(h);
     }
     // setNeighbors #3, pressure
@@ -369,46 +445,53 @@ static int step_core(mps_handle* h, int is_export, double* vmax_out) {
     if (h->want_snap) HIPCHK(h, launch_snapshot(h->dim, n, c, h->cold, h->snap_pos, h->snap_vel, h->st));
     if (h->want_raw) HIPCHK(h, hipMemcpyAsync(h->dbg_id, c.id, sizeof(int32_t) * (size_t) n, hipMemcpyDeviceToDevice, h->st));
     tbeg(h, F_BCSRC);
-    HIPCHK(h, launch_bc_source(h->dim, g, n, h->c, c, h->cold, h->w, h->st, h->bs[F_BCSRC]));
+    HIPCHK(h, launch_bc_source(h->dim, g, tab(h), n, h->c, c, h->cold, h->w, h->st, h->bs[F_BCSRC]));
     tend(h);
     tbeg(h, F_DIAGCC);
-    HIPCHK(h, launch_diag_cc(h->dim, g, n, h->c, c, h->w, h->ev, h->st, h->bs[F_DIAGCC]));
+    HIPCHK(h, launch_diag_cc(h->dim, g, tab(h), n, h->c, c, h->w, h->ev, h->st, h->bs[F_DIAGCC]));
     tend(h);
     int rows = 0;
     size_t entries = 0;
     tbeg(h, F_FILL);
-    HIPCHK(h, assemble_sell(h->dim, g, n, h->c, c, h->w, h->A, h->opt.precision == MPS_PREC_MIXED, h->want_raw,
+    HIPCHK(h, assemble_sell(h->dim, g, tab(h), n, h->c, c, h->w, h->A, h->opt.precision == MPS_PREC_MIXED, h->want_raw,
                             h->ev.err, &rows, &entries, h->st, h->bs[F_FILL]));
     tend(h);
     std::string msg;
-    HIPCHK(h, cg_solve(h->cg, h->A, c, n, h->w.b_s, h->cold.pprev, h->x_s, h->opt, h->stats, h->st, msg));
+    HIPCHK(h, cg_solve(h->cg, h->A, c, n, h->w.b_s, h->cold.pprev, h->x_s, h->opt, h->sstats.data(), h->st, msg));
     tbeg(h, F_POUT);
-    HIPCHK(h, launch_pressure_out(h->dim, g, n, h->c, c, h->x_s, h->cold, h->w, h->st, h->bs[F_POUT]));
+    HIPCHK(h, launch_pressure_out(h->dim, g, tab(h), n, h->c, c, h->x_s, h->cold, h->w, h->st, h->bs[F_POUT]));
     tend(h);
     h->have_system = h->want_raw;
     // pressure gradient + move
     {
         Hot o = hot_view(h, 1 - h->pv, h->it);
         tbeg(h, F_GRAD);
-        HIPCHK(h, launch_grad_move(h->dim, g, n, h->c, c, o, h->cold, h->w, h->ev, h->st, h->bs[F_GRAD]));
+        HIPCHK(h, launch_grad_move(h->dim, g, tab(h), n, h->c, c, o, h->cold, h->w, h->ev, h->st, h->bs[F_GRAD]));
         tend(h);
         h->pv ^= 1;
     }
     if (is_export || h->c.hos_on) {
         if ((rc = number_density(h))) return rc;
     }
-    HIPCHK(h, launch_courant(h->dim, n, cur(h), h->w.vmax, h->st));
+    HIPCHK(h, launch_courant(h->dim, n, cur(h), tab(h), h->nscene, h->w.vmax, h->st));
     if (vmax_out) {
-        HIPCHK(h, hipMemcpyAsync(h->pinned_d, h->w.vmax, sizeof(double), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(h, hipMemcpyAsync(vmax_out, h->w.vmax, sizeof(double) * (size_t) h->nscene, hipMemcpyDeviceToHost,
+                                 h->st));
     }
     return MPS_OK;
 }
 
-static double courant_of(mps_handle* h, double vmax) {
-    // src/mps.cpp:554-565
-    double cn = (vmax * h->s.dt) / h->s.particle_distance;
-    if (cn > h->s.cfl_condition) h->counts[2]++;
-    return cn;
+// src/mps.cpp:554-565 per scene; returns the largest Courant number.
+static double courant_of(mps_handle* h, const double* vmax) {
+    double mx = 0.0;
+    for (int k = 0; k < h->nscene; k++) {
+        const mps_settings& s = h->scenes[k].s;
+        const double cn = (vmax[k] * s.dt) / s.particle_distance;
+        if (cn > s.cfl_condition) h->counts[2]++;
+        h->scourant[k] = cn;
+        mx = (mx < cn) ? cn : mx;
+    }
+    return mx;
 }
 
 // ---------------- C ABI ----------------
@@ -417,109 +500,146 @@ extern "C" {
 
 const char* mps_last_error(const mps_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
 
-int mps_create(const mps_settings* s, int32_t n, const int32_t* id, const int8_t* type, const double* pos,
-               const double* vel, mps_handle** out) {
+static bool valid_settings(const mps_settings* s, std::string& err) {
+    if (s->dim != 2 && s->dim != 3) {
+        err = "dim must be 2 or 3";
+        return false;
+    }
+    if (!(s->re_max > 0) || s->re_max < s->re_pressure || s->re_max < s->re_viscosity || s->re_max < s->re_surface ||
+        !(s->re_pressure > 0) || !(s->re_viscosity > 0) || !(s->re_surface > 0) || !(s->particle_distance > 0) ||
+        !(s->dt > 0) || !(s->density > 0)) {
+        err = "invalid settings (radii, particle distance, dt or density)";
+        return false;
+    }
+    return true;
+}
+
+// Creates a handle for nscene independent scenes packed one after another:
+// scene k holds particles [off_k, off_k + counts[k]) with local ids 0..n_k-1
+// in order; ids[] carries the local ids.
+static int create_impl(int32_t nscene, const mps_settings* sset, const int32_t* counts, const int32_t* id,
+                       const int8_t* type, const double* pos, const double* vel, mps_handle** out) {
     if (!out) {
         g_create_error = "out is NULL";
         return MPS_E_INVALID;
     }
     *out = nullptr;
-    if (!s || n < 0 || (n > 0 && (!id || !type || !pos || !vel))) {
+    if (nscene < 1 || nscene > 32767 || !sset || !counts) {
         g_create_error = "invalid arguments";
         return MPS_E_INVALID;
     }
-    if (s->dim != 2 && s->dim != 3) {
-        g_create_error = "dim must be 2 or 3";
-        return MPS_E_INVALID;
-    }
-    if (!(s->re_max > 0) || s->re_max < s->re_pressure || s->re_max < s->re_viscosity || s->re_max < s->re_surface ||
-        !(s->re_pressure > 0) || !(s->re_viscosity > 0) || !(s->re_surface > 0) || !(s->particle_distance > 0) ||
-        !(s->dt > 0) || !(s->density > 0)) {
-        g_create_error = "invalid settings (radii, particle distance, dt or density)";
-        return MPS_E_INVALID;
-    }
-    for (int32_t i = 0; i < n; i++) {
-        if (id[i] != i) {
-            g_create_error = "particle ids must be 0..N-1 in order";
+    int64_t ntot = 0;
+    for (int k = 0; k < nscene; k++) {
+        if (counts[k] < 0) {
+            g_create_error = "invalid arguments";
             return MPS_E_INVALID;
         }
-        if (type[i] < -1 || type[i] > 2) {
-            g_create_error = "particle type out of range";
+        ntot += counts[k];
+    }
+    if (ntot > 0x7fffffff / 4 || (ntot > 0 && (!id || !type || !pos || !vel))) {
+        g_create_error = "invalid arguments";
+        return MPS_E_INVALID;
+    }
+    const int n = (int) ntot;
+    for (int k = 0; k < nscene; k++) {
+        std::string err;
+        if (!valid_settings(&sset[k], err)) {
+            g_create_error = err;
+            return MPS_E_INVALID;
+        }
+        if (sset[k].dim != sset[0].dim) {
+            g_create_error = "every scene of a handle must have the same dim";
             return MPS_E_INVALID;
         }
     }
     mps_handle* h = new mps_handle();
-    h->s = *s;
     h->n = n;
-    h->dim = s->dim;
-    if (h->dim == 2) {
-        // In 2D every non-ghost particle must sit on one z plane with zero
-        // z velocity and zero z gravity: the reference would ghost any other
-        // particle at its first move (src/mps.cpp:651-653).
-        if (s->gravity[2] != 0.0) {
-            g_create_error = "2D requires gravity z = 0";
+    h->dim = sset[0].dim;
+    h->nscene = nscene;
+    h->scenes.resize(nscene);
+    h->sstats.assign(nscene, mps_solver_stats{});
+    h->scourant.assign(nscene, 0.0);
+    int off = 0, key0 = 0;
+    for (int k = 0; k < nscene; k++) {
+        SceneInfo& sc = h->scenes[k];
+        const mps_settings* s = &sset[k];
+        sc.s = *s;
+        sc.id0 = off;
+        sc.n = counts[k];
+        for (int32_t i = 0; i < sc.n; i++) {
+            const size_t gi = (size_t) off + i;
+            if (id[gi] != i) {
+                g_create_error = "particle ids must be 0..N-1 in order within each scene";
+                delete h;
+                return MPS_E_INVALID;
+            }
+            if (type[gi] < -1 || type[gi] > 2) {
+                g_create_error = "particle type out of range";
+                delete h;
+                return MPS_E_INVALID;
+            }
+        }
+        if (h->dim == 2) {
+            // In 2D every non-ghost particle must sit on one z plane with zero
+            // z velocity and zero z gravity: the reference would ghost any other
+            // particle at its first move (src/mps.cpp:651-653).
+            if (s->gravity[2] != 0.0) {
+                g_create_error = "2D requires gravity z = 0";
+                delete h;
+                return MPS_E_INVALID;
+            }
+            bool have = false;
+            double z0 = 0.0;
+            for (int32_t i = 0; i < sc.n; i++) {
+                const size_t gi = (size_t) off + i;
+                if (type[gi] == MPS_GHOST) continue;
+                if (vel[3 * gi + 2] != 0.0 || (have && pos[3 * gi + 2] != z0)) {
+                    g_create_error = "2D requires every non-ghost particle on one z plane with zero z velocity";
+                    delete h;
+                    return MPS_E_INVALID;
+                }
+                z0 = pos[3 * gi + 2];
+                have = true;
+            }
+            if (have) {
+                const double q = (z0 - s->domain_min[2]) / s->re_max;
+                if (!(q > -1.0 && q < 1e8)) {
+                    g_create_error = "z plane outside the bucket grid";
+                    delete h;
+                    return MPS_E_INVALID;
+                }
+                sc.iz0 = (int) q + 1;
+            }
+        }
+        scene_consts(*s, sc.c);
+        std::string err;
+        if (!scene_grid(*s, h->dim, sc.g, err)) {
+            g_create_error = err;
             delete h;
             return MPS_E_INVALID;
         }
-        bool have = false;
-        double z0 = 0.0;
-        for (int32_t i = 0; i < n; i++) {
-            if (type[i] == MPS_GHOST) continue;
-            if (vel[3 * (size_t) i + 2] != 0.0 || (have && pos[3 * (size_t) i + 2] != z0)) {
-                g_create_error = "2D requires every non-ghost particle on one z plane with zero z velocity";
-                delete h;
-                return MPS_E_INVALID;
-            }
-            z0 = pos[3 * (size_t) i + 2];
-            have = true;
+        if (h->dim == 2 && (sc.iz0 < 1 || sc.iz0 > sc.g.nz - 2)) {
+            g_create_error = "z plane outside the bucket grid";
+            delete h;
+            return MPS_E_INVALID;
         }
-        if (have) {
-            const double q = (z0 - s->domain_min[2]) / s->re_max;
-            if (!(q > -1.0 && q < 1e8)) {
-                g_create_error = "z plane outside the bucket grid";
-                delete h;
-                return MPS_E_INVALID;
-            }
-            h->iz0 = (int) q + 1;
+        sc.g.key0 = key0;
+        if ((int64_t) key0 + sc.g.ncell > 0x7fffffff / 2) {
+            g_create_error = "bucket grids too large";
+            delete h;
+            return MPS_E_INVALID;
         }
+        key0 += sc.g.ncell;
+        off += sc.n;
     }
-    // constants (Settings-derived, src/settings.cpp:43-61; RefValues src/mps.cpp:21-35)
-    Consts& c = h->c;
-    c.dim = s->dim;
-    c.dt = s->dt;
-    c.l0 = s->particle_distance;
-    c.rho = s->density;
-    c.nu = s->kinematic_viscosity;
-    for (int k = 0; k < 3; k++) {
-        c.g[k] = s->gravity[k];
-        c.dmin[k] = s->domain_min[k];
-        c.dmax[k] = s->domain_max[k];
-    }
-    c.re_p = s->re_pressure;
-    c.re_v = s->re_viscosity;
-    c.re_s = s->re_surface;
-    c.re_max = s->re_max;
-    ref_values(s->dim, s->particle_distance, s->re_pressure, &c.n0_p, &c.lam_p);
-    ref_values(s->dim, s->particle_distance, s->re_viscosity, &c.n0_v, &c.lam_v);
-    ref_values(s->dim, s->particle_distance, s->re_surface, &c.n0_s, &c.lam_s);
-    c.beta = s->surface_threshold;
-    c.relax = s->relaxation;
-    c.qc_on = s->qc_on;
-    c.alpha = s->qc_on ? s->compressibility : 0.0;
-    c.hos_on = s->hos_on;
-    c.hos_gamma = s->hos_gamma;
-    c.dc = s->collision_distance;
-    c.restitution = s->restitution;
-    int rc = gridview_build(h);
-    if (rc) {
-        g_create_error = h->err;
-        delete h;
-        return rc;
-    }
-    if (h->dim == 2 && (h->iz0 < 1 || h->iz0 > h->g.nz - 2)) {
-        g_create_error = "z plane outside the bucket grid";
-        delete h;
-        return MPS_E_INVALID;
+    for (auto& sc : h->scenes) sc.g.ghost_key = key0;
+    h->s = h->scenes[0].s;
+    h->c = h->scenes[0].c;
+    h->g = h->scenes[0].g;
+    h->iz0 = h->scenes[0].iz0;
+    if (nscene > 1) {
+        // the launch-level flags cover every scene; kernels read each scene's own
+        for (auto& sc : h->scenes) h->c.hos_on |= sc.c.hos_on;
     }
     hipError_t e = hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -527,23 +647,50 @@ int mps_create(const mps_settings* s, int32_t n, const int32_t* id, const int8_t
         delete h;
         return MPS_E_HIP;
     }
-    rc = alloc_all(h);
+    int rc = alloc_all(h);
     if (rc) {
         g_create_error = h->err;
         free_all(h);
         delete h;
         return rc;
     }
-    // upload (slots = ids until the first build)
+    // scene tables
+    {
+        std::vector<int> k0(nscene + 1);
+        for (int k = 0; k < nscene; k++) k0[k] = h->scenes[k].g.key0;
+        k0[nscene] = key0;
+        e = hipMemcpy(h->d_key0, k0.data(), sizeof(int) * (size_t) (nscene + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess && nscene > 1) {
+            std::vector<GridView> gs(nscene);
+            std::vector<Consts> cs(nscene);
+            for (int k = 0; k < nscene; k++) {
+                gs[k] = h->scenes[k].g;
+                gs[k].cs = h->bb.cs;
+                cs[k] = h->scenes[k].c;
+            }
+            e = hipMemcpy(h->d_g, gs.data(), sizeof(GridView) * (size_t) nscene, hipMemcpyHostToDevice);
+            if (e == hipSuccess)
+                e = hipMemcpy(h->d_c, cs.data(), sizeof(Consts) * (size_t) nscene, hipMemcpyHostToDevice);
+            std::vector<int16_t> sid((size_t) n);
+            for (int k = 0; k < nscene; k++)
+                for (int i = 0; i < h->scenes[k].n; i++) sid[(size_t) h->scenes[k].id0 + i] = (int16_t) k;
+            if (e == hipSuccess && n > 0)
+                e = hipMemcpy(h->d_sid, sid.data(), sizeof(int16_t) * (size_t) n, hipMemcpyHostToDevice);
+        }
+    }
+    h->A.nseg = nscene;
+    h->A.scene_key0 = h->d_key0;
+    h->A.cs = h->bb.cs;
+    h->A.row_of = h->w.row_of;
+    // upload (slots = global ids until the first build)
     const size_t N = (size_t) n;
     std::vector<double> comp(N);
     const int fields = h->dim == 3 ? 3 : 2;
-    for (int f = 0; f < fields; f++) {
+    for (int f = 0; f < fields && e == hipSuccess; f++) {
         for (size_t i = 0; i < N; i++) comp[i] = pos[3 * i + f];
         e = hipMemcpy(h->P[0][f], comp.data(), N * sizeof(double), hipMemcpyHostToDevice);
         for (size_t i = 0; i < N && e == hipSuccess; i++) comp[i] = vel[3 * i + f];
         if (e == hipSuccess) e = hipMemcpy(h->P[0][3 + f], comp.data(), N * sizeof(double), hipMemcpyHostToDevice);
-        if (e != hipSuccess) break;
     }
     if (e == hipSuccess) {
         for (size_t i = 0; i < N; i++) comp[i] = pos[3 * i + 2];
@@ -578,6 +725,31 @@ int mps_create(const mps_settings* s, int32_t n, const int32_t* id, const int8_t
     return MPS_OK;
 }
 
+int mps_create(const mps_settings* s, int32_t n, const int32_t* id, const int8_t* type, const double* pos,
+               const double* vel, mps_handle** out) {
+    if (n < 0) {
+        g_create_error = "invalid arguments";
+        return MPS_E_INVALID;
+    }
+    return create_impl(1, s, &n, id, type, pos, vel, out);
+}
+
+int mps_create_ensemble(int32_t nscene, const mps_settings* s, const int32_t* counts, const int32_t* id,
+                        const int8_t* type, const double* pos, const double* vel, mps_handle** out) {
+    return create_impl(nscene, s, counts, id, type, pos, vel, out);
+}
+
+int mps_scene_count(const mps_handle* h) { return h ? h->nscene : 0; }
+
+int mps_ensemble_stats(const mps_handle* h, mps_solver_stats* out, double* courant) {
+    if (!h) return MPS_E_INVALID;
+    for (int k = 0; k < h->nscene; k++) {
+        if (out) out[k] = h->sstats[k];
+        if (courant) courant[k] = h->scourant[k];
+    }
+    return MPS_OK;
+}
+
 int mps_destroy(mps_handle* h) {
     if (!h) return MPS_OK;
     if (h->st) hipStreamSynchronize(h->st);
@@ -598,7 +770,20 @@ int mps_set_solver(mps_handle* h, const mps_solver_options* o) {
 
 int mps_get_solver_stats(const mps_handle* h, mps_solver_stats* out) {
     if (!h || !out) return MPS_E_INVALID;
-    *out = h->stats;
+    // one scene: its statistics; several: iterations and refinements summed,
+    // residuals the worst over the scenes
+    mps_solver_stats t = h->sstats[0];
+    for (int k = 1; k < h->nscene; k++) {
+        const mps_solver_stats& x = h->sstats[k];
+        t.rows += x.rows;
+        t.iterations += x.iterations;
+        t.refinements += x.refinements;
+        t.converged = t.converged && x.converged;
+        t.rhs_norm = sqrt(t.rhs_norm * t.rhs_norm + x.rhs_norm * x.rhs_norm);
+        t.rel_residual = t.rel_residual > x.rel_residual ? t.rel_residual : x.rel_residual;
+        t.initial_residual = t.initial_residual > x.initial_residual ? t.initial_residual : x.initial_residual;
+    }
+    *out = t;
     return MPS_OK;
 }
 
@@ -615,7 +800,7 @@ int mps_step(mps_handle* h, int32_t is_export, double* courant) {
     if (rc) return rc;
     rc = drain(h);
     if (rc) return rc;
-    const double cn = courant_of(h, h->pinned_d[0]);
+    const double cn = courant_of(h, h->pinned_d);
     if (courant) *courant = cn;
     return MPS_OK;
 }
@@ -626,7 +811,7 @@ int mps_run_steps(mps_handle* h, int32_t n_steps, double* courants) {
         int rc = step_core(h, 0, h->pinned_d);
         if (rc) return rc;
         if ((rc = drain(h))) return rc;
-        const double cn = courant_of(h, h->pinned_d[0]);
+        const double cn = courant_of(h, h->pinned_d);
         if (courants) courants[k] = cn;
     }
     return MPS_OK;
@@ -783,6 +968,7 @@ int mps_bucket_shape(const mps_handle* h, int32_t dims[4], double* length) {
 
 int mps_dump_bucket(mps_handle* h, int32_t* first, int32_t* last, int32_t* next) {
     if (!h || !first || !last || !next) return MPS_E_INVALID;
+    if (h->nscene != 1) return fail(h, MPS_E_INVALID, "mps_dump_bucket takes a single-scene handle");
     int rc = do_build(h);
     if (rc) return rc;
     if ((rc = drain(h))) return rc;
@@ -818,7 +1004,7 @@ int mps_dump_neighbors(mps_handle* h, int32_t* offsets, int32_t* ids, double* di
     Hot c = cur(h);
     int* cnt = h->w.coll_cnt;
     int* off = h->w.coll_off;
-    HIPCHK(h, dump_neighbor_counts(h->dim, g, n, c, cnt, h->s.re_max, h->st));
+    HIPCHK(h, dump_neighbor_counts(h->dim, g, tab(h), n, c, cnt, h->s.re_max, h->st));
     HIPCHK(h, scan_exclusive(cnt, off, n, h->bb.scan_tmp, h->st));
     HIPCHK(h, hipMemcpyAsync(offsets, off, ((size_t) n + 1) * sizeof(int), hipMemcpyDeviceToHost, h->st));
     HIPCHK(h, hipStreamSynchronize(h->st));
@@ -828,7 +1014,7 @@ int mps_dump_neighbors(mps_handle* h, int32_t* offsets, int32_t* ids, double* di
     double* d_dist = nullptr;
     HIPCHK(h, dalloc(&d_ids, tot));
     hipError_t e = dalloc(&d_dist, tot);
-    if (e == hipSuccess) e = dump_neighbor_lists(h->dim, g, n, c, off, d_ids, d_dist, h->s.re_max, h->st);
+    if (e == hipSuccess) e = dump_neighbor_lists(h->dim, g, tab(h), n, c, off, d_ids, d_dist, h->s.re_max, h->st);
     if (e == hipSuccess && ids && tot)
         e = hipMemcpyAsync(ids, d_ids, tot * sizeof(int32_t), hipMemcpyDeviceToHost, h->st);
     if (e == hipSuccess && dists && tot)
@@ -944,7 +1130,7 @@ int mps_autotune(mps_handle* h, int32_t steps_per_candidate, int32_t block_sizes
             h->timing = false;
             if (rc) return rc;
             if ((rc = drain(h))) return rc;
-            courant_of(h, h->pinned_d[0]);
+            courant_of(h, h->pinned_d);
             for (int sidx = 0; sidx < h->nsites; sidx++) {
                 float ms = 0.f;
                 HIPCHK(h, hipEventElapsedTime(&ms, h->tev[sidx][0], h->tev[sidx][1]));

@@ -157,12 +157,15 @@ hipError_t fill_double(double* p, double v, size_t n, hipStream_t st) {
 
 // ---------------- build ----------------
 
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_key_count(GridView g, int n, Hot cur, int* __restrict__ cnt,
-                                                      int* __restrict__ err) {
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_key_count(GridView g0, SceneTab tab, int n, Hot cur,
+                                                             int* __restrict__ cnt, int* __restrict__ err) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    int key = g.ncell;
+    GridView g = g0;
+    Consts c;
+    scene_of<MULTI>(tab, cur.id[s], g, c);
+    int key = g.ghost_key;
     if (cur.type[s] != MPS_GHOST) {
         int ix, iy, iz = 1;
         bool ok = cell_coord(cur.px[s], g.xmin, g.len, g.nx, ix) &&
@@ -170,7 +173,7 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_key_count(GridView g, int n, 
         if (DIM == 3) ok = ok && cell_coord(cur.pz[s], g.zmin, g.len, g.nz, iz);
         if (ok && g.flat && iz != 1) ok = false;
         if (ok) {
-            key = g.flat ? ix + iy * g.nx : ix + iy * g.nx + iz * g.nx * g.ny;
+            key = g.key0 + (g.flat ? ix + iy * g.nx : ix + iy * g.nx + iz * g.nx * g.ny);
         } else {
             atomicOr(err, kErrRange);
         }
@@ -191,7 +194,8 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_bin(int n, Hot cur, int* __re
 }
 
 template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_rank_permute(GridView g, int n, Hot cur, Hot nxt,
+__global__ void __launch_bounds__(kMaxTuneBlock) k_rank_permute(int ghost_key, const int* __restrict__ cs, int n,
+                                                                Hot cur, Hot nxt,
                                                          const int* __restrict__ bin_id,
                                                          const int* __restrict__ dst,
                                                          int* __restrict__ slot_of) {
@@ -200,8 +204,8 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_rank_permute(GridView g, int 
     const int key = cur.key[s];
     const int myid = cur.id[s];
     int t;
-    if (key < g.ncell) {
-        const int b = g.cs[key], e = g.cs[key + 1];
+    if (key != ghost_key) {
+        const int b = cs[key], e = cs[key + 1];
         int r = 0;
         for (int q = b; q < e; q++) r += bin_id[q] < myid;
         t = b + r;
@@ -222,34 +226,46 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_rank_permute(GridView g, int 
     slot_of[myid] = t;
 }
 
-hipError_t build_grid(int dim, const GridView& g, int n, const Hot& cur, Hot& nxt, const BuildBufs& b,
-                      int* err, hipStream_t st, int bs) {
+hipError_t build_grid(int dim, const GridView& g, const SceneTab& tab, int n, const Hot& cur, Hot& nxt,
+                      const BuildBufs& b, int* err, hipStream_t st, int bs) {
     const dim3 grid((n + bs - 1) / bs), blk(bs);
-    if (n == 0) return scan_exclusive(b.cnt, b.cs, g.ncell + 1, b.scan_tmp, st);
-    if (dim == 2)
-        hipLaunchKernelGGL(k_key_count<2>, grid, blk, 0, st, g, n, cur, b.cnt, err);
-    else
-        hipLaunchKernelGGL(k_key_count<3>, grid, blk, 0, st, g, n, cur, b.cnt, err);
-    hipError_t e = scan_exclusive(b.cnt, b.cs, g.ncell + 1, b.scan_tmp, st);
+    const int keys = g.ghost_key + 1;  // every scene's cells and the ghost bin
+    if (n == 0) return scan_exclusive(b.cnt, b.cs, keys, b.scan_tmp, st);
+    const bool multi = tab.g != nullptr;
+    if (dim == 2) {
+        if (multi)
+            hipLaunchKernelGGL((k_key_count<2, true>), grid, blk, 0, st, g, tab, n, cur, b.cnt, err);
+        else
+            hipLaunchKernelGGL((k_key_count<2, false>), grid, blk, 0, st, g, tab, n, cur, b.cnt, err);
+    } else {
+        if (multi)
+            hipLaunchKernelGGL((k_key_count<3, true>), grid, blk, 0, st, g, tab, n, cur, b.cnt, err);
+        else
+            hipLaunchKernelGGL((k_key_count<3, false>), grid, blk, 0, st, g, tab, n, cur, b.cnt, err);
+    }
+    hipError_t e = scan_exclusive(b.cnt, b.cs, keys, b.scan_tmp, st);
     if (e != hipSuccess) return e;
-    int* dst = b.dst;
-    hipLaunchKernelGGL(k_bin, grid, blk, 0, st, n, cur, b.cnt, b.cs, b.bin_id, dst);
-    GridView gv = g;
-    gv.cs = b.cs;
+    hipLaunchKernelGGL(k_bin, grid, blk, 0, st, n, cur, b.cnt, b.cs, b.bin_id, b.dst);
     if (dim == 2)
-        hipLaunchKernelGGL(k_rank_permute<2>, grid, blk, 0, st, gv, n, cur, nxt, b.bin_id, dst, b.slot_of);
+        hipLaunchKernelGGL(k_rank_permute<2>, grid, blk, 0, st, g.ghost_key, b.cs, n, cur, nxt, b.bin_id, b.dst,
+                           b.slot_of);
     else
-        hipLaunchKernelGGL(k_rank_permute<3>, grid, blk, 0, st, gv, n, cur, nxt, b.bin_id, dst, b.slot_of);
+        hipLaunchKernelGGL(k_rank_permute<3>, grid, blk, 0, st, g.ghost_key, b.cs, n, cur, nxt, b.bin_id, b.dst,
+                           b.slot_of);
     return hipGetLastError();
 }
 
 // ---------------- neighbour dumps (test entry points) ----------------
 
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_nb_count(GridView g, int n, Hot h, int* __restrict__ cnt_by_id,
-                                                     double re) {
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_nb_count(GridView g0, SceneTab tab, int n, Hot h,
+                                                            int* __restrict__ cnt_by_id, double re0) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    GridView g = g0;
+    Consts cc;
+    scene_of<MULTI>(tab, h.id[s], g, cc);
+    const double re = MULTI ? cc.re_max : re0;
     int c = 0;
     if (h.type[s] != MPS_GHOST) {
         const double zi = DIM == 3 ? h.pz[s] : 0.0;
@@ -259,12 +275,17 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_nb_count(GridView g, int n, H
     cnt_by_id[h.id[s]] = c;
 }
 
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_nb_list(GridView g, int n, Hot h, const int* __restrict__ off_by_id,
-                                                    int* __restrict__ ids, double* __restrict__ dists, double re) {
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_nb_list(GridView g0, SceneTab tab, int n, Hot h,
+                                                           const int* __restrict__ off_by_id, int* __restrict__ ids,
+                                                           double* __restrict__ dists, double re0) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     if (h.type[s] == MPS_GHOST) return;
+    GridView g = g0;
+    Consts cc;
+    scene_of<MULTI>(tab, h.id[s], g, cc);
+    const double re = MULTI ? cc.re_max : re0;
     int o = off_by_id[h.id[s]];
     const double zi = DIM == 3 ? h.pz[s] : 0.0;
     for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
@@ -275,25 +296,35 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_nb_list(GridView g, int n, Ho
                        });
 }
 
-hipError_t dump_neighbor_counts(int dim, const GridView& g, int n, const Hot& h, int* cnt_by_id, double re,
-                                hipStream_t st) {
+hipError_t dump_neighbor_counts(int dim, const GridView& g, const SceneTab& tab, int n, const Hot& h,
+                                int* cnt_by_id, double re, hipStream_t st) {
     if (n == 0) return hipSuccess;
     const dim3 grid((n + kBlock - 1) / kBlock), blk(kBlock);
-    if (dim == 2)
-        hipLaunchKernelGGL(k_nb_count<2>, grid, blk, 0, st, g, n, h, cnt_by_id, re);
+    const bool m = tab.g != nullptr;
+    if (dim == 2 && m)
+        hipLaunchKernelGGL((k_nb_count<2, true>), grid, blk, 0, st, g, tab, n, h, cnt_by_id, re);
+    else if (dim == 2)
+        hipLaunchKernelGGL((k_nb_count<2, false>), grid, blk, 0, st, g, tab, n, h, cnt_by_id, re);
+    else if (m)
+        hipLaunchKernelGGL((k_nb_count<3, true>), grid, blk, 0, st, g, tab, n, h, cnt_by_id, re);
     else
-        hipLaunchKernelGGL(k_nb_count<3>, grid, blk, 0, st, g, n, h, cnt_by_id, re);
+        hipLaunchKernelGGL((k_nb_count<3, false>), grid, blk, 0, st, g, tab, n, h, cnt_by_id, re);
     return hipGetLastError();
 }
 
-hipError_t dump_neighbor_lists(int dim, const GridView& g, int n, const Hot& h, const int* off_by_id, int* ids,
-                               double* dists, double re, hipStream_t st) {
+hipError_t dump_neighbor_lists(int dim, const GridView& g, const SceneTab& tab, int n, const Hot& h,
+                               const int* off_by_id, int* ids, double* dists, double re, hipStream_t st) {
     if (n == 0) return hipSuccess;
     const dim3 grid((n + kBlock - 1) / kBlock), blk(kBlock);
-    if (dim == 2)
-        hipLaunchKernelGGL(k_nb_list<2>, grid, blk, 0, st, g, n, h, off_by_id, ids, dists, re);
+    const bool m = tab.g != nullptr;
+    if (dim == 2 && m)
+        hipLaunchKernelGGL((k_nb_list<2, true>), grid, blk, 0, st, g, tab, n, h, off_by_id, ids, dists, re);
+    else if (dim == 2)
+        hipLaunchKernelGGL((k_nb_list<2, false>), grid, blk, 0, st, g, tab, n, h, off_by_id, ids, dists, re);
+    else if (m)
+        hipLaunchKernelGGL((k_nb_list<3, true>), grid, blk, 0, st, g, tab, n, h, off_by_id, ids, dists, re);
     else
-        hipLaunchKernelGGL(k_nb_list<3>, grid, blk, 0, st, g, n, h, off_by_id, ids, dists, re);
+        hipLaunchKernelGGL((k_nb_list<3, false>), grid, blk, 0, st, g, tab, n, h, off_by_id, ids, dists, re);
     return hipGetLastError();
 }
 

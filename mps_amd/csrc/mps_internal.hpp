@@ -29,8 +29,10 @@ struct GridView {
     double len;         // bucket side = re_max (src/bucket.cpp:11)
     int nx, ny, nz;     // numX, numY, numZ (src/bucket.cpp:13-16)
     int flat;           // 1: every in-domain particle has iz = 1 (numZ == 3), keys drop z
-    int ncell;          // keys in use: flat ? nx*ny : nx*ny*nz
-    const int* cs;      // ncell + 2 entries: cell starts, then ghost segment end
+    int ncell;          // keys of this scene: flat ? nx*ny : nx*ny*nz
+    int key0;           // first key of this scene in the handle's key space
+    int ghost_key;      // key of the ghost segment (after every scene's cells)
+    const int* cs;      // cell starts over the whole key space (+ ghost segment end)
 };
 
 // Hot particle arrays in slot order (one of two ping-pong sets).
@@ -64,7 +66,33 @@ struct Consts {
     double dc;
     double restitution;
     double dmin[3], dmax[3];
+    double cfl;
+    // coefficients, each computed once in the reference's expression order
+    double avisc;      // nu * (2 D) / (n0_v lambda_v)          src/mps.cpp:182
+    double src_scale;  // gamma * (1 / (dt dt))                  src/mps.cpp:313
+    double hos_scale;  // gamma_hos * (1 / (dt dt))              src/mps.cpp:291
+    double hos_coef;   // 1 / (n0 dt)                            src/mps.cpp:304
+    double coefA;      // 2 D / (lambda n0)                      src/mps.cpp:343
+    double qc_add;     // alpha / (dt dt)                        src/mps.cpp:354
+    double gscale;     // D / n0                                 src/mps.cpp:530
 };
+
+// Per-scene lookup for handles holding several independent scenes packed in
+// one particle array (ensembles).  Single-scene kernels ignore it.
+struct SceneTab {
+    const GridView* g;      // per scene
+    const Consts* c;        // per scene
+    const int16_t* sid;     // scene of each particle id
+};
+
+template <bool MULTI>
+__device__ __forceinline__ void scene_of(const SceneTab& t, int id, GridView& g, Consts& c) {
+    if constexpr (MULTI) {
+        const int si = t.sid[id];
+        g = t.g[si];
+        c = t.c[si];
+    }
+}
 
 // Device error flags (bit mask written with atomicOr).
 constexpr int kErrRange = 1;     // particle outside the bucket grid
@@ -109,6 +137,8 @@ __device__ __forceinline__ void for_neighbors(const GridView& g, const double* _
                                               const double* __restrict__ py,
                                               const double* __restrict__ pz, int key, int s,
                                               double xi, double yi, double zi, double re, F&& f) {
+    key -= g.key0;  // key inside this scene's grid
+    const int* __restrict__ cs = g.cs + g.key0;
     if constexpr (DIM == 2) {
         const int ix = key % g.nx;
         const int iy = key / g.nx;
@@ -117,8 +147,8 @@ __device__ __forceinline__ void for_neighbors(const GridView& g, const double* _
             for (int jy = iy - 1; jy <= iy + 1; jy++) {
                 if (jy < 0 || jy >= g.ny) continue;
                 const int k = jx + jy * g.nx;
-                const int e = g.cs[k + 1];
-                for (int q = g.cs[k]; q < e; q++) {
+                const int e = cs[k + 1];
+                for (int q = cs[k]; q < e; q++) {
                     const double dx = px[q] - xi;
                     const double dy = py[q] - yi;
                     const double dist = sqrt(dx * dx + dy * dy);
@@ -148,8 +178,8 @@ __device__ __forceinline__ void for_neighbors(const GridView& g, const double* _
                         continue;
                     }
                     const int k = g.flat ? (jx + jy * g.nx) : (jx + jy * g.nx + jz * g.nx * g.ny);
-                    const int e = g.cs[k + 1];
-                    for (int q = g.cs[k]; q < e; q++) {
+                    const int e = cs[k + 1];
+                    for (int q = cs[k]; q < e; q++) {
                         const double dx = px[q] - xi;
                         const double dy = py[q] - yi;
                         const double dz = pz[q] - zi;
@@ -178,12 +208,12 @@ struct BuildBufs {
     int* scan_tmp;
     int* slot_of;    // n: id -> slot
 };
-hipError_t build_grid(int dim, const GridView& g, int n, const Hot& cur, Hot& nxt,
+hipError_t build_grid(int dim, const GridView& g, const SceneTab& tab, int n, const Hot& cur, Hot& nxt,
                       const BuildBufs& b, int* err, hipStream_t st, int bs);
-hipError_t dump_neighbor_counts(int dim, const GridView& g, int n, const Hot& h, int* cnt_by_id,
-                                double re, hipStream_t st);
-hipError_t dump_neighbor_lists(int dim, const GridView& g, int n, const Hot& h, const int* off_by_id,
-                               int* ids, double* dists, double re, hipStream_t st);
+hipError_t dump_neighbor_counts(int dim, const GridView& g, const SceneTab& tab, int n, const Hot& h,
+                                int* cnt_by_id, double re, hipStream_t st);
+hipError_t dump_neighbor_lists(int dim, const GridView& g, const SceneTab& tab, int n, const Hot& h,
+                               const int* off_by_id, int* ids, double* dists, double re, hipStream_t st);
 
 // stages.hip
 struct Cold {
@@ -219,21 +249,22 @@ struct StepScratch {
     int* scan_tmp;
 };
 
-hipError_t launch_visc_move(int dim, const GridView& g, int n, const Consts& c, const Hot& cur,
+hipError_t launch_visc_move(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& cur,
                             Hot& out, Cold& cold, Events& ev, hipStream_t st, int bs);
-hipError_t launch_collision(int dim, const GridView& g, int n, const Consts& c, Hot& h, StepScratch& w,
-                            int* err, hipStream_t st, int bs);
-hipError_t launch_bc_source(int dim, const GridView& g, int n, const Consts& c, const Hot& h, Cold& cold,
-                            StepScratch& w, hipStream_t st, int bs);
-hipError_t launch_diag_cc(int dim, const GridView& g, int n, const Consts& c, const Hot& h, StepScratch& w,
-                          Events& ev, hipStream_t st, int bs);
-hipError_t launch_pressure_out(int dim, const GridView& g, int n, const Consts& c, const Hot& h,
+hipError_t launch_collision(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, Hot& h,
+                            StepScratch& w, int* err, hipStream_t st, int bs);
+hipError_t launch_bc_source(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
+                            Cold& cold, StepScratch& w, hipStream_t st, int bs);
+hipError_t launch_diag_cc(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
+                          StepScratch& w, Events& ev, hipStream_t st, int bs);
+hipError_t launch_pressure_out(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
                                const double* x_s, Cold& cold, StepScratch& w, hipStream_t st, int bs);
-hipError_t launch_grad_move(int dim, const GridView& g, int n, const Consts& c, const Hot& cur, Hot& out,
-                            Cold& cold, StepScratch& w, Events& ev, hipStream_t st, int bs);
-hipError_t launch_ndr(int dim, const GridView& g, int n, const Consts& c, const Hot& h, Cold& cold,
-                      hipStream_t st, int bs);
-hipError_t launch_courant(int dim, int n, const Hot& h, double* vmax, hipStream_t st);
+hipError_t launch_grad_move(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& cur,
+                            Hot& out, Cold& cold, StepScratch& w, Events& ev, hipStream_t st, int bs);
+hipError_t launch_ndr(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
+                      Cold& cold, hipStream_t st, int bs);
+hipError_t launch_courant(int dim, int n, const Hot& h, const SceneTab& tab, int nscene, double* vmax,
+                          hipStream_t st);
 hipError_t launch_snapshot(int dim, int n, const Hot& h, const Cold& cold, double* pos_id, double* vel_id,
                            hipStream_t st);
 
@@ -241,6 +272,10 @@ hipError_t launch_snapshot(int dim, int n, const Hot& h, const Cold& cold, doubl
 struct Sell {
     int rows;            // Inner rows
     const int* rows_dev; // Inner rows, on the device
+    int nseg;            // independent segments (scenes)
+    const int* scene_key0;  // device, nseg + 1: first key of each scene (last = ghost key)
+    const int* cs;       // device: cell starts
+    const int* row_of;   // device: slot -> compact row
     int nslices;
     int* slice_off;      // nslices + 1 (in entries)
     int* slice_w;        // nslices (max nnz of the slice)
@@ -261,7 +296,7 @@ struct Sell {
     size_t raw_cap;
 };
 
-hipError_t assemble_sell(int dim, const GridView& g, int n, const Consts& c, const Hot& h,
+hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
                          const StepScratch& w, Sell& A, int want_val32, int want_raw, int* err,
                          int* host_rows, size_t* host_entries, hipStream_t st, int bs);
 
@@ -271,8 +306,9 @@ void cg_destroy(CgWork* cw);
 hipError_t cg_reserve(CgWork* cw, int rows);
 // Solves A x = b for the Inner rows.  b_s / x_s in slot order; x0_id (id order,
 // may be NULL) gives the warm start.  Fills stats.
+// stats: one entry per segment.
 hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double* b_s, const double* pprev_id,
-                    double* x_s, const mps_solver_options& o, mps_solver_stats& stats, hipStream_t st,
+                    double* x_s, const mps_solver_options& o, mps_solver_stats* stats, hipStream_t st,
                     std::string& msg);
 hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, hipStream_t st, double ms[4],
                            long long bytes[4]);

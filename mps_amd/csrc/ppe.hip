@@ -72,8 +72,9 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_rows(int n, const int* __rest
     atomicMax(&slice_w[r >> 6], nnz_s[s]);
 }
 
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_fill_sell(GridView g, Consts c, Hot h, const int* __restrict__ rows_p,
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_fill_sell(GridView g0, Consts c0, SceneTab tab, Hot h,
+                                                             const int* __restrict__ rows_p,
                                                       const int* __restrict__ row_slot, const int* __restrict__ row_of,
                                                       const int8_t* __restrict__ bc_s,
                                                       const double* __restrict__ diag_s,
@@ -82,10 +83,13 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_fill_sell(GridView g, Consts 
                                                       double* __restrict__ val64, float* __restrict__ val32,
                                                       double* __restrict__ sdiag, double* __restrict__ dgl,
                                                       const int* __restrict__ raw_off, int* __restrict__ raw_col,
-                                                      double* __restrict__ raw_val, double coefA) {
+                                                      double* __restrict__ raw_val) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= *rows_p) return;
     const int s = row_slot[r];
+    GridView g = g0;
+    Consts c = c0;
+    scene_of<MULTI>(tab, h.id[s], g, c);
     const double di = diag_s[s];
     const double si = 1.0 / sqrt(di);
     sdiag[r] = si;
@@ -102,7 +106,7 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_fill_sell(GridView g, Consts 
                        [&](int q, double dist, double, double, double) {
                            if (bc_s[q] != MPS_BC_INNER) return;
                            // the off-diagonal value of src/mps.cpp:343-348
-                           const double a = coefA * wfun(dist, re) / c.rho;
+                           const double a = c.coefA * wfun(dist, re) / c.rho;
                            const double v = -a;
                            const double sj = 1.0 / sqrt(diag_s[q]);
                            const double sv = v * (si * sj);
@@ -132,7 +136,8 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_fill_sell(GridView g, Consts 
     if (md > 0) atomicMax(maxdelta, md);
 }
 
-hipError_t assemble_sell(int dim, const GridView& g, int n, const Consts& c, const Hot& h, const StepScratch& w,
+hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
+                         const StepScratch& w,
                          Sell& A, int want_val32, int want_raw, int* err, int* host_rows, size_t* host_entries,
                          hipStream_t st, int bs) {
     (void) err;
@@ -200,20 +205,22 @@ hipError_t assemble_sell(int dim, const GridView& g, int n, const Consts& c, con
     }
     A.use16 = 0;
     if (rows == 0) return hipSuccess;
-    const double D = (double) c.dim;
-    const double coefA = (2.0 * D / (c.lam_p * c.n0_p));
     const int* rows_p = w.row_of + n;
     if ((e = fill_int(A.maxdelta, 0, 1, st)) != hipSuccess) return e;
-    if (dim == 2)
-        hipLaunchKernelGGL(k_fill_sell<2>, dim3((rows + bs - 1) / bs), dim3(bs), 0, st, g, c, h, rows_p, A.row_slot, w.row_of,
-                           w.bc_s, w.diag_s, A.slice_off, A.col, A.dcol, A.maxdelta, A.val64, want_val32 ? A.val32 : nullptr, A.sdiag,
-                           A.dgl, want_raw ? A.raw_off : nullptr, want_raw ? A.raw_col : nullptr,
-                           want_raw ? A.raw_val : nullptr, coefA);
+    const dim3 fg((rows + bs - 1) / bs), fb(bs);
+#define FILL_ARGS                                                                                                  \
+    g, c, tab, h, rows_p, A.row_slot, w.row_of, w.bc_s, w.diag_s, A.slice_off, A.col, A.dcol, A.maxdelta, A.val64, \
+        want_val32 ? A.val32 : nullptr, A.sdiag, A.dgl, want_raw ? A.raw_off : nullptr,                           \
+        want_raw ? A.raw_col : nullptr, want_raw ? A.raw_val : nullptr
+    if (dim == 2 && tab.g)
+        hipLaunchKernelGGL((k_fill_sell<2, true>), fg, fb, 0, st, FILL_ARGS);
+    else if (dim == 2)
+        hipLaunchKernelGGL((k_fill_sell<2, false>), fg, fb, 0, st, FILL_ARGS);
+    else if (tab.g)
+        hipLaunchKernelGGL((k_fill_sell<3, true>), fg, fb, 0, st, FILL_ARGS);
     else
-        hipLaunchKernelGGL(k_fill_sell<3>, dim3((rows + bs - 1) / bs), dim3(bs), 0, st, g, c, h, rows_p, A.row_slot, w.row_of,
-                           w.bc_s, w.diag_s, A.slice_off, A.col, A.dcol, A.maxdelta, A.val64, want_val32 ? A.val32 : nullptr, A.sdiag,
-                           A.dgl, want_raw ? A.raw_off : nullptr, want_raw ? A.raw_col : nullptr,
-                           want_raw ? A.raw_val : nullptr, coefA);
+        hipLaunchKernelGGL((k_fill_sell<3, false>), fg, fb, 0, st, FILL_ARGS);
+#undef FILL_ARGS
     // 16-bit column offsets when every row's neighbours lie within +-32767 rows
     int md = 0;
     if ((e = hipMemcpyAsync(&md, A.maxdelta, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
@@ -224,13 +231,18 @@ hipError_t assemble_sell(int dim, const GridView& g, int n, const Consts& c, con
 
 // ---------------- CG ----------------
 //
-// Every CG kernel launches the same grid of nb <= kMaxBlocks blocks; block b
-// owns a contiguous range of rows (a multiple of 64, so whole SELL slices)
-// and writes its partial sums to part[b * NV + k].  A kernel that needs the
-// sums starts by reducing all nb partials in one fixed order, redundantly in
-// every block: the kernel boundary makes the partials visible on every XCD,
-// no fence or atomic is needed, and every block derives the same scalars bit
-// for bit, so results are deterministic.
+// The system holds one or more independent segments (one per scene of the
+// handle); a segment is a contiguous range of rows with its own scalars,
+// stopping test and done flag.  Segment s is solved by nb_s = cg_blocks(rows_s)
+// consecutive blocks of every CG launch; block v of the segment owns a
+// contiguous range of its rows (a multiple of 64) and writes its partial sums
+// to part[(first block of s + v) * NV + k].  A kernel that needs the sums
+// starts by reducing the segment's partials in one fixed order, redundantly
+// in every block of the segment: the kernel boundary makes them visible on
+// every XCD, no fence or atomic is needed, and every block derives the same
+// scalars bit for bit.  A segment's arithmetic is the same whether it is
+// solved alone or next to others, so an ensemble member reproduces its solo
+// run exactly.
 //
 // Iteration (Chronopoulos-Gear form of CG, one reduction point):
 //   apply : w = Ahat r; partials of r.r, w.r and the stopping norm
@@ -238,7 +250,7 @@ hipError_t assemble_sell(int dim, const GridView& g, int n, const Consts& c, con
 //           p = r + beta p; s = w + beta s; y += alpha p; r -= alpha s
 // apply gathers only r, once per stored entry; step is elementwise.
 
-constexpr int kMaxBlocks = 1024;
+constexpr int kMaxBlocks = 1024;  // per segment
 
 static int cg_blocks(int rows) {
     int nb = (rows + kBlock - 1) / kBlock;
@@ -259,6 +271,48 @@ struct CgScalars {
     int frozen;        // timing mode: kernels leave every scalar as it is
     int pad;
 };
+
+struct Seg {
+    int nseg;
+    const int* blk_off;  // nseg + 1: first block of each segment
+    const int* row_off;  // nseg + 1: first row of each segment
+};
+
+__device__ __forceinline__ int seg_of_block(const Seg& sg, int b) {
+    int lo = 0, hi = sg.nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sg.blk_off[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+struct BlockRows {
+    int seg, v, nb, b0;  // segment, block within it, its block count, its first block
+    int r0, r1;          // rows of this block
+};
+
+// Row range of this block: contiguous within the segment, whole multiples of
+// 64 rows from the segment start, XCD-grouped.
+__device__ __forceinline__ BlockRows block_rows(const Seg& sg) {
+    BlockRows br;
+    br.seg = seg_of_block(sg, blockIdx.x);
+    br.b0 = sg.blk_off[br.seg];
+    br.v = blockIdx.x - br.b0;
+    br.nb = sg.blk_off[br.seg + 1] - br.b0;
+    const int base = sg.row_off[br.seg];
+    const int rows = sg.row_off[br.seg + 1] - base;
+    const int per = ((rows + br.nb - 1) / br.nb + 63) & ~63;
+    const int lb = xcd_block(br.v, br.nb);
+    br.r0 = base + lb * per;
+    const int end = base + rows;
+    br.r1 = br.r0 + per < end ? br.r0 + per : end;
+    if (br.r0 > end) br.r0 = end;
+    return br;
+}
 
 // Fixed-order sum of NV interleaved partials (part[b*NV + k], b < nb) in every
 // block; result broadcast to all threads.
@@ -296,15 +350,6 @@ __device__ __forceinline__ void write_partials(const double (&v)[NV], double* __
         for (int w = 0; w < (int) (blockDim.x >> 6); w++) t += red[w * NV + threadIdx.x];
         part[(size_t) NV * blockIdx.x + threadIdx.x] = t;
     }
-}
-
-// Row range of this block: contiguous, whole slices, XCD-grouped.
-__device__ __forceinline__ void block_rows(int rows, int& r0, int& r1) {
-    const int nb = gridDim.x;
-    const int per = ((rows + nb - 1) / nb + 63) & ~63;
-    const int lb = xcd_block(blockIdx.x, nb);
-    r0 = lb * per;
-    r1 = r0 + per < rows ? r0 + per : rows;
 }
 
 __device__ __forceinline__ int block_uniform_done(CgScalars* sc) {
@@ -352,19 +397,19 @@ __device__ __forceinline__ T sell_row(const int* __restrict__ slice_pos, const I
 
 // apply: w = Ahat r (unit diagonal); partials r.r, w.r, stopping norm.
 template <typename T, typename IDX, bool WEIGHTED>
-__global__ void __launch_bounds__(kBlock) k_cg_apply(const int* __restrict__ rows_p, const int* __restrict__ slice_pos,
+__global__ void __launch_bounds__(kBlock) k_cg_apply(Seg sg, const int* __restrict__ slice_pos,
                                                      const IDX* __restrict__ col, const T* __restrict__ val,
                                                      const T* __restrict__ r, T* __restrict__ w,
                                                      const double* __restrict__ dgl, double* __restrict__ part,
-                                                     CgScalars* sc) {
+                                                     CgScalars* scs) {
 #pragma clang fp contract(fast)
     __shared__ double red[3 * kBlock / 64];
+    const BlockRows br = block_rows(sg);
+    CgScalars* sc = scs + br.seg;
     if (block_uniform_done(sc)) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && !sc->frozen) sc->iter += 1;
-    int r0, r1;
-    block_rows(*rows_p, r0, r1);
+    if (br.v == 0 && threadIdx.x == 0 && !sc->frozen) sc->iter += 1;
     double g = 0.0, d = 0.0, rw = 0.0;
-    for (int row = r0 + (int) threadIdx.x; row < r1; row += kBlock) {
+    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) {
         const T rr = r[row];
         const T acc = sell_row<T, IDX>(slice_pos, col, val, r, row, rr);
         w[row] = acc;
@@ -379,17 +424,18 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(const int* __restrict__ row
 
 // step: convergence test on r, then the vector recurrences.
 template <typename T, bool WEIGHTED>
-__global__ void __launch_bounds__(kBlock) k_cg_step(const int* __restrict__ rows_p, T* __restrict__ y,
-                                                    T* __restrict__ r, T* __restrict__ p, T* __restrict__ s,
-                                                    const T* __restrict__ w, const double* __restrict__ part,
-                                                    CgScalars* sc, int par) {
+__global__ void __launch_bounds__(kBlock) k_cg_step(Seg sg, T* __restrict__ y, T* __restrict__ r, T* __restrict__ p,
+                                                    T* __restrict__ s, const T* __restrict__ w,
+                                                    const double* __restrict__ part, CgScalars* scs, int par) {
 #pragma clang fp contract(fast)
     __shared__ double red[3 * kBlock / 64];
+    const BlockRows br = block_rows(sg);
+    CgScalars* sc = scs + br.seg;
     if (block_uniform_done(sc)) return;
     double sums[3];
-    reduce_partials<3>(part, gridDim.x, sums, red);
+    reduce_partials<3>(part + (size_t) 3 * br.b0, br.nb, sums, red);
     const double g = sums[0], dd = sums[1], rho = sums[2];
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const bool lead = br.v == 0 && threadIdx.x == 0;
     const int it = sc->iter;  // this iteration's apply has run: it = k + 1
     const bool first = it == 1;
     T alpha, beta;
@@ -417,9 +463,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_step(const int* __restrict__ rows
         alpha = (T) a;
         beta = (T) b;
     }
-    int r0, r1;
-    block_rows(*rows_p, r0, r1);
-    for (int row = r0 + (int) threadIdx.x; row < r1; row += kBlock) {
+    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) {
         const T rr = r[row], ww = w[row];
         const T pp = first ? rr : rr + beta * p[row];
         const T ss = first ? ww : ww + beta * s[row];
@@ -430,27 +474,28 @@ __global__ void __launch_bounds__(kBlock) k_cg_step(const int* __restrict__ rows
     }
 }
 
-// One block: sums nv-interleaved partials into sc->red[0..nv-1].
+// One block per segment: sums its NV-interleaved partials into sc[seg].red.
 template <int NV>
-__global__ void __launch_bounds__(kBlock) k_reduce_parts(const double* __restrict__ part, int nb, CgScalars* sc) {
+__global__ void __launch_bounds__(kBlock) k_reduce_parts(Seg sg, const double* __restrict__ part, CgScalars* scs) {
     __shared__ double red[3 * kBlock / 64];
+    const int sgi = blockIdx.x;
+    const int b0 = sg.blk_off[sgi], nb = sg.blk_off[sgi + 1] - b0;
     double o[NV];
-    reduce_partials<NV>(part, nb, o, red);
+    reduce_partials<NV>(part + (size_t) NV * b0, nb, o, red);
     if (threadIdx.x == 0)
-        for (int k = 0; k < NV; k++) sc->red[k] = o[k];
+        for (int k = 0; k < NV; k++) scs[sgi].red[k] = o[k];
 }
 
 // FP64 init: yhat = x0 sqrt(d), bhat = S b; partial ||b||^2.
-__global__ void __launch_bounds__(kBlock) k_cg_init(const int* __restrict__ rows_p, const int* __restrict__ row_slot,
+__global__ void __launch_bounds__(kBlock) k_cg_init(Seg sg, const int* __restrict__ row_slot,
                                                     const int32_t* __restrict__ id, const double* __restrict__ b_s,
                                                     const double* __restrict__ pprev, const double* __restrict__ sdiag,
                                                     double* __restrict__ y, double* __restrict__ bh,
                                                     double* __restrict__ part) {
     __shared__ double red[3 * kBlock / 64];
-    int r0, r1;
-    block_rows(*rows_p, r0, r1);
+    const BlockRows br = block_rows(sg);
     double bb = 0.0;
-    for (int row = r0 + (int) threadIdx.x; row < r1; row += kBlock) {
+    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) {
         const int s = row_slot[row];
         const double b = b_s[s];
         const double si = sdiag[row];
@@ -464,17 +509,16 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const int* __restrict__ rows
 
 // FP64 residual r = bhat - Ahat y (unit diagonal); partials r.r, sum d r^2.
 template <typename IDX>
-__global__ void __launch_bounds__(kBlock) k_residual64(const int* __restrict__ rows_p, const int* __restrict__ slice_pos,
+__global__ void __launch_bounds__(kBlock) k_residual64(Seg sg, const int* __restrict__ slice_pos,
                                                        const IDX* __restrict__ col, const double* __restrict__ val,
                                                        const double* __restrict__ y, const double* __restrict__ bh,
                                                        const double* __restrict__ dgl, double* __restrict__ r,
                                                        double* __restrict__ part) {
 #pragma clang fp contract(fast)
     __shared__ double red[3 * kBlock / 64];
-    int r0, r1;
-    block_rows(*rows_p, r0, r1);
+    const BlockRows br = block_rows(sg);
     double rr = 0.0, rw = 0.0;
-    for (int row = r0 + (int) threadIdx.x; row < r1; row += kBlock) {
+    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) {
         const double acc = sell_row<double, IDX>(slice_pos, col, val, y, row, y[row]);
         const double rv = bh[row] - acc;
         r[row] = rv;
@@ -486,14 +530,13 @@ __global__ void __launch_bounds__(kBlock) k_residual64(const int* __restrict__ r
 }
 
 // Inner solve setup: r32 = r64 (scaled residual), e = 0; partial r.r.
-__global__ void __launch_bounds__(kBlock) k_inner_init(const int* __restrict__ rows_p, const double* __restrict__ r64,
+__global__ void __launch_bounds__(kBlock) k_inner_init(Seg sg, const double* __restrict__ r64,
                                                        float* __restrict__ r32, float* __restrict__ e32,
                                                        double* __restrict__ part) {
     __shared__ double red[3 * kBlock / 64];
-    int r0, r1;
-    block_rows(*rows_p, r0, r1);
+    const BlockRows br = block_rows(sg);
     double rr = 0.0;
-    for (int row = r0 + (int) threadIdx.x; row < r1; row += kBlock) {
+    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) {
         const float rv = (float) r64[row];
         r32[row] = rv;
         e32[row] = 0.0f;
@@ -503,24 +546,30 @@ __global__ void __launch_bounds__(kBlock) k_inner_init(const int* __restrict__ r
     write_partials<1>(v, part, red);
 }
 
-__global__ void __launch_bounds__(kBlock) k_add_correction(const int* __restrict__ rows_p, double* __restrict__ y,
-                                                           const float* __restrict__ e) {
-    int r0, r1;
-    block_rows(*rows_p, r0, r1);
-    for (int row = r0 + (int) threadIdx.x; row < r1; row += kBlock) y[row] += (double) e[row];
+__global__ void __launch_bounds__(kBlock) k_add_correction(Seg sg, double* __restrict__ y, const float* __restrict__ e) {
+    const BlockRows br = block_rows(sg);
+    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) y[row] += (double) e[row];
 }
 
-__global__ void __launch_bounds__(kBlock) k_cg_finish(const int* __restrict__ rows_p, const int* __restrict__ row_slot,
+__global__ void __launch_bounds__(kBlock) k_cg_finish(Seg sg, const int* __restrict__ row_slot,
                                                       const double* __restrict__ y, const double* __restrict__ sdiag,
                                                       double* __restrict__ x_s) {
-    int r0, r1;
-    block_rows(*rows_p, r0, r1);
-    for (int row = r0 + (int) threadIdx.x; row < r1; row += kBlock) x_s[row_slot[row]] = y[row] * sdiag[row];
+    const BlockRows br = block_rows(sg);
+    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) x_s[row_slot[row]] = y[row] * sdiag[row];
+}
+
+// Per-segment first rows: row_off[s] = compact row of the first slot of scene s.
+__global__ void k_seg_rows(int nseg, const int* __restrict__ scene_key0, const int* __restrict__ cs,
+                           const int* __restrict__ row_of, int* __restrict__ row_off) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s <= nseg) row_off[s] = row_of[cs[scene_key0[s]]];
 }
 
 struct CgWork {
     int cap = 0;
-    int nb = 1;               // blocks of every CG launch (fixed for the capacity)
+    int nseg_cap = 0;
+    int nseg = 1;
+    int total_blocks = 1;     // sum of the segments' block counts
     double* y = nullptr;      // FP64 solution (scaled)
     double* bh = nullptr;     // scaled rhs
     double* r64 = nullptr;    // FP64 residual (scaled)
@@ -533,35 +582,64 @@ struct CgWork {
     float* w32 = nullptr;
     float* e32 = nullptr;     // inner correction
     double* part = nullptr;   // block partials (3 per block)
-    CgScalars* sc = nullptr;  // device scalars
+    size_t part_cap = 0;
+    CgScalars* sc = nullptr;  // device scalars, one per segment
     CgScalars* host = nullptr;
+    int* blk_off = nullptr;   // device, nseg + 1
+    int* row_off = nullptr;   // device, nseg + 1
+    int* h_blk = nullptr;     // pinned host mirrors
+    int* h_row = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // chunk graphs: FP64 CG, FP32 inner CG
     int graph_chunk[2] = {0, 0};
     const void* graph_key[2] = {nullptr, nullptr};
     int graph_idx16[2] = {-1, -1};
-    int graph_nb[2] = {0, 0};
+    int graph_blocks[2] = {0, 0};
+    Seg seg() const { return Seg{nseg, blk_off, row_off}; }
 };
 
 CgWork* cg_create() { return new CgWork(); }
 
 static void cg_free_vectors(CgWork* cw) {
-    void* ptrs[] = {cw->y, cw->bh, cw->r64, cw->p64, cw->s64, cw->w64, cw->r32, cw->p32, cw->s32, cw->w32, cw->e32,
-                    cw->part};
+    void* ptrs[] = {cw->y, cw->bh, cw->r64, cw->p64, cw->s64, cw->w64, cw->r32, cw->p32, cw->s32, cw->w32, cw->e32};
     for (void* p : ptrs) hipFree(p);
 }
 
 void cg_destroy(CgWork* cw) {
     if (!cw) return;
     cg_free_vectors(cw);
+    hipFree(cw->part);
     hipFree(cw->sc);
+    hipFree(cw->blk_off);
+    hipFree(cw->row_off);
     hipHostFree(cw->host);
+    hipHostFree(cw->h_blk);
+    hipHostFree(cw->h_row);
     for (int k = 0; k < 2; k++)
         if (cw->graph[k]) hipGraphExecDestroy(cw->graph[k]);
     delete cw;
 }
 
+static hipError_t cg_reserve_segs(CgWork* cw, int nseg) {
+    if (nseg <= cw->nseg_cap) return hipSuccess;
+    hipFree(cw->sc);
+    hipFree(cw->blk_off);
+    hipFree(cw->row_off);
+    hipHostFree(cw->host);
+    hipHostFree(cw->h_blk);
+    hipHostFree(cw->h_row);
+    hipError_t e;
+    if ((e = hipMalloc(&cw->sc, sizeof(CgScalars) * (size_t) nseg))) return e;
+    if ((e = hipMalloc(&cw->blk_off, sizeof(int) * (size_t) (nseg + 1)))) return e;
+    if ((e = hipMalloc(&cw->row_off, sizeof(int) * (size_t) (nseg + 1)))) return e;
+    if ((e = hipHostMalloc(&cw->host, sizeof(CgScalars) * (size_t) nseg, hipHostMallocDefault))) return e;
+    if ((e = hipHostMalloc(&cw->h_blk, sizeof(int) * (size_t) (nseg + 1), hipHostMallocDefault))) return e;
+    if ((e = hipHostMalloc(&cw->h_row, sizeof(int) * (size_t) (nseg + 1), hipHostMallocDefault))) return e;
+    cw->nseg_cap = nseg;
+    return hipSuccess;
+}
+
 hipError_t cg_reserve(CgWork* cw, int rows) {
-    if (rows <= cw->cap && cw->sc) return hipSuccess;
+    if (rows <= cw->cap && cw->y) return hipSuccess;
     const int cap = rows + rows / 8 + 1024;
     cg_free_vectors(cw);
     hipError_t e;
@@ -571,12 +649,6 @@ hipError_t cg_reserve(CgWork* cw, int rows) {
         (e = hipMalloc(&cw->r32, f)) || (e = hipMalloc(&cw->p32, f)) || (e = hipMalloc(&cw->s32, f)) ||
         (e = hipMalloc(&cw->w32, f)) || (e = hipMalloc(&cw->e32, f)))
         return e;
-    cw->nb = cg_blocks(rows);
-    if ((e = hipMalloc(&cw->part, sizeof(double) * 3 * (size_t) kMaxBlocks))) return e;
-    if (!cw->sc) {
-        if ((e = hipMalloc(&cw->sc, sizeof(CgScalars)))) return e;
-        if ((e = hipHostMalloc(&cw->host, sizeof(CgScalars), hipHostMallocDefault))) return e;
-    }
     cw->cap = cap;
     for (int k = 0; k < 2; k++) {
         if (cw->graph[k]) hipGraphExecDestroy(cw->graph[k]);
@@ -585,18 +657,59 @@ hipError_t cg_reserve(CgWork* cw, int rows) {
     return hipSuccess;
 }
 
+// Segment layout of the current system: per-segment rows from the device,
+// block counts from the live row counts, partial storage sized to match.
+static hipError_t cg_layout(CgWork* cw, const Sell& A, hipStream_t st) {
+    hipError_t e = cg_reserve_segs(cw, A.nseg);
+    if (e) return e;
+    cw->nseg = A.nseg;
+    if (A.nseg == 1) {
+        cw->h_row[0] = 0;
+        cw->h_row[1] = A.rows;
+    } else {
+        hipLaunchKernelGGL(k_seg_rows, dim3((A.nseg + 1 + 255) / 256), dim3(256), 0, st, A.nseg, A.scene_key0, A.cs,
+                           A.row_of, cw->row_off);
+        if ((e = hipMemcpyAsync(cw->h_row, cw->row_off, sizeof(int) * (size_t) (A.nseg + 1), hipMemcpyDeviceToHost,
+                                st)))
+            return e;
+        if ((e = hipStreamSynchronize(st))) return e;
+    }
+    cw->h_blk[0] = 0;
+    for (int s = 0; s < A.nseg; s++) cw->h_blk[s + 1] = cw->h_blk[s] + cg_blocks(cw->h_row[s + 1] - cw->h_row[s]);
+    cw->total_blocks = cw->h_blk[A.nseg];
+    const size_t need = (size_t) 3 * cw->total_blocks;
+    if (need > cw->part_cap) {
+        hipFree(cw->part);
+        cw->part = nullptr;
+        if ((e = hipMalloc(&cw->part, sizeof(double) * (need + 3 * 1024)))) return e;
+        cw->part_cap = need + 3 * 1024;
+    }
+    if ((e = hipMemcpyAsync(cw->blk_off, cw->h_blk, sizeof(int) * (size_t) (A.nseg + 1), hipMemcpyHostToDevice, st)))
+        return e;
+    if (A.nseg == 1 &&
+        (e = hipMemcpyAsync(cw->row_off, cw->h_row, sizeof(int) * 2, hipMemcpyHostToDevice, st)))
+        return e;
+    return hipSuccess;
+}
+
 // Enqueue `chunk` CG iterations (two kernels each); chunk is even, so the
 // scalar parities line up from one chunk to the next.
 template <typename T, typename IDX, bool WEIGHTED>
 static void enqueue_chunk(CgWork* cw, const Sell& A, const IDX* col, const T* val, T* y, T* r, T* p, T* s, T* w,
                           int chunk, hipStream_t st) {
-    const dim3 gr(cw->nb), bl(kBlock);
+    const dim3 gr(cw->total_blocks), bl(kBlock);
+    const Seg sg = cw->seg();
     for (int k = 0; k < chunk; k++) {
-        hipLaunchKernelGGL((k_cg_apply<T, IDX, WEIGHTED>), gr, bl, 0, st, A.rows_dev, A.slice_off, col, val, r, w,
-                           A.dgl, cw->part, cw->sc);
-        hipLaunchKernelGGL((k_cg_step<T, WEIGHTED>), gr, bl, 0, st, A.rows_dev, y, r, p, s, w, cw->part, cw->sc,
-                           k & 1);
+        hipLaunchKernelGGL((k_cg_apply<T, IDX, WEIGHTED>), gr, bl, 0, st, sg, A.slice_off, col, val, r, w, A.dgl,
+                           cw->part, cw->sc);
+        hipLaunchKernelGGL((k_cg_step<T, WEIGHTED>), gr, bl, 0, st, sg, y, r, p, s, w, cw->part, cw->sc, k & 1);
     }
+}
+
+static bool all_done(const CgWork* cw) {
+    for (int s = 0; s < cw->nseg; s++)
+        if (!cw->host[s].done) return false;
+    return true;
 }
 
 template <typename T, typename IDX, bool WEIGHTED>
@@ -604,11 +717,12 @@ static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* c
                               T* s, T* w, const mps_solver_options& o, hipStream_t st) {
     const int chunk = std::max(2, (o.check_interval + 1) & ~1);
     const int idx16 = sizeof(IDX) == 2 ? 1 : 0;
+    const size_t sbytes = sizeof(CgScalars) * (size_t) cw->nseg;
     hipError_t e;
     for (;;) {
         if (o.use_graph) {
             if (!cw->graph[which] || cw->graph_chunk[which] != chunk || cw->graph_key[which] != (const void*) val ||
-                cw->graph_idx16[which] != idx16 || cw->graph_nb[which] != cw->nb) {
+                cw->graph_idx16[which] != idx16 || cw->graph_blocks[which] != cw->total_blocks) {
                 if (cw->graph[which]) hipGraphExecDestroy(cw->graph[which]);
                 cw->graph[which] = nullptr;
                 hipGraph_t gph;
@@ -621,46 +735,44 @@ static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* c
                 cw->graph_chunk[which] = chunk;
                 cw->graph_key[which] = val;
                 cw->graph_idx16[which] = idx16;
-                cw->graph_nb[which] = cw->nb;
+                cw->graph_blocks[which] = cw->total_blocks;
             }
             if ((e = hipGraphLaunch(cw->graph[which], st))) return e;
         } else {
             enqueue_chunk<T, IDX, WEIGHTED>(cw, A, col, val, y, r, p, s, w, chunk, st);
         }
-        if ((e = hipMemcpyAsync(cw->host, cw->sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st))) return e;
+        if ((e = hipMemcpyAsync(cw->host, cw->sc, sbytes, hipMemcpyDeviceToHost, st))) return e;
         if ((e = hipStreamSynchronize(st))) return e;
-        if (cw->host->done) return hipSuccess;
+        if (all_done(cw)) return hipSuccess;
     }
 }
 
-static hipError_t set_scalars(CgWork* cw, const CgScalars& s, hipStream_t st) {
-    *cw->host = s;
-    return hipMemcpyAsync(cw->sc, cw->host, sizeof(CgScalars), hipMemcpyHostToDevice, st);
+static hipError_t upload_scalars(CgWork* cw, hipStream_t st) {
+    return hipMemcpyAsync(cw->sc, cw->host, sizeof(CgScalars) * (size_t) cw->nseg, hipMemcpyHostToDevice, st);
 }
 
-static hipError_t read_scalars(CgWork* cw, hipStream_t st) {
-    hipError_t e = hipMemcpyAsync(cw->host, cw->sc, sizeof(CgScalars), hipMemcpyDeviceToHost, st);
+// Reduces nv-interleaved partials of every segment into sc[s].red and reads
+// all scalars back.
+static hipError_t reduce_and_read(CgWork* cw, int nv, hipStream_t st) {
+    const Seg sg = cw->seg();
+    if (nv == 1)
+        hipLaunchKernelGGL(k_reduce_parts<1>, dim3(cw->nseg), dim3(kBlock), 0, st, sg, cw->part, cw->sc);
+    else
+        hipLaunchKernelGGL(k_reduce_parts<2>, dim3(cw->nseg), dim3(kBlock), 0, st, sg, cw->part, cw->sc);
+    hipError_t e = hipMemcpyAsync(cw->host, cw->sc, sizeof(CgScalars) * (size_t) cw->nseg, hipMemcpyDeviceToHost, st);
     if (e) return e;
     return hipStreamSynchronize(st);
 }
 
-// Reduces nv-interleaved partials into sc->red and reads the scalars back.
-static hipError_t reduce_and_read(CgWork* cw, int nv, hipStream_t st) {
-    if (nv == 1)
-        hipLaunchKernelGGL(k_reduce_parts<1>, dim3(1), dim3(kBlock), 0, st, cw->part, cw->nb, cw->sc);
-    else
-        hipLaunchKernelGGL(k_reduce_parts<2>, dim3(1), dim3(kBlock), 0, st, cw->part, cw->nb, cw->sc);
-    return read_scalars(cw, st);
-}
-
 static void launch_residual(CgWork* cw, const Sell& A, hipStream_t st) {
-    const dim3 gr(cw->nb), bl(kBlock);
+    const dim3 gr(cw->total_blocks), bl(kBlock);
+    const Seg sg = cw->seg();
     if (A.use16)
-        hipLaunchKernelGGL(k_residual64<short>, gr, bl, 0, st, A.rows_dev, A.slice_off, A.dcol, A.val64, cw->y, cw->bh,
-                           A.dgl, cw->r64, cw->part);
+        hipLaunchKernelGGL(k_residual64<short>, gr, bl, 0, st, sg, A.slice_off, A.dcol, A.val64, cw->y, cw->bh, A.dgl,
+                           cw->r64, cw->part);
     else
-        hipLaunchKernelGGL(k_residual64<int>, gr, bl, 0, st, A.rows_dev, A.slice_off, A.col, A.val64, cw->y, cw->bh,
-                           A.dgl, cw->r64, cw->part);
+        hipLaunchKernelGGL(k_residual64<int>, gr, bl, 0, st, sg, A.slice_off, A.col, A.val64, cw->y, cw->bh, A.dgl,
+                           cw->r64, cw->part);
 }
 
 template <typename T, bool WEIGHTED>
@@ -671,115 +783,157 @@ static hipError_t run_cg(CgWork* cw, int which, const Sell& A, const T* val, T* 
 }
 
 hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double* b_s, const double* pprev_id,
-                    double* x_s, const mps_solver_options& o, mps_solver_stats& stats, hipStream_t st,
+                    double* x_s, const mps_solver_options& o, mps_solver_stats* stats, hipStream_t st,
                     std::string& msg) {
     (void) n;
     const int rows = A.rows;
-    stats = mps_solver_stats{};
-    stats.rows = rows;
-    stats.converged = 1;
+    const int nseg = A.nseg;
+    for (int s = 0; s < nseg; s++) {
+        stats[s] = mps_solver_stats{};
+        stats[s].converged = 1;
+    }
     if (rows == 0) return hipSuccess;
     hipError_t e = cg_reserve(cw, rows);
     if (e) return e;
-    // the block grid (and with it the order of every reduction) depends on
-    // the live row count only, so equal systems give equal bits
-    cw->nb = cg_blocks(rows);
-    const dim3 gr(cw->nb), bl(kBlock);
-    const int max_it = o.max_iterations > 0 ? o.max_iterations : 2 * rows;
-    const double tol = o.tolerance;
+    if ((e = cg_layout(cw, A, st))) return e;
+    const dim3 gr(cw->total_blocks), bl(kBlock);
+    const Seg sg = cw->seg();
 
-    hipLaunchKernelGGL(k_cg_init, gr, bl, 0, st, A.rows_dev, A.row_slot, h.id, b_s, o.warm_start ? pprev_id : nullptr,
+    hipLaunchKernelGGL(k_cg_init, gr, bl, 0, st, sg, A.row_slot, h.id, b_s, o.warm_start ? pprev_id : nullptr,
                        A.sdiag, cw->y, cw->bh, cw->part);
     if ((e = reduce_and_read(cw, 1, st))) return e;
-    const double bn2 = cw->host->red[0];
-    stats.rhs_norm = sqrt(bn2);
-    if (bn2 == 0.0) {
-        // Eigen: a zero right-hand side gives x = 0 (no iteration)
-        if ((e = fill_double(cw->y, 0.0, (size_t) rows, st))) return e;
-        hipLaunchKernelGGL(k_cg_finish, gr, bl, 0, st, A.rows_dev, A.row_slot, cw->y, A.sdiag, x_s);
-        stats.rel_residual = 0.0;
-        return hipGetLastError();
+    std::vector<double> bn2(nseg), thr(nseg), rho2(nseg);
+    std::vector<int> segrows(nseg), conv(nseg), total_it(nseg, 0), refinements(nseg, 0), max_it(nseg);
+    for (int s = 0; s < nseg; s++) {
+        segrows[s] = cw->h_row[s + 1] - cw->h_row[s];
+        bn2[s] = cw->host[s].red[0];
+        stats[s].rows = segrows[s];
+        stats[s].rhs_norm = sqrt(bn2[s]);
+        thr[s] = o.tolerance * o.tolerance * bn2[s];
+        max_it[s] = o.max_iterations > 0 ? o.max_iterations : 2 * segrows[s];
     }
-    const double thr = tol * tol * bn2;
+    // Eigen: a zero right-hand side gives x = 0 (no iteration); warm starts
+    // are discarded for such segments
+    bool any_zero = false;
+    for (int s = 0; s < nseg; s++) any_zero |= bn2[s] == 0.0;
+    if (any_zero) {
+        if (nseg == 1) {
+            if ((e = fill_double(cw->y, 0.0, (size_t) rows, st))) return e;
+        } else {
+            for (int s = 0; s < nseg; s++)
+                if (bn2[s] == 0.0 && segrows[s] > 0)
+                    if ((e = fill_double(cw->y + cw->h_row[s], 0.0, (size_t) segrows[s], st))) return e;
+        }
+    }
     launch_residual(cw, A, st);
     if ((e = reduce_and_read(cw, 2, st))) return e;
-    double rho2 = cw->host->red[1];
-    stats.initial_residual = sqrt(rho2 / bn2);
-    int total_it = 0, refinements = 0;
-    bool conv = rho2 <= thr;
+    bool all_conv = true;
+    for (int s = 0; s < nseg; s++) {
+        rho2[s] = cw->host[s].red[1];
+        conv[s] = bn2[s] == 0.0 || rho2[s] <= thr[s];
+        stats[s].initial_residual = bn2[s] > 0 ? sqrt(rho2[s] / bn2[s]) : 0.0;
+        all_conv = all_conv && conv[s];
+    }
 
     if (o.precision == MPS_PREC_FP64) {
-        if (!conv) {
-            CgScalars s{};
-            s.thr = thr;
-            s.max_iter = max_it;
-            if ((e = set_scalars(cw, s, st))) return e;
+        if (!all_conv) {
+            // part holds the initial residual partials for the first apply
+            for (int s = 0; s < nseg; s++) {
+                CgScalars z{};
+                z.thr = thr[s];
+                z.max_iter = max_it[s];
+                z.done = conv[s];
+                cw->host[s] = z;
+            }
+            if ((e = upload_scalars(cw, st))) return e;
             if ((e = run_cg<double, true>(cw, 0, A, A.val64, cw->y, cw->r64, cw->p64, cw->s64, cw->w64, o, st)))
                 return e;
-            total_it = cw->host->iter;
-            conv = cw->host->converged != 0;
+            for (int s = 0; s < nseg; s++) {
+                if (conv[s]) continue;
+                total_it[s] = cw->host[s].iter;
+                conv[s] = cw->host[s].converged != 0;
+            }
         }
     } else {
         const int max_ref = o.max_refinements > 0 ? o.max_refinements : 20;
         const double eta = o.inner_tolerance > 0 ? o.inner_tolerance : 1e-4;
-        while (!conv && refinements < max_ref && total_it < max_it) {
-            hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, A.rows_dev, cw->r64, cw->r32, cw->e32, cw->part);
+        for (;;) {
+            bool active = false;
+            for (int s = 0; s < nseg; s++)
+                active = active || (!conv[s] && refinements[s] < max_ref && total_it[s] < max_it[s]);
+            if (!active) break;
+            hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, sg, cw->r64, cw->r32, cw->e32, cw->part);
             if ((e = reduce_and_read(cw, 1, st))) return e;
-            const double init = cw->host->red[0];
-            // reduce the inner residual by eta, or by what the outer loop
-            // still needs when that is less
-            const double need = thr / rho2;
-            double red2 = eta * eta;
-            if (need > red2) red2 = need * 0.25;
-            CgScalars s{};
-            s.thr = init * red2;
-            s.max_iter = max_it - total_it;
-            s.done = init == 0.0 ? 1 : 0;
-            if ((e = set_scalars(cw, s, st))) return e;
-            if (!s.done) {
-                if ((e = run_cg<float, false>(cw, 1, A, A.val32, cw->e32, cw->r32, cw->p32, cw->s32, cw->w32, o, st)))
-                    return e;
-                total_it += cw->host->iter;
+            std::vector<int> run(nseg, 0);
+            for (int s = 0; s < nseg; s++) {
+                const double init = cw->host[s].red[0];
+                CgScalars z{};
+                run[s] = !conv[s] && refinements[s] < max_ref && total_it[s] < max_it[s] && init > 0.0;
+                // reduce the inner residual by eta, or by what the outer loop
+                // still needs when that is less
+                double red2 = eta * eta;
+                if (run[s]) {
+                    const double need = thr[s] / rho2[s];
+                    if (need > red2) red2 = need * 0.25;
+                }
+                z.thr = init * red2;
+                z.max_iter = max_it[s] - total_it[s];
+                z.done = run[s] ? 0 : 1;
+                cw->host[s] = z;
             }
-            hipLaunchKernelGGL(k_add_correction, gr, bl, 0, st, A.rows_dev, cw->y, cw->e32);
+            if ((e = upload_scalars(cw, st))) return e;
+            if ((e = run_cg<float, false>(cw, 1, A, A.val32, cw->e32, cw->r32, cw->p32, cw->s32, cw->w32, o, st)))
+                return e;
+            for (int s = 0; s < nseg; s++)
+                if (run[s]) total_it[s] += cw->host[s].iter;
+            // segments that did not run keep e = 0 from k_inner_init
+            hipLaunchKernelGGL(k_add_correction, gr, bl, 0, st, sg, cw->y, cw->e32);
             launch_residual(cw, A, st);
             if ((e = reduce_and_read(cw, 2, st))) return e;
-            rho2 = cw->host->red[1];
-            conv = rho2 <= thr;
-            refinements++;
+            for (int s = 0; s < nseg; s++) {
+                if (!run[s]) continue;
+                rho2[s] = cw->host[s].red[1];
+                conv[s] = rho2[s] <= thr[s];
+                refinements[s]++;
+            }
         }
     }
     // true FP64 residual of the final iterate
     launch_residual(cw, A, st);
     if ((e = reduce_and_read(cw, 2, st))) return e;
-    stats.iterations = total_it;
-    stats.refinements = refinements;
-    stats.rel_residual = sqrt(cw->host->red[1] / bn2);
-    stats.converged = conv ? 1 : 0;
-    hipLaunchKernelGGL(k_cg_finish, gr, bl, 0, st, A.rows_dev, A.row_slot, cw->y, A.sdiag, x_s);
-    if (!conv) msg = "pressure solve stopped above its tolerance";
+    bool all_ok = true;
+    for (int s = 0; s < nseg; s++) {
+        stats[s].iterations = total_it[s];
+        stats[s].refinements = refinements[s];
+        stats[s].rel_residual = bn2[s] > 0 ? sqrt(cw->host[s].red[1] / bn2[s]) : 0.0;
+        stats[s].converged = conv[s] ? 1 : 0;
+        all_ok = all_ok && conv[s];
+    }
+    hipLaunchKernelGGL(k_cg_finish, gr, bl, 0, st, sg, A.row_slot, cw->y, A.sdiag, x_s);
+    if (!all_ok) msg = "pressure solve stopped above its tolerance";
     return hipGetLastError();
 }
 
 template <typename IDX>
 static void time_launch(CgWork* cw, const Sell& A, const IDX* col, int kind, hipStream_t st) {
-    const dim3 gr(cw->nb), bl(kBlock);
+    const dim3 gr(cw->total_blocks), bl(kBlock);
+    const Seg sg = cw->seg();
     switch (kind) {
         case 0:
-            hipLaunchKernelGGL((k_cg_apply<double, IDX, true>), gr, bl, 0, st, A.rows_dev, A.slice_off, col, A.val64,
-                               cw->r64, cw->w64, A.dgl, cw->part, cw->sc);
+            hipLaunchKernelGGL((k_cg_apply<double, IDX, true>), gr, bl, 0, st, sg, A.slice_off, col, A.val64, cw->r64,
+                               cw->w64, A.dgl, cw->part, cw->sc);
             break;
         case 1:
-            hipLaunchKernelGGL((k_cg_step<double, true>), gr, bl, 0, st, A.rows_dev, cw->y, cw->r64, cw->p64, cw->s64,
+            hipLaunchKernelGGL((k_cg_step<double, true>), gr, bl, 0, st, sg, cw->y, cw->r64, cw->p64, cw->s64,
                                cw->w64, cw->part, cw->sc, 0);
             break;
         case 2:
-            hipLaunchKernelGGL((k_cg_apply<float, IDX, false>), gr, bl, 0, st, A.rows_dev, A.slice_off, col, A.val32,
-                               cw->r32, cw->w32, A.dgl, cw->part, cw->sc);
+            hipLaunchKernelGGL((k_cg_apply<float, IDX, false>), gr, bl, 0, st, sg, A.slice_off, col, A.val32, cw->r32,
+                               cw->w32, A.dgl, cw->part, cw->sc);
             break;
         default:
-            hipLaunchKernelGGL((k_cg_step<float, false>), gr, bl, 0, st, A.rows_dev, cw->e32, cw->r32, cw->p32,
-                               cw->s32, cw->w32, cw->part, cw->sc, 0);
+            hipLaunchKernelGGL((k_cg_step<float, false>), gr, bl, 0, st, sg, cw->e32, cw->r32, cw->p32, cw->s32,
+                               cw->w32, cw->part, cw->sc, 0);
             break;
     }
 }
@@ -794,24 +948,28 @@ hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, h
     if (rows == 0 || reps <= 0) return hipSuccess;
     hipError_t e = cg_reserve(cw, rows);
     if (e) return e;
-    cw->nb = cg_blocks(rows);
-    const dim3 gr(cw->nb), bl(kBlock);
+    if ((e = cg_layout(cw, A, st))) return e;
+    const dim3 gr(cw->total_blocks), bl(kBlock);
+    const Seg sg = cw->seg();
     // finite vector contents
     const size_t R = (size_t) rows;
     if ((e = fill_double(cw->r64, 1e-3, R, st)) || (e = fill_double(cw->p64, 0.0, R, st)) ||
         (e = fill_double(cw->s64, 0.0, R, st)) || (e = fill_double(cw->w64, 0.0, R, st)) ||
         (e = fill_double(cw->y, 0.0, R, st)))
         return e;
-    hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, A.rows_dev, cw->r64, cw->r32, cw->e32, cw->part);
-    hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, A.rows_dev, cw->p64, cw->p32, cw->s32, cw->part);
-    hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, A.rows_dev, cw->p64, cw->w32, cw->e32, cw->part);
-    if ((e = fill_double(cw->part, 1.0, 3 * (size_t) cw->nb, st))) return e;
-    CgScalars s{};
-    s.frozen = 1;
-    s.iter = 2;
-    s.max_iter = 0x7fffffff;
-    s.thr = -1.0;
-    if ((e = set_scalars(cw, s, st))) return e;
+    hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, sg, cw->r64, cw->r32, cw->e32, cw->part);
+    hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, sg, cw->p64, cw->p32, cw->s32, cw->part);
+    hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, sg, cw->p64, cw->w32, cw->e32, cw->part);
+    if ((e = fill_double(cw->part, 1.0, 3 * (size_t) cw->total_blocks, st))) return e;
+    for (int s = 0; s < cw->nseg; s++) {
+        CgScalars z{};
+        z.frozen = 1;
+        z.iter = 2;
+        z.max_iter = 0x7fffffff;
+        z.thr = -1.0;
+        cw->host[s] = z;
+    }
+    if ((e = upload_scalars(cw, st))) return e;
     hipEvent_t a, b;
     if ((e = hipEventCreate(&a))) return e;
     if ((e = hipEventCreate(&b))) return e;

@@ -12,6 +12,22 @@ namespace mps {
 
 static inline dim3 grid_for(int n, int bs) { return dim3((unsigned) ((n + bs - 1) / bs)); }
 
+// Launches kern<DIM, MULTI> for the handle's dimension and scene layout.
+#define LAUNCH_DM(kern, grid, block, st, ...)                                                   \
+    do {                                                                                        \
+        if (dim == 2) {                                                                         \
+            if (tab.g)                                                                          \
+                hipLaunchKernelGGL((kern<2, true>), grid, block, 0, st, __VA_ARGS__);           \
+            else                                                                                \
+                hipLaunchKernelGGL((kern<2, false>), grid, block, 0, st, __VA_ARGS__);          \
+        } else {                                                                                \
+            if (tab.g)                                                                          \
+                hipLaunchKernelGGL((kern<3, true>), grid, block, 0, st, __VA_ARGS__);           \
+            else                                                                                \
+                hipLaunchKernelGGL((kern<3, false>), grid, block, 0, st, __VA_ARGS__);          \
+        }                                                                                       \
+    } while (0)
+
 __device__ __forceinline__ void push_event(int* counter, int* ids, int cap, int id) {
     int k = atomicAdd(counter, 1);
     if (k < cap) ids[k] = id;
@@ -37,11 +53,14 @@ __device__ __forceinline__ double ndens(const GridView& g, const Hot& h, int s, 
 
 // ---------------- gravity + viscosity + explicit move (src/mps.cpp:165-215) ----------------
 
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_visc_move(GridView g, int n, Consts c, Hot cur, Hot out, Cold cold,
-                                                      Events ev, double avisc) {
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_visc_move(GridView g0, Consts c0, SceneTab tab, int n, Hot cur, Hot out, Cold cold,
+                                                      Events ev) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    GridView g = g0;
+    Consts c = c0;
+    scene_of<MULTI>(tab, cur.id[s], g, c);
     double x = cur.px[s], y = cur.py[s], z = DIM == 3 ? cur.pz[s] : 0.0;
     double vx = cur.vx[s], vy = cur.vy[s], vz = DIM == 3 ? cur.vz[s] : 0.0;
     if (cur.type[s] == MPS_FLUID) {
@@ -55,9 +74,9 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_visc_move(GridView g, int n, 
                                ty += (cur.vy[q] - vy) * w;
                                if (DIM == 3) tz += (cur.vz[q] - vz) * w;
                            });
-        tx *= avisc;
-        ty *= avisc;
-        tz *= avisc;
+        tx *= c.avisc;
+        ty *= c.avisc;
+        tz *= c.avisc;
         ax += tx;
         ay += ty;
         az += tz;
@@ -85,14 +104,10 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_visc_move(GridView g, int n, 
     }
 }
 
-hipError_t launch_visc_move(int dim, const GridView& g, int n, const Consts& c, const Hot& cur, Hot& out,
-                            Cold& cold, Events& ev, hipStream_t st, int bs) {
+hipError_t launch_visc_move(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& cur,
+                            Hot& out, Cold& cold, Events& ev, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
-    const double avisc = (c.nu) * (2.0 * (double) c.dim) / (c.n0_v * c.lam_v);
-    if (dim == 2)
-        hipLaunchKernelGGL(k_visc_move<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, cur, out, cold, ev, avisc);
-    else
-        hipLaunchKernelGGL(k_visc_move<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, cur, out, cold, ev, avisc);
+    LAUNCH_DM(k_visc_move, grid_for(n, bs), dim3(bs), st, g, c, tab, n, cur, out, cold, ev);
     return hipGetLastError();
 }
 
@@ -107,10 +122,13 @@ hipError_t launch_visc_move(int dim, const GridView& g, int n, const Consts& c, 
 // walls never change, so they carry no dependency.  Pairs of one level touch
 // disjoint fluid particles and are resolved together; levels run in order.
 
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_coll_count(GridView g, int n, Consts c, Hot h, int* __restrict__ cnt) {
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_coll_count(GridView g0, Consts c0, SceneTab tab, int n, Hot h, int* __restrict__ cnt) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    GridView g = g0;
+    Consts c = c0;
+    scene_of<MULTI>(tab, h.id[s], g, c);
     const int id = h.id[s];
     int k = 0;
     if (h.type[s] == MPS_FLUID) {
@@ -124,12 +142,15 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_coll_count(GridView g, int n,
     cnt[id] = k;
 }
 
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_coll_emit(GridView g, int n, Consts c, Hot h, const int* __restrict__ cnt,
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_coll_emit(GridView g0, Consts c0, SceneTab tab, int n, Hot h, const int* __restrict__ cnt,
                                                       const int* __restrict__ off, int* __restrict__ pairs,
                                                       double* __restrict__ pdist, int cap, int* __restrict__ err) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    GridView g = g0;
+    Consts c = c0;
+    scene_of<MULTI>(tab, h.id[s], g, c);
     const int id = h.id[s];
     if (cnt[id] == 0) return;
     int o = off[id];
@@ -151,7 +172,9 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_coll_emit(GridView g, int n, 
 
 constexpr int kResolveThreads = 1024;
 
-__global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts c, Hot h, const int* __restrict__ off,
+template <bool MULTI>
+__global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts c0, SceneTab tab, int dim, Hot h,
+                                                                  const int* __restrict__ off,
                                                                   const int* __restrict__ pairs,
                                                                   const double* __restrict__ pdist,
                                                                   int* __restrict__ level, int* __restrict__ last,
@@ -182,12 +205,15 @@ __global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts 
     volatile double* vx = h.vx;
     volatile double* vy = h.vy;
     volatile double* vz = h.vz;
-    const bool d3 = c.dim == 3;
-    const double e = c.restitution, dc = c.dc;
+    const bool d3 = dim == 3;
     for (int L = 1; L <= s_max; L++) {
         for (int p = threadIdx.x; p < np; p += blockDim.x) {
             if (level[p] != L) continue;
             const int i = pairs[2 * p], j = pairs[2 * p + 1];
+            Consts c = c0;
+            GridView gdummy;
+            scene_of<MULTI>(tab, h.id[i], gdummy, c);
+            const double e = c.restitution, dc = c.dc;
             const double dist = pdist[p];
             const bool jf = h.type[j] == MPS_FLUID;
             // src/particle.cpp:19-29: 1/density for fluid, 0 otherwise
@@ -235,34 +261,33 @@ __global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts 
     }
 }
 
-hipError_t launch_collision(int dim, const GridView& g, int n, const Consts& c, Hot& h, StepScratch& w, int* err,
-                            hipStream_t st, int bs) {
+hipError_t launch_collision(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, Hot& h,
+                            StepScratch& w, int* err, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
-    if (dim == 2)
-        hipLaunchKernelGGL(k_coll_count<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, w.coll_cnt);
-    else
-        hipLaunchKernelGGL(k_coll_count<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, w.coll_cnt);
+    LAUNCH_DM(k_coll_count, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, w.coll_cnt);
     hipError_t e = scan_exclusive(w.coll_cnt, w.coll_off, n, w.scan_tmp, st);
     if (e != hipSuccess) return e;
-    if (dim == 2)
-        hipLaunchKernelGGL(k_coll_emit<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, w.coll_cnt, w.coll_off,
-                           w.coll_pairs, w.coll_dist, w.coll_cap, err);
+    LAUNCH_DM(k_coll_emit, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, w.coll_cnt, w.coll_off, w.coll_pairs,
+              w.coll_dist, w.coll_cap, err);
+    if (tab.g)
+        hipLaunchKernelGGL(k_coll_resolve<true>, dim3(1), dim3(kResolveThreads), 0, st, n, c, tab, dim, h, w.coll_off,
+                           w.coll_pairs, w.coll_dist, w.coll_level, w.coll_last, w.coll_cap);
     else
-        hipLaunchKernelGGL(k_coll_emit<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, w.coll_cnt, w.coll_off,
-                           w.coll_pairs, w.coll_dist, w.coll_cap, err);
-    hipLaunchKernelGGL(k_coll_resolve, dim3(1), dim3(kResolveThreads), 0, st, n, c, h, w.coll_off, w.coll_pairs,
-                       w.coll_dist, w.coll_level, w.coll_last, w.coll_cap);
+        hipLaunchKernelGGL(k_coll_resolve<false>, dim3(1), dim3(kResolveThreads), 0, st, n, c, tab, dim, h, w.coll_off,
+                           w.coll_pairs, w.coll_dist, w.coll_level, w.coll_last, w.coll_cap);
     return hipGetLastError();
 }
 
 // ---------------- boundary condition + source term (src/mps.cpp:262-320) ----------------
 
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g, int n, Consts c, Hot h, Cold cold,
-                                                      int8_t* __restrict__ bc_s, double* __restrict__ b_s,
-                                                      double src_scale) {
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g0, Consts c0, SceneTab tab, int n, Hot h, Cold cold,
+                                                      int8_t* __restrict__ bc_s, double* __restrict__ b_s) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    GridView g = g0;
+    Consts c = c0;
+    scene_of<MULTI>(tab, h.id[s], g, c);
     const int t = h.type[s];
     int8_t bc;
     double b = 0.0;
@@ -279,7 +304,7 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g, int n, 
                                if (dist < rp) np += wfun(dist, rp);
                            });
         bc = ns < c.beta * c.n0_s ? MPS_BC_SURFACE : MPS_BC_INNER;
-        if (bc == MPS_BC_INNER && !c.hos_on) b = src_scale * (np - c.n0_p) / c.n0_p;
+        if (bc == MPS_BC_INNER && !c.hos_on) b = c.src_scale * (np - c.n0_p) / c.n0_p;
     }
     bc_s[s] = bc;
     b_s[s] = b;
@@ -287,14 +312,16 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g, int n, 
 }
 
 // Higher-order source term (src/mps.cpp:288-308); needs every slot's BC.
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_source_hos(GridView g, int n, Consts c, Hot h, Cold cold,
-                                                       const int8_t* __restrict__ bc_s, double* __restrict__ b_s,
-                                                       double hos_scale, double hos_coef) {
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_source_hos(GridView g0, Consts c0, SceneTab tab, int n, Hot h, Cold cold,
+                                                       const int8_t* __restrict__ bc_s, double* __restrict__ b_s) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    if (bc_s[s] != MPS_BC_INNER) return;
-    double b = hos_scale * (cold.ndr[h.id[s]] - 1.0);
+    GridView g = g0;
+    Consts c = c0;
+    scene_of<MULTI>(tab, h.id[s], g, c);
+    if (!c.hos_on || bc_s[s] != MPS_BC_INNER) return;
+    double b = c.hos_scale * (cold.ndr[h.id[s]] - 1.0);
     const double re = c.re_p;
     const double xi = h.px[s], yi = h.py[s], zi = DIM == 3 ? h.pz[s] : 0.0;
     const double vxi = h.vx[s], vyi = h.vy[s], vzi = DIM == 3 ? h.vz[s] : 0.0;
@@ -303,43 +330,29 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_source_hos(GridView g, int n,
                            if (bc_s[q] == MPS_BC_IGNORED) return;
                            const double ux = h.vx[q] - vxi, uy = h.vy[q] - vyi;
                            const double uz = DIM == 3 ? h.vz[q] - vzi : 0.0;
-                           if (dist < re) b -= hos_coef * re * ((rx * ux + ry * uy) + rz * uz) / (dist * dist * dist);
+                           if (dist < re) b -= c.hos_coef * re * ((rx * ux + ry * uy) + rz * uz) / (dist * dist * dist);
                        });
     b_s[s] = b;
 }
 
-hipError_t launch_bc_source(int dim, const GridView& g, int n, const Consts& c, const Hot& h, Cold& cold,
-                            StepScratch& w, hipStream_t st, int bs) {
+hipError_t launch_bc_source(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
+                            Cold& cold, StepScratch& w, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
-    const double dt = c.dt;
-    const double src_scale = c.relax * (1.0 / (dt * dt));
-    if (dim == 2)
-        hipLaunchKernelGGL(k_bc_source<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
-                           src_scale);
-    else
-        hipLaunchKernelGGL(k_bc_source<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
-                           src_scale);
-    if (c.hos_on) {
-        const double hos_scale = c.hos_gamma * (1.0 / (dt * dt));
-        const double hos_coef = (1.0 / (c.n0_p * dt));
-        if (dim == 2)
-            hipLaunchKernelGGL(k_source_hos<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
-                               hos_scale, hos_coef);
-        else
-            hipLaunchKernelGGL(k_source_hos<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, cold, w.bc_s, w.b_s,
-                               hos_scale, hos_coef);
-    }
+    LAUNCH_DM(k_bc_source, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, cold, w.bc_s, w.b_s);
+    if (c.hos_on) LAUNCH_DM(k_source_hos, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, cold, w.bc_s, w.b_s);
     return hipGetLastError();
 }
 
 // ---------------- matrix diagonal + Dirichlet connectivity (src/mps.cpp:322-415) ----------------
 
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_diag(GridView g, int n, Consts c, Hot h, const int8_t* __restrict__ bc_s,
-                                                 double* __restrict__ diag, int* __restrict__ nnz, double coefA,
-                                                 double qc_add) {
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_diag(GridView g0, Consts c0, SceneTab tab, int n, Hot h, const int8_t* __restrict__ bc_s,
+                                                 double* __restrict__ diag, int* __restrict__ nnz) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    GridView g = g0;
+    Consts c = c0;
+    scene_of<MULTI>(tab, h.id[s], g, c);
     double d = 0.0;
     int k = 0;
     if (bc_s[s] == MPS_BC_INNER) {
@@ -349,11 +362,11 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_diag(GridView g, int n, Const
                            [&](int q, double dist, double, double, double) {
                                const int8_t bq = bc_s[q];
                                if (bq == MPS_BC_IGNORED) return;
-                               const double a = coefA * wfun(dist, re) / c.rho;
+                               const double a = c.coefA * wfun(dist, re) / c.rho;
                                d += a;
                                if (bq == MPS_BC_INNER) k++;
                            });
-        if (c.qc_on) d += qc_add;
+        if (c.qc_on) d += c.qc_add;
     }
     diag[s] = d;
     nnz[s] = k;
@@ -377,10 +390,13 @@ __device__ __forceinline__ int cc_find(volatile int* parent, int v) {
     return curr;
 }
 
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_cc_init(GridView g, int n, Consts c, Hot h, int* __restrict__ parent) {
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_cc_init(GridView g0, Consts c0, SceneTab tab, int n, Hot h, int* __restrict__ parent) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    GridView g = g0;
+    Consts c = c0;
+    scene_of<MULTI>(tab, h.id[s], g, c);
     int m = s;
     if (h.type[s] != MPS_GHOST) {
         const double zi = DIM == 3 ? h.pz[s] : 0.0;
@@ -392,10 +408,13 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_cc_init(GridView g, int n, Co
     parent[s] = m;
 }
 
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_cc_join(GridView g, int n, Consts c, Hot h, int* parent) {
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_cc_join(GridView g0, Consts c0, SceneTab tab, int n, Hot h, int* parent) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    GridView g = g0;
+    Consts c = c0;
+    scene_of<MULTI>(tab, h.id[s], g, c);
     if (h.type[s] == MPS_GHOST) return;
     volatile int* vp = parent;
     const double zi = DIM == 3 ? h.pz[s] : 0.0;
@@ -443,22 +462,13 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_cc_apply(int n, Hot h, const 
     }
 }
 
-hipError_t launch_diag_cc(int dim, const GridView& g, int n, const Consts& c, const Hot& h, StepScratch& w,
-                          Events& ev, hipStream_t st, int bs) {
+hipError_t launch_diag_cc(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
+                          StepScratch& w, Events& ev, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
-    const double D = (double) c.dim;
-    const double coefA = (2.0 * D / (c.lam_p * c.n0_p));
-    const double qc_add = c.alpha / (c.dt * c.dt);
     const dim3 gr = grid_for(n, bs), bl(bs);
-    if (dim == 2) {
-        hipLaunchKernelGGL(k_diag<2>, gr, bl, 0, st, g, n, c, h, w.bc_s, w.diag_s, w.nnz_s, coefA, qc_add);
-        hipLaunchKernelGGL(k_cc_init<2>, gr, bl, 0, st, g, n, c, h, w.parent);
-        hipLaunchKernelGGL(k_cc_join<2>, gr, bl, 0, st, g, n, c, h, w.parent);
-    } else {
-        hipLaunchKernelGGL(k_diag<3>, gr, bl, 0, st, g, n, c, h, w.bc_s, w.diag_s, w.nnz_s, coefA, qc_add);
-        hipLaunchKernelGGL(k_cc_init<3>, gr, bl, 0, st, g, n, c, h, w.parent);
-        hipLaunchKernelGGL(k_cc_join<3>, gr, bl, 0, st, g, n, c, h, w.parent);
-    }
+    LAUNCH_DM(k_diag, gr, bl, st, g, c, tab, n, h, w.bc_s, w.diag_s, w.nnz_s);
+    LAUNCH_DM(k_cc_init, gr, bl, st, g, c, tab, n, h, w.parent);
+    LAUNCH_DM(k_cc_join, gr, bl, st, g, c, tab, n, h, w.parent);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // surf[] is a byte array: clear it with the int fill over its rounded-up size
@@ -484,13 +494,16 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_pressure_out(int n, Hot h, co
     cold.p[id] = p;
 }
 
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_min_pressure(GridView g, int n, Consts c, Hot h,
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_min_pressure(GridView g0, Consts c0, SceneTab tab, int n, Hot h,
                                                          const int8_t* __restrict__ bc_s,
                                                          const double* __restrict__ p_s, Cold cold,
                                                          double* __restrict__ pmin_s) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    GridView g = g0;
+    Consts c = c0;
+    scene_of<MULTI>(tab, h.id[s], g, c);
     const int id = h.id[s];
     if (bc_s[s] == MPS_BC_IGNORED) {
         pmin_s[s] = cold.pmin[id];
@@ -507,27 +520,25 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_min_pressure(GridView g, int 
     cold.pmin[id] = pm;
 }
 
-hipError_t launch_pressure_out(int dim, const GridView& g, int n, const Consts& c, const Hot& h, const double* x_s,
-                               Cold& cold, StepScratch& w, hipStream_t st, int bs) {
+hipError_t launch_pressure_out(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
+                               const double* x_s, Cold& cold, StepScratch& w, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_pressure_out, grid_for(n, bs), dim3(bs), 0, st, n, h, w.bc_s, x_s, cold, w.p_s);
-    if (dim == 2)
-        hipLaunchKernelGGL(k_min_pressure<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, w.bc_s, w.p_s, cold,
-                           w.pmin_s);
-    else
-        hipLaunchKernelGGL(k_min_pressure<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, w.bc_s, w.p_s, cold,
-                           w.pmin_s);
+    LAUNCH_DM(k_min_pressure, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, w.bc_s, w.p_s, cold, w.pmin_s);
     return hipGetLastError();
 }
 
 // ---------------- pressure gradient + implicit move (src/mps.cpp:505-545) ----------------
 
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g, int n, Consts c, Hot cur, Hot out, Cold cold,
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g0, Consts c0, SceneTab tab, int n, Hot cur, Hot out, Cold cold,
                                                       const double* __restrict__ p_s,
-                                                      const double* __restrict__ pmin_s, Events ev, double gscale) {
+                                                      const double* __restrict__ pmin_s, Events ev) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    GridView g = g0;
+    Consts c = c0;
+    scene_of<MULTI>(tab, cur.id[s], g, c);
     double x = cur.px[s], y = cur.py[s], z = DIM == 3 ? cur.pz[s] : 0.0;
     double vx = cur.vx[s], vy = cur.vy[s], vz = DIM == 3 ? cur.vz[s] : 0.0;
     if (cur.type[s] == MPS_FLUID) {
@@ -544,9 +555,9 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g, int n, 
                                gy += dy * dp * w / d2;
                                if (DIM == 3) gz += dz * dp * w / d2;
                            });
-        gx *= gscale;
-        gy *= gscale;
-        gz *= gscale;
+        gx *= c.gscale;
+        gy *= c.gscale;
+        gz *= c.gscale;
         const double ax = -1.0 * gx / c.rho, ay = -1.0 * gy / c.rho, az = -1.0 * gz / c.rho;
         vx += ax * c.dt;
         vy += ay * c.dt;
@@ -572,25 +583,22 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g, int n, 
     }
 }
 
-hipError_t launch_grad_move(int dim, const GridView& g, int n, const Consts& c, const Hot& cur, Hot& out,
-                            Cold& cold, StepScratch& w, Events& ev, hipStream_t st, int bs) {
+hipError_t launch_grad_move(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& cur,
+                            Hot& out, Cold& cold, StepScratch& w, Events& ev, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
-    const double gscale = (double) c.dim / c.n0_p;
-    if (dim == 2)
-        hipLaunchKernelGGL(k_grad_move<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, cur, out, cold, w.p_s, w.pmin_s,
-                           ev, gscale);
-    else
-        hipLaunchKernelGGL(k_grad_move<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, cur, out, cold, w.p_s, w.pmin_s,
-                           ev, gscale);
+    LAUNCH_DM(k_grad_move, grid_for(n, bs), dim3(bs), st, g, c, tab, n, cur, out, cold, w.p_s, w.pmin_s, ev);
     return hipGetLastError();
 }
 
 // ---------------- number density ratio (src/mps.cpp:627-641) ----------------
 
-template <int DIM>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_ndr(GridView g, int n, Consts c, Hot h, Cold cold) {
+template <int DIM, bool MULTI>
+__global__ void __launch_bounds__(kMaxTuneBlock) k_ndr(GridView g0, Consts c0, SceneTab tab, int n, Hot h, Cold cold) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
+    GridView g = g0;
+    Consts c = c0;
+    scene_of<MULTI>(tab, h.id[s], g, c);
     const int id = h.id[s];
     double nd;
     if (h.type[s] != MPS_GHOST) {
@@ -602,13 +610,10 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_ndr(GridView g, int n, Consts
     cold.ndr[id] = cold.bc[id] == MPS_BC_SURFACE ? 0.0 : nd / c.n0_p;
 }
 
-hipError_t launch_ndr(int dim, const GridView& g, int n, const Consts& c, const Hot& h, Cold& cold, hipStream_t st,
-                      int bs) {
+hipError_t launch_ndr(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
+                      Cold& cold, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
-    if (dim == 2)
-        hipLaunchKernelGGL(k_ndr<2>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, cold);
-    else
-        hipLaunchKernelGGL(k_ndr<3>, grid_for(n, bs), dim3(bs), 0, st, g, n, c, h, cold);
+    LAUNCH_DM(k_ndr, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, cold);
     return hipGetLastError();
 }
 
@@ -616,13 +621,22 @@ hipError_t launch_ndr(int dim, const GridView& g, int n, const Consts& c, const 
 // max |v| over fluid; (|v| dt)/l0 is monotone in |v|, so the host applies it
 // to the maximum and gets the reference's value bit for bit.
 
-__global__ void __launch_bounds__(kBlock) k_vmax(int n, Hot h, int dim3d, unsigned long long* __restrict__ out) {
+template <bool MULTI>
+__global__ void __launch_bounds__(kBlock) k_vmax(int n, Hot h, int dim3d, const int16_t* __restrict__ sid,
+                                                 unsigned long long* __restrict__ out) {
     __shared__ double red[kBlock / 64];
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     double v = 0.0;
+    int si = 0;
     if (s < n && h.type[s] == MPS_FLUID) {
         const double a = h.vx[s], b = h.vy[s], cz = dim3d ? h.vz[s] : 0.0;
         v = sqrt((a * a + b * b) + cz * cz);
+    }
+    if constexpr (MULTI) {
+        // per-scene maxima: one atomic per lane that carries a value
+        if (s < n) si = sid[h.id[s]];
+        if (v > 0.0) atomicMax(&out[si], (unsigned long long) __double_as_longlong(v));
+        return;
     }
     for (int o = 32; o > 0; o >>= 1) {
         const double u = __shfl_down(v, o, 64);
@@ -637,11 +651,16 @@ __global__ void __launch_bounds__(kBlock) k_vmax(int n, Hot h, int dim3d, unsign
     }
 }
 
-hipError_t launch_courant(int dim, int n, const Hot& h, double* vmax, hipStream_t st) {
-    hipError_t e = fill_double(vmax, 0.0, 1, st);
+hipError_t launch_courant(int dim, int n, const Hot& h, const SceneTab& tab, int nscene, double* vmax,
+                          hipStream_t st) {
+    hipError_t e = fill_double(vmax, 0.0, (size_t) nscene, st);
     if (e != hipSuccess || n == 0) return e;
-    hipLaunchKernelGGL(k_vmax, grid_for(n, kBlock), dim3(kBlock), 0, st, n, h, dim == 3 ? 1 : 0,
-                       reinterpret_cast<unsigned long long*>(vmax));
+    if (tab.g)
+        hipLaunchKernelGGL(k_vmax<true>, grid_for(n, kBlock), dim3(kBlock), 0, st, n, h, dim == 3 ? 1 : 0, tab.sid,
+                           reinterpret_cast<unsigned long long*>(vmax));
+    else
+        hipLaunchKernelGGL(k_vmax<false>, grid_for(n, kBlock), dim3(kBlock), 0, st, n, h, dim == 3 ? 1 : 0, tab.sid,
+                           reinterpret_cast<unsigned long long*>(vmax));
     return hipGetLastError();
 }
 

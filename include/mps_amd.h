@@ -223,6 +223,34 @@ int mps_autotune(mps_handle* h, int32_t steps_per_candidate, int32_t block_sizes
  * vectors each read or written once).  Leaves the solver scratch undefined. */
 int mps_time_pressure_kernels(mps_handle* h, int32_t reps, double out_ms[4], int64_t out_bytes[4]);
 
+/* Reverse-mode gradients of the step (no counterpart in the reference, which
+ * has no derivatives; an extension for inverse problems).
+ *
+ * mps_adjoint_record runs `steps` steps from the current state, exactly as
+ * mps_run_steps does (courants may be NULL), and keeps a device checkpoint of
+ * the state before every `checkpoint_interval`-th step.  The handle ends at
+ * the final state.
+ *
+ * mps_adjoint_backward then takes dL/dx and dL/dv at that final state
+ * (pos_bar, vel_bar: N x 3, id order; either may be NULL for zero) and returns
+ * dL/dx and dL/dv at the recorded run's start state (pos_bar0, vel_bar0:
+ * N x 3) and param_bar = {dL/dnu (kinematic viscosity), dL/de (coefficient of
+ * restitution), dL/drelaxation (relaxation coefficient for pressure)}.  Each
+ * segment between checkpoints is recomputed once to recover its step starts;
+ * each step is then run again with its hooks on and reversed.  Every discrete
+ * decision of the recorded run is held fixed (neighbour sets, boundary
+ * conditions, Dirichlet connectivity, collision pairs that fired, the
+ * negative-pressure clamp, the minimum-pressure argument, ghosting), so the
+ * result is the derivative of the piecewise-smooth map on the piece the run
+ * lies in.  The adjoint pressure equation is solved with the recorded run's
+ * solver options (zero start); adjoint_stats (may be NULL) gets the total
+ * iterations and the worst final relative residual.  Afterwards the handle is
+ * back at the final state with its events and statistics unchanged.
+ * Single-scene handles without the higher-order source term only. */
+int mps_adjoint_record(mps_handle* h, int32_t steps, int32_t checkpoint_interval, double* courants);
+int mps_adjoint_backward(mps_handle* h, const double* pos_bar, const double* vel_bar, double* pos_bar0,
+                         double* vel_bar0, double* param_bar, mps_solver_stats* adjoint_stats);
+
 #ifdef __cplusplus
 }
 #endif

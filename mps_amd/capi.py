@@ -29,6 +29,7 @@ EXPORTED = [
     "mps_dump_neighbors", "mps_get_pressure_system", "mps_set_snapshot", "mps_get_snapshot",
     "mps_time_pressure_kernels", "mps_set_block_sizes", "mps_autotune",
     "mps_create_ensemble", "mps_scene_count", "mps_ensemble_stats",
+    "mps_adjoint_record", "mps_adjoint_backward",
 ]
 
 
@@ -106,6 +107,8 @@ def lib():
     L.mps_create_ensemble.argtypes = [i32, vp, vp, vp, vp, vp, vp, ctypes.POINTER(vp)]
     L.mps_scene_count.argtypes = [vp]
     L.mps_ensemble_stats.argtypes = [vp, vp, vp]
+    L.mps_adjoint_record.argtypes = [vp, i32, i32, vp]
+    L.mps_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.POINTER(SolverStats)]
     for name in EXPORTED:
         if name not in ("mps_last_error",):
             getattr(L, name).restype = i32
@@ -160,6 +163,26 @@ class Simulation:
         cn = np.zeros(self.nscene)
         self._check(self.L.mps_ensemble_stats(self.h, ctypes.cast(st, ctypes.c_void_p), _p(cn)))
         return [x.as_dict() for x in st], cn
+
+    def adjoint_record(self, steps, checkpoint_interval=50):
+        """Runs `steps` steps keeping device checkpoints for the reverse sweep;
+        returns the Courant numbers."""
+        out = np.zeros(max(steps, 1))
+        self._check(self.L.mps_adjoint_record(self.h, int(steps), int(checkpoint_interval), _p(out)))
+        return out[:steps]
+
+    def adjoint_backward(self, pos_bar, vel_bar=None):
+        """dL/dx, dL/dv at the recorded run's start and dL/d(nu, restitution,
+        relaxation) from dL/dx, dL/dv at its end (N x 3 arrays, id order)."""
+        pb = np.ascontiguousarray(pos_bar, np.float64).reshape(self.n, 3)
+        vb = None if vel_bar is None else np.ascontiguousarray(vel_bar, np.float64).reshape(self.n, 3)
+        pos0 = np.zeros((self.n, 3))
+        vel0 = np.zeros((self.n, 3))
+        par = np.zeros(3)
+        st = SolverStats()
+        self._check(self.L.mps_adjoint_backward(self.h, _p(pb), _p(vb), _p(pos0), _p(vel0), _p(par),
+                                                ctypes.byref(st)))
+        return pos0, vel0, par, st.as_dict()
 
     def scene_state(self, k, state=None):
         """The fields of scene k (local ids) out of a state() dict."""

@@ -28,6 +28,26 @@ struct SceneInfo {
     int n = 0;
 };
 
+// Device memory of the reverse-mode step (allocated on first use).
+struct AdjStore {
+    int steps = 0, interval = 0, ready = 0;
+    mps_solver_options opt{};          // solver options of the recorded run
+    std::vector<double*> cpf;          // checkpoints: state before steps 0, M, 2M, ...
+    std::vector<int8_t*> cpt;
+    double* finf = nullptr;            // state after the recorded run
+    int8_t* fint = nullptr;
+    std::vector<double*> segf;         // step starts of the segment being reversed
+    std::vector<int8_t*> segt;
+    bool bufs = false;
+    AdjBufs a{};
+    int8_t* type_g = nullptr;          // slot-order types at the pressure-gradient stage
+    CollTape coll{};
+    int* ident = nullptr;              // 0..n-1
+    int32_t* key0 = nullptr;           // keys of the start-state view
+    Hot hs{};                          // start state rebuilt into slots
+    std::vector<void*> owned;          // everything above, for release
+};
+
 struct mps_handle {
     std::string err;
     mps_settings s{};  // scene 0
@@ -84,6 +104,7 @@ struct mps_handle {
     int tfam[16] = {};
     int nsites = 0;
     double tacc[9] = {};
+    AdjStore adj;
 };
 
 // Kernel families the autotuner sizes independently.
@@ -348,6 +369,8 @@ static void free_all(mps_handle* h) {
                     h->ev.ghost_ids, h->ev.unconn_ids, h->snap_pos,  h->snap_vel,     h->d_g,         h->d_c,
                     h->d_sid,       h->d_key0};
     for (void* p : ptrs) hipFree(p);
+    for (void* p : h->adj.owned) hipFree(p);
+    h->adj.owned.clear();
     if (h->pinned_i) hipHostFree(h->pinned_i);
     if (h->pinned_d) hipHostFree(h->pinned_d);
     if (h->cg) cg_destroy(h->cg);
@@ -415,7 +438,16 @@ static int number_density(mps_handle* h) {
     return MPS_OK;
 }
 
-static int step_core(mps_handle* h, int is_export, double* vmax_out) {
+// Hooks of a step run for the reverse sweep: the collision record, the
+// slot-order types at the pressure-gradient stage (before that stage ghosts
+// anything) and the hot view of the pressure stage's bucket build.
+struct StepTap {
+    const CollTape* coll;
+    int8_t* type_g;
+    Hot c3;
+};
+
+static int step_core(mps_handle* h, int is_export, double* vmax_out, StepTap* tap = nullptr) {
     int rc;
     const int n = h->n;
     // setNeighbors #1, gravity, viscosity, move
@@ -435,13 +467,15 @@ static int step_core(mps_handle* h, int is_export, double* vmax_out) {
         GridView g = gview(h);
         Hot c = cur(h);
         tbeg(h, F_COLL);
-        HIPCHK(h, launch_collision(h->dim, g, tab(h), n, h->c, c, h->w, h->ev.err, h->st, h->bs[F_COLL]));
+        HIPCHK(h, launch_collision(h->dim, g, tab(h), n, h->c, c, h->w, h->ev.err, h->st, h->bs[F_COLL],
+                                   tap ? tap->coll : nullptr));
         tend(h);
     }
     // setNeighbors #3, pressure
     if ((rc = do_build(h))) return rc;
     GridView g = gview(h);
     Hot c = cur(h);
+    if (tap) tap->c3 = c;
     if (h->want_snap) HIPCHK(h, launch_snapshot(h->dim, n, c, h->cold, h->snap_pos, h->snap_vel, h->st));
     if (h->want_raw) HIPCHK(h, hipMemcpyAsync(h->dbg_id, c.id, sizeof(int32_t) * (size_t) n, hipMemcpyDeviceToDevice, h->st));
     tbeg(h, F_BCSRC);
@@ -463,6 +497,7 @@ static int step_core(mps_handle* h, int is_export, double* vmax_out) {
     tend(h);
     h->have_system = h->want_raw;
     // pressure gradient + move
+    if (tap && n > 0) HIPCHK(h, hipMemcpyAsync(tap->type_g, c.type, (size_t) n, hipMemcpyDeviceToDevice, h->st));
     {
         Hot o = hot_view(h, 1 - h->pv, h->it);
         tbeg(h, F_GRAD);
@@ -492,6 +527,193 @@ static double courant_of(mps_handle* h, const double* vmax) {
         mx = (mx < cn) ? cn : mx;
     }
     return mx;
+}
+
+
+// ---------------- reverse-mode step: device memory and one step's sweep ----------------
+
+template <typename T>
+static int adj_alloc(mps_handle* h, T** p, size_t count) {
+    HIPCHK(h, dalloc(p, count));
+    h->adj.owned.push_back((void*) *p);
+    return MPS_OK;
+}
+
+static void adj_free(mps_handle* h, void* p) {
+    if (!p) return;
+    hipFree(p);
+    auto& o = h->adj.owned;
+    o.erase(std::remove(o.begin(), o.end(), p), o.end());
+}
+
+static int adj_alloc_state(mps_handle* h, double** f, int8_t** t) {
+    const size_t N = (size_t) h->n;
+    int rc = adj_alloc(h, f, (size_t) kStateDoubles * N);
+    if (!rc) rc = adj_alloc(h, t, (size_t) kStateBytes * N);
+    return rc;
+}
+
+static void adj_free_states(mps_handle* h, std::vector<double*>& f, std::vector<int8_t*>& t) {
+    for (double* p : f) adj_free(h, p);
+    for (int8_t* p : t) adj_free(h, p);
+    f.clear();
+    t.clear();
+}
+
+static int adj_alloc_bufs(mps_handle* h) {
+    AdjStore& S = h->adj;
+    if (S.bufs) return MPS_OK;
+    const size_t N = (size_t) h->n;
+    const size_t cap = (size_t) h->w.coll_cap;
+    AdjBufs& a = S.a;
+    int rc = MPS_OK;
+    for (int d = 0; d < 3 && !rc; d++) {
+        rc = adj_alloc(h, &a.xb[d], N);
+        if (!rc) rc = adj_alloc(h, &a.vb[d], N);
+        if (!rc) rc = adj_alloc(h, &a.gb[d], N);
+        if (!rc) rc = adj_alloc(h, &a.ab[d], N);
+    }
+    if (!rc) rc = adj_alloc(h, &a.pb, N);
+    if (!rc) rc = adj_alloc(h, &a.phb, N);
+    if (!rc) rc = adj_alloc(h, &a.amin, N);
+    if (!rc) rc = adj_alloc(h, &a.rhs, N);
+    if (!rc) rc = adj_alloc(h, &a.lam, N);
+    if (!rc) rc = adj_alloc(h, &a.cnu, N);
+    if (!rc) rc = adj_alloc(h, &a.crelax, N);
+    if (!rc) rc = adj_alloc(h, &a.cpair, adj_pair_scratch((int) cap));
+    if (!rc) rc = adj_alloc(h, &a.part, adj_partials(h->n));
+    if (!rc) rc = adj_alloc(h, &a.acc, 3);
+    if (!rc) rc = adj_alloc(h, &S.type_g, N);
+    if (!rc) rc = adj_alloc(h, &S.coll.pid, 2 * cap);
+    if (!rc) rc = adj_alloc(h, &S.coll.flags, cap);
+    if (!rc) rc = adj_alloc(h, &S.coll.pre, 12 * cap);
+    if (!rc) rc = adj_alloc(h, &S.coll.dir, 3 * cap);
+    if (!rc) rc = adj_alloc(h, &S.coll.nlev, 1);
+    if (!rc) rc = adj_alloc(h, &S.ident, N);
+    if (!rc) rc = adj_alloc(h, &S.key0, N);
+    const int d3 = h->dim == 3;
+    if (!rc) rc = adj_alloc(h, &S.hs.px, N);
+    if (!rc) rc = adj_alloc(h, &S.hs.py, N);
+    if (!rc) rc = adj_alloc(h, &S.hs.vx, N);
+    if (!rc) rc = adj_alloc(h, &S.hs.vy, N);
+    if (!rc && d3) rc = adj_alloc(h, &S.hs.pz, N);
+    if (!rc && d3) rc = adj_alloc(h, &S.hs.vz, N);
+    if (!rc) rc = adj_alloc(h, &S.hs.id, N);
+    if (!rc) rc = adj_alloc(h, &S.hs.type, N);
+    if (!rc) rc = adj_alloc(h, &S.hs.key, N);
+    if (rc) return rc;
+    if (N) {
+        std::vector<int> ids(N);
+        for (size_t i = 0; i < N; i++) ids[i] = (int) i;
+        HIPCHK(h, hipMemcpy(S.ident, ids.data(), N * sizeof(int), hipMemcpyHostToDevice));
+    }
+    S.bufs = true;
+    return MPS_OK;
+}
+
+static int adj_save(mps_handle* h, double* f, int8_t* t) {
+    HIPCHK(h, save_state(h->dim, h->n, cur(h), h->cold, f, t, h->st));
+    return MPS_OK;
+}
+
+static int adj_restore(mps_handle* h, const double* f, const int8_t* t) {
+    Hot c = cur(h);
+    HIPCHK(h, restore_state(h->dim, h->n, c, h->cold, h->bb.slot_of, f, t, h->st));
+    return MPS_OK;
+}
+
+// Reverse of one step.  The step has just been run from the saved start
+// state (f0, t0) with its hooks on, so the pressure stage's slots, boundary
+// conditions, matrix, solution and collision record are those of the forward
+// step.  Adds the step's transpose to the adjoint accumulators.
+static int adj_reverse_step(mps_handle* h, const StepTap& tap, const double* f0, const int8_t* t0,
+                            const mps_solver_options& oa, mps_solver_stats& agg) {
+    AdjStore& S = h->adj;
+    AdjBufs& a = S.a;
+    const int n = h->n, dim = h->dim;
+    const size_t N = (size_t) n;
+    GridView g = gview(h);
+    HIPCHK(h, adj_pressure_stage(dim, g, n, h->c, tap.c3, S.type_g, h->w, h->x_s, a, h->st));
+    mps_solver_stats ss{};
+    std::string msg;
+    HIPCHK(h, cg_solve(h->cg, h->A, tap.c3, n, a.rhs, nullptr, a.lam, oa, &ss, h->st, msg));
+    agg.rows = std::max(agg.rows, ss.rows);
+    agg.iterations += ss.iterations;
+    agg.refinements += ss.refinements;
+    agg.converged = agg.converged && ss.converged;
+    agg.rel_residual = std::max(agg.rel_residual, ss.rel_residual);
+    agg.initial_residual = std::max(agg.initial_residual, ss.initial_residual);
+    agg.rhs_norm = std::max(agg.rhs_norm, ss.rhs_norm);
+    HIPCHK(h, adj_ppe_stage(dim, g, n, h->c, tap.c3, S.type_g, h->w, h->x_s, a, h->st));
+    HIPCHK(h, adj_collision_stage(dim, n, h->c, h->w, S.coll, a, h->st));
+    // the start state, rebuilt into slots, for the viscosity stage
+    Hot h0{};
+    h0.px = const_cast<double*>(f0);
+    h0.py = const_cast<double*>(f0) + N;
+    h0.vx = const_cast<double*>(f0) + 3 * N;
+    h0.vy = const_cast<double*>(f0) + 4 * N;
+    if (dim == 3) {
+        h0.pz = const_cast<double*>(f0) + 2 * N;
+        h0.vz = const_cast<double*>(f0) + 5 * N;
+    }
+    h0.id = S.ident;
+    h0.type = const_cast<int8_t*>(t0);
+    h0.key = S.key0;
+    HIPCHK(h, build_grid(dim, h->g, tab(h), n, h0, S.hs, h->bb, h->ev.err, h->st, h->bs[F_BUILD]));
+    HIPCHK(h, adj_explicit_stage(dim, gview(h), n, h->c, t0, S.hs, a, h->st));
+    return MPS_OK;
+}
+
+static int adj_backward_impl(mps_handle* h, const double* pos_bar, const double* vel_bar, mps_solver_stats& agg) {
+    AdjStore& S = h->adj;
+    const int n = h->n;
+    const size_t N = (size_t) n;
+    const int M = S.interval;
+    int rc = adj_alloc_bufs(h);
+    if (rc) return rc;
+    if ((int) S.segf.size() < M) {
+        adj_free_states(h, S.segf, S.segt);
+        for (int k = 0; k < M; k++) {
+            double* f;
+            int8_t* t;
+            if ((rc = adj_alloc_state(h, &f, &t))) return rc;
+            S.segf.push_back(f);
+            S.segt.push_back(t);
+        }
+    }
+    // seeds dL/dx(T), dL/dv(T)
+    std::vector<double> comp(N);
+    for (int d = 0; d < 3; d++) {
+        for (size_t i = 0; i < N; i++) comp[i] = pos_bar ? pos_bar[3 * i + d] : 0.0;
+        if (N) HIPCHK(h, hipMemcpy(S.a.xb[d], comp.data(), N * sizeof(double), hipMemcpyHostToDevice));
+        for (size_t i = 0; i < N; i++) comp[i] = vel_bar ? vel_bar[3 * i + d] : 0.0;
+        if (N) HIPCHK(h, hipMemcpy(S.a.vb[d], comp.data(), N * sizeof(double), hipMemcpyHostToDevice));
+    }
+    HIPCHK(h, fill_double(S.a.acc, 0.0, 3, h->st));
+    mps_solver_options oa = S.opt;
+    oa.warm_start = 0;
+    agg = mps_solver_stats{};
+    agg.converged = 1;
+    const int nseg = (S.steps + M - 1) / M;
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+        const int k0 = sg * M, k1 = std::min(S.steps, k0 + M);
+        if ((rc = adj_restore(h, S.cpf[sg], S.cpt[sg]))) return rc;
+        for (int k = k0; k < k1; k++) {
+            if ((rc = adj_save(h, S.segf[k - k0], S.segt[k - k0]))) return rc;
+            if (k + 1 < k1) {
+                if ((rc = step_core(h, 0, h->pinned_d))) return rc;
+                if ((rc = drain(h))) return rc;
+            }
+        }
+        for (int k = k1 - 1; k >= k0; k--) {
+            if ((rc = adj_restore(h, S.segf[k - k0], S.segt[k - k0]))) return rc;
+            StepTap tap{&S.coll, S.type_g, Hot{}};
+            if ((rc = step_core(h, 0, h->pinned_d, &tap))) return rc;
+            if ((rc = drain(h))) return rc;
+            if ((rc = adj_reverse_step(h, tap, S.segf[k - k0], S.segt[k - k0], oa, agg))) return rc;
+        }
+    }
+    return MPS_OK;
 }
 
 // ---------------- C ABI ----------------
@@ -1170,6 +1392,90 @@ int mps_time_pressure_kernels(mps_handle* h, int32_t reps, double out_ms[4], int
     long long bytes[4];
     HIPCHK(h, cg_time_kernels(h->cg, h->A, reps, nnz, h->st, out_ms, bytes));
     for (int k = 0; k < 4; k++) out_bytes[k] = bytes[k];
+    return MPS_OK;
+}
+
+int mps_adjoint_record(mps_handle* h, int32_t steps, int32_t checkpoint_interval, double* courants) {
+    if (!h || steps < 1 || checkpoint_interval < 1) return MPS_E_INVALID;
+    if (h->nscene != 1) return fail(h, MPS_E_INVALID, "the adjoint takes a single-scene handle");
+    if (h->c.hos_on) return fail(h, MPS_E_INVALID, "the adjoint does not cover the higher-order source term");
+    AdjStore& S = h->adj;
+    S.ready = 0;
+    adj_free_states(h, S.cpf, S.cpt);
+    adj_free(h, S.finf);
+    adj_free(h, S.fint);
+    S.finf = nullptr;
+    S.fint = nullptr;
+    const int ncp = (steps + checkpoint_interval - 1) / checkpoint_interval;
+    int rc;
+    for (int k = 0; k < ncp; k++) {
+        double* f;
+        int8_t* t;
+        if ((rc = adj_alloc_state(h, &f, &t))) return rc;
+        S.cpf.push_back(f);
+        S.cpt.push_back(t);
+    }
+    if ((rc = adj_alloc_state(h, &S.finf, &S.fint))) return rc;
+    S.steps = steps;
+    S.interval = checkpoint_interval;
+    S.opt = h->opt;
+    for (int k = 0; k < steps; k++) {
+        if (k % checkpoint_interval == 0)
+            if ((rc = adj_save(h, S.cpf[k / checkpoint_interval], S.cpt[k / checkpoint_interval]))) return rc;
+        if ((rc = step_core(h, 0, h->pinned_d))) return rc;
+        if ((rc = drain(h))) return rc;
+        const double cn = courant_of(h, h->pinned_d);
+        if (courants) courants[k] = cn;
+    }
+    if ((rc = adj_save(h, S.finf, S.fint))) return rc;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    S.ready = 1;
+    return MPS_OK;
+}
+
+int mps_adjoint_backward(mps_handle* h, const double* pos_bar, const double* vel_bar, double* pos_bar0,
+                         double* vel_bar0, double* param_bar, mps_solver_stats* adjoint_stats) {
+    if (!h || !pos_bar0 || !vel_bar0 || !param_bar) return MPS_E_INVALID;
+    AdjStore& S = h->adj;
+    if (!S.ready) return fail(h, MPS_E_INVALID, "no recorded run: call mps_adjoint_record first");
+    int rc = drain(h);
+    if (rc) return rc;
+    // the recomputation repeats the recorded steps: keep the handle's events,
+    // statistics and options as they were
+    const int64_t counts0[3] = {h->counts[0], h->counts[1], h->counts[2]};
+    const std::vector<int32_t> ghost0 = h->ghost_ids, unconn0 = h->unconn_ids;
+    const std::vector<mps_solver_stats> sstats0 = h->sstats;
+    const std::vector<double> scourant0 = h->scourant;
+    const mps_solver_options opt0 = h->opt;
+    h->opt = S.opt;
+    mps_solver_stats agg{};
+    rc = adj_backward_impl(h, pos_bar, vel_bar, agg);
+    const std::string err = h->err;
+    h->opt = opt0;
+    int rc2 = adj_restore(h, S.finf, S.fint);
+    if (!rc2) rc2 = drain(h);
+    h->counts[0] = counts0[0];
+    h->counts[1] = counts0[1];
+    h->counts[2] = counts0[2];
+    h->ghost_ids = ghost0;
+    h->unconn_ids = unconn0;
+    h->sstats = sstats0;
+    h->scourant = scourant0;
+    if (rc) {
+        h->err = err;
+        return rc;
+    }
+    if (rc2) return rc2;
+    const size_t N = (size_t) h->n;
+    std::vector<double> comp(N);
+    for (int d = 0; d < 3; d++) {
+        if (N) HIPCHK(h, hipMemcpy(comp.data(), S.a.xb[d], N * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; i++) pos_bar0[3 * i + d] = comp[i];
+        if (N) HIPCHK(h, hipMemcpy(comp.data(), S.a.vb[d], N * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; i++) vel_bar0[3 * i + d] = comp[i];
+    }
+    HIPCHK(h, hipMemcpy(param_bar, S.a.acc, 3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (adjoint_stats) *adjoint_stats = agg;
     return MPS_OK;
 }
 

@@ -249,10 +249,20 @@ struct StepScratch {
     int* scan_tmp;
 };
 
+// Record of one step's collision pass for the reverse sweep (every pointer
+// null: recording off).  Indexed like the candidate pair list.
+struct CollTape {
+    int* pid;     // 2 per pair: particle ids (i, j)
+    int* flags;   // per pair: bit 0 the impulse was applied, bit 1 j is fluid
+    double* pre;  // 12 per pair: x_i, x_j, v_i, v_j (3 each) just before the pair acted
+    double* dir;  // 3 per pair: (x_j - x_i) / dist at the bucket build
+    int* nlev;    // number of levels
+};
+
 hipError_t launch_visc_move(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& cur,
                             Hot& out, Cold& cold, Events& ev, hipStream_t st, int bs);
 hipError_t launch_collision(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, Hot& h,
-                            StepScratch& w, int* err, hipStream_t st, int bs);
+                            StepScratch& w, int* err, hipStream_t st, int bs, const CollTape* tape = nullptr);
 hipError_t launch_bc_source(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
                             Cold& cold, StepScratch& w, hipStream_t st, int bs);
 hipError_t launch_diag_cc(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
@@ -312,5 +322,39 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
                     std::string& msg);
 hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, hipStream_t st, double ms[4],
                            long long bytes[4]);
+
+// adjoint.hip — reverse-mode step
+struct AdjBufs {
+    double* xb[3];   // dL/dx, id order
+    double* vb[3];   // dL/dv, id order
+    double* gb[3];   // adjoint of the pressure-gradient sum, slots of the pressure build
+    double* pb;      // adjoint of the clamped pressure (slots)
+    double* phb;     // adjoint of the minimum pressure (slots)
+    int* amin;       // slot holding each slot's minimum pressure (-1: Ignored)
+    double* rhs;     // adjoint Poisson right-hand side (slots)
+    double* lam;     // adjoint Poisson solution (slots)
+    double* ab[3];   // adjoint of the explicit acceleration, id order
+    double* cnu;     // per-slot terms of dL/dnu
+    double* crelax;  // per-slot terms of dL/drelaxation
+    double* cpair;   // per-pair scratch of the collision sweep
+    double* part;    // block partials of the parameter sums
+    double* acc;     // dL/dnu, dL/drestitution, dL/drelaxation
+};
+constexpr int kStateDoubles = 11;  // doubles per particle of a saved state
+constexpr int kStateBytes = 2;     // bytes per particle of a saved state
+hipError_t save_state(int dim, int n, const Hot& h, const Cold& cold, double* f, int8_t* t8, hipStream_t st);
+hipError_t restore_state(int dim, int n, Hot& h, Cold& cold, int* slot_of, const double* f, const int8_t* t8,
+                         hipStream_t st);
+size_t adj_partials(int n);
+size_t adj_pair_scratch(int cap);
+hipError_t adj_pressure_stage(int dim, const GridView& g, int n, const Consts& c, const Hot& h,
+                              const int8_t* type_g, const StepScratch& w, const double* x_s, AdjBufs& a,
+                              hipStream_t st);
+hipError_t adj_ppe_stage(int dim, const GridView& g, int n, const Consts& c, const Hot& h, const int8_t* type_g,
+                         const StepScratch& w, const double* x_s, AdjBufs& a, hipStream_t st);
+hipError_t adj_collision_stage(int dim, int n, const Consts& c, const StepScratch& w, const CollTape& tp, AdjBufs& a,
+                               hipStream_t st);
+hipError_t adj_explicit_stage(int dim, const GridView& g, int n, const Consts& c, const int8_t* type0_id,
+                              const Hot& hs, AdjBufs& a, hipStream_t st);
 
 }  // namespace mps

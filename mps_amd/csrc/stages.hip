@@ -145,7 +145,8 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_coll_count(GridView g0, Const
 template <int DIM, bool MULTI>
 __global__ void __launch_bounds__(kMaxTuneBlock) k_coll_emit(GridView g0, Consts c0, SceneTab tab, int n, Hot h, const int* __restrict__ cnt,
                                                       const int* __restrict__ off, int* __restrict__ pairs,
-                                                      double* __restrict__ pdist, int cap, int* __restrict__ err) {
+                                                      double* __restrict__ pdist, int cap, int* __restrict__ err,
+                                                      double* __restrict__ pdir) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     GridView g = g0;
@@ -157,12 +158,17 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_coll_emit(GridView g0, Consts
     const double re = c.dc < c.re_max ? c.dc : c.re_max;
     const double zi = DIM == 3 ? h.pz[s] : 0.0;
     for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
-                       [&](int q, double dist, double, double, double) {
+                       [&](int q, double dist, double dx, double dy, double dz) {
                            if (h.type[q] == MPS_FLUID && h.id[q] >= id) return;
                            if (o < cap) {
                                pairs[2 * o] = s;
                                pairs[2 * o + 1] = q;
                                pdist[o] = dist;
+                               if (pdir) {  // d dist / d x_j, kept for the reverse sweep
+                                   pdir[3 * (size_t) o] = dx / dist;
+                                   pdir[3 * (size_t) o + 1] = dy / dist;
+                                   pdir[3 * (size_t) o + 2] = dz / dist;
+                               }
                            } else {
                                atomicOr(err, kErrCollCap);
                            }
@@ -178,7 +184,7 @@ __global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts 
                                                                   const int* __restrict__ pairs,
                                                                   const double* __restrict__ pdist,
                                                                   int* __restrict__ level, int* __restrict__ last,
-                                                                  int cap) {
+                                                                  int cap, CollTape tp) {
     __shared__ int s_max;
     int np = off[n];
     if (np > cap) np = cap;
@@ -197,6 +203,7 @@ __global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts 
             if (L > mx) mx = L;
         }
         s_max = mx;
+        if (tp.nlev) *tp.nlev = mx;
     }
     __syncthreads();
     volatile double* px = h.px;
@@ -230,6 +237,24 @@ __global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts 
             }
             const double dvx = vx[j] - vx[i], dvy = vy[j] - vy[i], dvz = d3 ? vz[j] - vz[i] : 0.0;
             const double rv = (dvx * nx + dvy * ny) + dvz * nz;
+            if (tp.flags) {  // record for the reverse sweep
+                double* pre = tp.pre + 12 * (size_t) p;
+                pre[0] = px[i];
+                pre[1] = py[i];
+                pre[2] = d3 ? pz[i] : 0.0;
+                pre[3] = px[j];
+                pre[4] = py[j];
+                pre[5] = d3 ? pz[j] : 0.0;
+                pre[6] = vx[i];
+                pre[7] = vy[i];
+                pre[8] = d3 ? vz[i] : 0.0;
+                pre[9] = vx[j];
+                pre[10] = vy[j];
+                pre[11] = d3 ? vz[j] : 0.0;
+                tp.pid[2 * p] = h.id[i];
+                tp.pid[2 * p + 1] = h.id[j];
+                tp.flags[p] = (rv < 0.0 ? 1 : 0) | (jf ? 2 : 0);
+            }
             if (rv < 0.0) {
                 const double impulse = -(1.0 + e) * rv * reduced;
                 const double si = impulse * invMi, sj = impulse * invMj;
@@ -262,19 +287,20 @@ __global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts 
 }
 
 hipError_t launch_collision(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, Hot& h,
-                            StepScratch& w, int* err, hipStream_t st, int bs) {
+                            StepScratch& w, int* err, hipStream_t st, int bs, const CollTape* tape) {
     if (n == 0) return hipSuccess;
+    const CollTape tp = tape ? *tape : CollTape{nullptr, nullptr, nullptr, nullptr, nullptr};
     LAUNCH_DM(k_coll_count, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, w.coll_cnt);
     hipError_t e = scan_exclusive(w.coll_cnt, w.coll_off, n, w.scan_tmp, st);
     if (e != hipSuccess) return e;
     LAUNCH_DM(k_coll_emit, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, w.coll_cnt, w.coll_off, w.coll_pairs,
-              w.coll_dist, w.coll_cap, err);
+              w.coll_dist, w.coll_cap, err, tp.dir);
     if (tab.g)
         hipLaunchKernelGGL(k_coll_resolve<true>, dim3(1), dim3(kResolveThreads), 0, st, n, c, tab, dim, h, w.coll_off,
-                           w.coll_pairs, w.coll_dist, w.coll_level, w.coll_last, w.coll_cap);
+                           w.coll_pairs, w.coll_dist, w.coll_level, w.coll_last, w.coll_cap, tp);
     else
         hipLaunchKernelGGL(k_coll_resolve<false>, dim3(1), dim3(kResolveThreads), 0, st, n, c, tab, dim, h, w.coll_off,
-                           w.coll_pairs, w.coll_dist, w.coll_level, w.coll_last, w.coll_cap);
+                           w.coll_pairs, w.coll_dist, w.coll_level, w.coll_last, w.coll_cap, tp);
     return hipGetLastError();
 }
 

@@ -28,12 +28,12 @@ HBM_PEAK_GBS = 8000.0
 TOL = 1e-9
 
 
-def run_path(settings, scene, precision, steps, warmup, check_interval, reps, use_graph=True):
+def run_path(settings, scene, precision, steps, warmup, check_interval, reps, use_graph=True, precond=1):
     """settings, scene: one scene; settings = list of (settings, scene), scene =
     None: an ensemble in one handle."""
     sim = capi.Simulation(settings, scene)
     sim.set_solver(precision=precision, tolerance=TOL, warm_start=True, check_interval=check_interval,
-                   inner_tolerance=1e-4, use_graph=use_graph)
+                   inner_tolerance=1e-4, use_graph=use_graph, preconditioner=precond)
     sim.load_initial_state()
     for _ in range(warmup):
         sim.step()
@@ -87,11 +87,15 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="dambreak2d_2M")
-    ap.add_argument("--check-interval", type=int, default=32)
+    ap.add_argument("--check-interval", type=int, default=8)
     ap.add_argument("--reps", type=int, default=30, help="launches per kernel for the event timing")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="enqueue CG chunks directly (for profilers)")
+    ap.add_argument("--precond", choices=("multigrid", "jacobi"), default="multigrid",
+                    help="CG preconditioner of both paths (ensembles always use jacobi)")
+    ap.add_argument("--paths", default="fp64,mixed", help="paths to run (mixed sets value)")
     a = ap.parse_args()
+    precond = capi.MPS_PRECOND_MULTIGRID if a.precond == "multigrid" else capi.MPS_PRECOND_JACOBI
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {a.gpus} is not supported (the MPS step here is a single-GPU "
               f"path; multi-GPU sharding is a design note only)", file=sys.stderr)
@@ -115,7 +119,9 @@ def main():
 
     out = {}
     for name, prec in (("fp64", capi.MPS_PREC_FP64), ("mixed", capi.MPS_PREC_MIXED)):
-        r = run_path(settings, scene, prec, a.steps, a.warmup, a.check_interval, a.reps, not a.no_graph)
+        if name not in a.paths.split(","):
+            continue
+        r = run_path(settings, scene, prec, a.steps, a.warmup, a.check_interval, a.reps, not a.no_graph, precond)
         worst = max(r["residuals"]) if r["residuals"] else 0.0
         if not worst <= TOL:
             print(f"[bench] {name}: residual {worst:.3e} above {TOL:g}", file=sys.stderr)
@@ -124,6 +130,10 @@ def main():
         print(f"[bench] {name}: {1e3 * r['seconds'] / a.steps:.2f} ms/step, mean CG iterations "
               f"{np.mean(r['iterations']):.1f}, worst residual {worst:.2e}", file=sys.stderr, flush=True)
 
+    if "mixed" not in out or "fp64" not in out:
+        print(json.dumps({k: dict(ms_per_step=1e3 * v["seconds"] / a.steps,
+                                  mean_iterations=float(np.mean(v["iterations"]))) for k, v in out.items()}))
+        return
     mixed, fp64 = out["mixed"], out["fp64"]
     ms_mixed = 1e3 * mixed["seconds"] / a.steps
     ms_fp64 = 1e3 * fp64["seconds"] / a.steps
@@ -166,6 +176,7 @@ def main():
             "dt": settings.dt if scene is not None else "per scene: 0.2 l0 / 3.5",
             "tolerance": TOL,
             "warm_start": True,
+            "preconditioner": a.precond if scene is not None else "jacobi",
             "per_particle_value_basis": "n_fluid",
         },
         "fp64": {

@@ -86,7 +86,16 @@ typedef struct mps_solver_options {
     double inner_tolerance;    /* mixed: relative reduction asked of each inner solve */
     int32_t check_interval;    /* CG iterations between host convergence checks (>=1) */
     int32_t use_graph;         /* 1: replay captured HIP graphs for the CG chunks */
+    int32_t preconditioner;    /* MPS_PRECOND_*: JACOBI is Eigen's default; MULTIGRID
+                                  changes the iterations, not the tolerance met */
+    int32_t reserved;          /* 0 */
 } mps_solver_options;
+
+/* Preconditioners of the pressure CG. */
+#define MPS_PRECOND_JACOBI 0    /* diagonal (Eigen's DiagonalPreconditioner) */
+#define MPS_PRECOND_MULTIGRID 1 /* aggregation multigrid V-cycle on the bucket
+                                   cells; single-scene handles (ensembles use
+                                   JACOBI) */
 
 /* Statistics of the last pressure solve. */
 typedef struct mps_solver_stats {

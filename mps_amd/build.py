@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libmps_amd.so")
 ROOT = os.path.dirname(HERE)
 INCLUDE = os.path.join(ROOT, "include")
 
-SOURCES = ["grid.hip", "stages.hip", "ppe.hip", "adjoint.hip", "capi.hip"]
+SOURCES = ["grid.hip", "stages.hip", "ppe.hip", "mg.hip", "adjoint.hip", "capi.hip"]
 HOST = os.path.join(HERE, "host")
 HOST_SOURCES = ["particle.cpp", "logger.cpp", "settings.cpp", "exporter.cpp", "bucket.cpp", "mps.cpp",
                 "simulation.cpp", "hooks.cpp"]

@@ -18,6 +18,8 @@ LIB_PATH = os.path.join(HERE, "libmps_amd.so")
 
 MPS_PREC_FP64 = 0
 MPS_PREC_MIXED = 1
+MPS_PRECOND_JACOBI = 0
+MPS_PRECOND_MULTIGRID = 1
 
 _ERRORS = {1: "MPS_E_INVALID", 2: "MPS_E_HIP", 3: "MPS_E_NOMEM", 4: "MPS_E_RANGE", 5: "MPS_E_SOLVER",
            6: "MPS_E_ASYM"}
@@ -49,6 +51,8 @@ class SolverOptions(ctypes.Structure):
         ("inner_tolerance", ctypes.c_double),
         ("check_interval", ctypes.c_int32),
         ("use_graph", ctypes.c_int32),
+        ("preconditioner", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -206,7 +210,8 @@ class Simulation:
             pass
 
     def set_solver(self, precision=MPS_PREC_FP64, tolerance=None, warm_start=False, max_iterations=0,
-                   max_refinements=0, inner_tolerance=1e-4, check_interval=32, use_graph=True):
+                   max_refinements=0, inner_tolerance=1e-4, check_interval=32, use_graph=True,
+                   preconditioner=0):
         o = SolverOptions()
         o.precision = precision
         o.warm_start = int(bool(warm_start))
@@ -216,6 +221,7 @@ class Simulation:
         o.inner_tolerance = inner_tolerance
         o.check_interval = check_interval
         o.use_graph = int(bool(use_graph))
+        o.preconditioner = int(preconditioner)
         self._check(self.L.mps_set_solver(self.h, ctypes.byref(o)))
 
     def solver_stats(self):

@@ -943,6 +943,9 @@ static int create_impl(int32_t nscene, const mps_settings* sset, const int32_t* 
     h->opt.inner_tolerance = 1e-4;
     h->opt.check_interval = 32;
     h->opt.use_graph = 1;
+    h->opt.preconditioner = MPS_PRECOND_JACOBI;
+    h->A.dim = h->dim;
+    h->A.grid = h->g;
     *out = h;
     return MPS_OK;
 }
@@ -985,6 +988,8 @@ int mps_set_solver(mps_handle* h, const mps_solver_options* o) {
     if (o->precision != MPS_PREC_FP64 && o->precision != MPS_PREC_MIXED)
         return fail(h, MPS_E_INVALID, "unknown precision");
     if (!(o->tolerance > 0)) return fail(h, MPS_E_INVALID, "tolerance must be positive");
+    if (o->preconditioner != MPS_PRECOND_JACOBI && o->preconditioner != MPS_PRECOND_MULTIGRID)
+        return fail(h, MPS_E_INVALID, "unknown preconditioner");
     h->opt = *o;
     if (h->opt.check_interval < 1) h->opt.check_interval = 1;
     return MPS_OK;

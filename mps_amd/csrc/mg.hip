@@ -1,0 +1,743 @@
+// mg.hip — aggregation multigrid preconditioner for the pressure CG.
+//
+// The Jacobi-scaled system Ahat (unit diagonal, SELL storage) is coarsened
+// along the bucket grid the step already has:
+//   level 1: one aggregate per bucket cell.  Rows are compacted in slot order
+//            and slots are sorted by cell, so a cell's Inner rows are one
+//            contiguous row range;
+//   level l+1 (l >= 1): 2 x 2 (x 2 in 3D) blocks of level-l cells.
+// Interpolation is piecewise constant (P_{i,K} = 1 when row i lies in
+// aggregate K); coarse operators are Galerkin, A_{l+1} = P^T A_l P, built by
+// gathers (one thread per coarse row sums its children's entries in a fixed
+// order), so they are deterministic.  Particles interact only within
+// re_p <= re_max = cell side, so a coarse row couples at most to its 8 (26)
+// neighbouring aggregates: every coarse operator is an ELL matrix of width
+// 9 (27) with the diagonal kept apart.
+//
+// The preconditioner is one V(1,1) cycle: damped Jacobi pre-smoothing from a
+// zero guess, residual, restriction, recursion, prolongation with a factor
+// (over-correction for piecewise-constant interpolation), damped Jacobi
+// post-smoothing; the coarsest level (<= 64 rows) gets a fixed number of
+// Jacobi sweeps in one workgroup.  Pre- and post-smoother are the same
+// symmetric operator and the coarsest solve is a fixed polynomial, so the
+// cycle is a symmetric positive definite linear operator and plain
+// (preconditioned) CG applies.  Every kernel returns at once when the CG's
+// done flag is up, like the CG kernels around it.
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "mps_internal.hpp"
+
+namespace mps {
+
+constexpr int kMgMaxLevels = 16;
+constexpr int kMgCoarse = 64;  // rows of the coarsest level at most
+
+static inline dim3 mg_grid(int n) { return dim3((unsigned) std::max(1, (n + kBlock - 1) / kBlock)); }
+
+__device__ __forceinline__ bool mg_done(const int* done) {
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = done ? *(volatile const int*) done : 0;
+    __syncthreads();
+    return s_done != 0;
+}
+
+template <typename IDX>
+__device__ __forceinline__ int mg_col(const IDX* __restrict__ c, size_t idx, int row) {
+    if constexpr (sizeof(IDX) == 2)
+        return row + (int) c[idx];
+    else
+        return (int) c[idx];
+}
+
+// ---------------- setup ----------------
+
+// Cell key of each compact row and the first row of each cell.
+__global__ void __launch_bounds__(kBlock) k_mg_heads(int R, const int* __restrict__ row_slot,
+                                                     const int32_t* __restrict__ key_s, int* __restrict__ head,
+                                                     int* __restrict__ rkey) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const int k = key_s[row_slot[r]];
+    rkey[r] = k;
+    head[r] = (r == 0 || key_s[row_slot[r - 1]] != k) ? 1 : 0;
+}
+
+// Level 1: fine row -> aggregate, aggregate -> first fine row and its cell.
+__global__ void __launch_bounds__(kBlock) k_mg_level1(int R, const int* __restrict__ head,
+                                                      const int* __restrict__ hscan, const int* __restrict__ rkey,
+                                                      int key0, int* __restrict__ agg0, int* __restrict__ cst1,
+                                                      int* __restrict__ coord1) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const int K = hscan[r] + head[r] - 1;
+    agg0[r] = K;
+    if (head[r]) {
+        cst1[K] = r;
+        coord1[K] = rkey[r] - key0;  // the cell's key is its place on the level-1 grid
+    }
+    if (r == R - 1) cst1[K + 1] = R;
+}
+
+// Galerkin level 1 from the fine SELL matrix (unit diagonal not stored;
+// padding entries point at their own row with value 0).
+template <typename T, typename IDX, int W>
+__global__ void __launch_bounds__(kBlock) k_mg_assemble1(int n1, const int* __restrict__ cst1,
+                                                         const int* __restrict__ slice_pos,
+                                                         const IDX* __restrict__ col, const T* __restrict__ val,
+                                                         const int* __restrict__ agg0, int cap1,
+                                                         int* __restrict__ col1, T* __restrict__ val1,
+                                                         T* __restrict__ diag1, T* __restrict__ dinv1,
+                                                         int* __restrict__ err) {
+    const int K = blockIdx.x * blockDim.x + threadIdx.x;
+    if (K >= n1) return;
+    int cs[W];
+    double vs[W];
+    int m = 0;
+    double dg = 0.0;
+    for (int r = cst1[K]; r < cst1[K + 1]; r++) {
+        dg += 1.0;
+        const int sl = r >> 6, lane = r & 63;
+        const int s0 = slice_pos[sl], w = slice_pos[sl + 1] - s0;
+        const size_t base = (size_t) s0 * 64 + lane;
+        for (int k = 0; k < w; k++) {
+            const size_t idx = base + (size_t) k * 64;
+            const int K2 = agg0[mg_col(col, idx, r)];
+            const double v = (double) val[idx];
+            if (K2 == K) {
+                dg += v;
+                continue;
+            }
+            int j = 0;
+            while (j < m && cs[j] != K2) j++;
+            if (j < m) {
+                vs[j] += v;
+            } else if (m < W) {
+                cs[m] = K2;
+                vs[m] = v;
+                m++;
+            } else {
+                atomicOr(err, 1);
+            }
+        }
+    }
+    for (int j = 0; j < W; j++) {
+        col1[(size_t) j * cap1 + K] = j < m ? cs[j] : -1;
+        val1[(size_t) j * cap1 + K] = j < m ? (T) vs[j] : (T) 0;
+    }
+    diag1[K] = (T) dg;
+    dinv1[K] = (T) (1.0 / dg);
+}
+
+struct MgDims {
+    int x, y, z;
+};
+
+__device__ __forceinline__ void mg_decode(int key, MgDims d, int& cx, int& cy, int& cz) {
+    cx = key % d.x;
+    const int t = key / d.x;
+    cy = t % d.y;
+    cz = t / d.y;
+}
+
+__global__ void __launch_bounds__(kBlock) k_mg_mark(int n, const int* __restrict__ coord, MgDims d, MgDims dp,
+                                                    int* __restrict__ flag) {
+    const int K = blockIdx.x * blockDim.x + threadIdx.x;
+    if (K >= n) return;
+    int cx, cy, cz;
+    mg_decode(coord[K], d, cx, cy, cz);
+    flag[(cx >> 1) + dp.x * ((cy >> 1) + dp.y * (cz >> 1))] = 1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_mg_parents(int G, const int* __restrict__ flag,
+                                                       const int* __restrict__ pscan, int* __restrict__ coordp) {
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= G || !flag[P]) return;
+    coordp[pscan[P]] = P;
+}
+
+// agg[K] = parent row; the parent's child slot (cx&1) + 2(cy&1) + 4(cz&1) = K.
+__global__ void __launch_bounds__(kBlock) k_mg_link(int n, const int* __restrict__ coord, MgDims d, MgDims dp,
+                                                    const int* __restrict__ pscan, int* __restrict__ agg,
+                                                    int* __restrict__ child, int capp) {
+    const int K = blockIdx.x * blockDim.x + threadIdx.x;
+    if (K >= n) return;
+    int cx, cy, cz;
+    mg_decode(coord[K], d, cx, cy, cz);
+    const int pr = pscan[(cx >> 1) + dp.x * ((cy >> 1) + dp.y * (cz >> 1))];
+    agg[K] = pr;
+    child[(size_t) ((cx & 1) + 2 * (cy & 1) + 4 * (cz & 1)) * capp + pr] = K;
+}
+
+template <typename T, int W, int C>
+__global__ void __launch_bounds__(kBlock) k_mg_assemble(int np, const int* __restrict__ child, int capp,
+                                                        const int* __restrict__ col, const T* __restrict__ val,
+                                                        const T* __restrict__ diag, int cap,
+                                                        const int* __restrict__ agg, int* __restrict__ colp,
+                                                        T* __restrict__ valp, T* __restrict__ diagp,
+                                                        T* __restrict__ dinvp, int* __restrict__ err) {
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= np) return;
+    int cs[W];
+    double vs[W];
+    int m = 0;
+    double dg = 0.0;
+    for (int q = 0; q < C; q++) {
+        const int K = child[(size_t) q * capp + P];
+        if (K < 0) continue;
+        dg += (double) diag[K];
+        for (int j = 0; j < W; j++) {
+            const int c = col[(size_t) j * cap + K];
+            if (c < 0) break;
+            const double v = (double) val[(size_t) j * cap + K];
+            const int P2 = agg[c];
+            if (P2 == P) {
+                dg += v;
+                continue;
+            }
+            int i = 0;
+            while (i < m && cs[i] != P2) i++;
+            if (i < m) {
+                vs[i] += v;
+            } else if (m < W) {
+                cs[m] = P2;
+                vs[m] = v;
+                m++;
+            } else {
+                atomicOr(err, 1);
+            }
+        }
+    }
+    for (int j = 0; j < W; j++) {
+        colp[(size_t) j * capp + P] = j < m ? cs[j] : -1;
+        valp[(size_t) j * capp + P] = j < m ? (T) vs[j] : (T) 0;
+    }
+    diagp[P] = (T) dg;
+    dinvp[P] = (T) (1.0 / dg);
+}
+
+// ---------------- the V-cycle ----------------
+
+// Fine level, down: x0 = omega b (unit diagonal); res = b - Ahat x0.
+template <typename T, typename IDX>
+__global__ void __launch_bounds__(kBlock) k_mg_down0(const int* __restrict__ nrow, const int* __restrict__ slice_pos,
+                                                     const IDX* __restrict__ col, const T* __restrict__ val,
+                                                     const T* __restrict__ b, T* __restrict__ res, T omega,
+                                                     const int* __restrict__ done) {
+#pragma clang fp contract(fast)
+    if (mg_done(done)) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= *nrow) return;
+    const int sl = r >> 6, lane = r & 63;
+    const int s0 = slice_pos[sl], w = slice_pos[sl + 1] - s0;
+    const size_t base = (size_t) s0 * 64 + lane;
+    const T br = b[r];
+    T acc = br;
+    for (int k = 0; k < w; k++) {
+        const size_t idx = base + (size_t) k * 64;
+        acc += val[idx] * b[mg_col(col, idx, r)];
+    }
+    res[r] = br - omega * acc;
+}
+
+// Restriction to level 1: sum of the cell's fine residuals.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_mg_restrict1(const int* __restrict__ nrow, const int* __restrict__ cst1,
+                                                         const T* __restrict__ res, T* __restrict__ b1,
+                                                         const int* __restrict__ done) {
+    if (mg_done(done)) return;
+    const int K = blockIdx.x * blockDim.x + threadIdx.x;
+    if (K >= *nrow) return;
+    T s = (T) 0;
+    for (int r = cst1[K]; r < cst1[K + 1]; r++) s += res[r];
+    b1[K] = s;
+}
+
+// Residual of level-l row K at x = omega D^-1 b (x recomputed where read).
+template <typename T, int W>
+__device__ __forceinline__ T mg_child_residual(int K, const int* __restrict__ col, const T* __restrict__ val,
+                                               const T* __restrict__ diag, const T* __restrict__ dinv, int cap,
+                                               const T* __restrict__ b, T omega) {
+#pragma clang fp contract(fast)
+    const T bK = b[K];
+    T acc = diag[K] * (omega * bK * dinv[K]);
+    for (int j = 0; j < W; j++) {
+        const int c = col[(size_t) j * cap + K];
+        if (c < 0) break;
+        acc += val[(size_t) j * cap + K] * (omega * b[c] * dinv[c]);
+    }
+    return bK - acc;
+}
+
+// Restriction of level-l residuals to parent P: one lane per child slot, the
+// C lanes of a parent summed in a fixed tree.  Every lane must call it.
+template <typename T, int W, int C>
+__device__ __forceinline__ void mg_down_item(int t, int np, const int* __restrict__ child, int capp,
+                                             const int* __restrict__ col, const T* __restrict__ val,
+                                             const T* __restrict__ diag, const T* __restrict__ dinv, int cap,
+                                             const T* __restrict__ b, T* __restrict__ bp, T omega) {
+    const int P = t / C, q = t % C;
+    T v = (T) 0;
+    if (P < np) {
+        const int K = child[(size_t) q * capp + P];
+        if (K >= 0) v = mg_child_residual<T, W>(K, col, val, diag, dinv, cap, b, omega);
+    }
+    for (int o = C / 2; o > 0; o >>= 1) v += __shfl_down(v, o, C);
+    if (P < np && q == 0) bp[P] = v;
+}
+
+// Coarse level l, down: b_{l+1}[P] = sum over P's children K of (b - A x)_K.
+template <typename T, int W, int C>
+__global__ void __launch_bounds__(kBlock) k_mg_down(const int* __restrict__ nparent, const int* __restrict__ child,
+                                                    int capp, const int* __restrict__ col, const T* __restrict__ val,
+                                                    const T* __restrict__ diag, const T* __restrict__ dinv, int cap,
+                                                    const T* __restrict__ b, T* __restrict__ bp, T omega,
+                                                    const int* __restrict__ done) {
+    if (mg_done(done)) return;
+    mg_down_item<T, W, C>(blockIdx.x * blockDim.x + threadIdx.x, *nparent, child, capp, col, val, diag, dinv, cap, b,
+                          bp, omega);
+}
+
+// Coarse level l, up at row K: xx = omega D^-1 b + cf x_{l+1}[agg] (recomputed
+// where read), then x = xx + omega D^-1 (b - A xx).
+template <typename T, int W>
+__device__ __forceinline__ void mg_up_row(int K, const int* __restrict__ col, const T* __restrict__ val,
+                                          const T* __restrict__ diag, const T* __restrict__ dinv, int cap,
+                                          const int* __restrict__ agg, const T* __restrict__ b,
+                                          const T* __restrict__ xc, T* __restrict__ x, T omega, T cf) {
+#pragma clang fp contract(fast)
+    const T bK = b[K], dK = dinv[K];
+    const T xx = omega * bK * dK + cf * xc[agg[K]];
+    T acc = diag[K] * xx;
+    for (int j = 0; j < W; j++) {
+        const int c = col[(size_t) j * cap + K];
+        if (c < 0) break;
+        acc += val[(size_t) j * cap + K] * (omega * b[c] * dinv[c] + cf * xc[agg[c]]);
+    }
+    x[K] = xx + omega * dK * (bK - acc);
+}
+
+template <typename T, int W>
+__global__ void __launch_bounds__(kBlock) k_mg_up(const int* __restrict__ nrow, const int* __restrict__ col,
+                                                  const T* __restrict__ val, const T* __restrict__ diag,
+                                                  const T* __restrict__ dinv, int cap, const int* __restrict__ agg,
+                                                  const T* __restrict__ b, const T* __restrict__ xc,
+                                                  T* __restrict__ x, T omega, T cf, const int* __restrict__ done) {
+    if (mg_done(done)) return;
+    const int K = blockIdx.x * blockDim.x + threadIdx.x;
+    if (K >= *nrow) return;
+    mg_up_row<T, W>(K, col, val, diag, dinv, cap, agg, b, xc, x, omega, cf);
+}
+
+// Device table of one level's arrays, for the tail kernel.
+struct MgLevelDesc {
+    int cap;
+    int pad;
+    const int* col;
+    const void* val;
+    const void* diag;
+    const void* dinv;
+    const int* agg;
+    const int* child;
+    void* b;
+    void* x;
+};
+
+constexpr int kMgTailRows = 1024;  // levels this small run inside the tail kernel (measured best at 2M)
+constexpr int kMgTailThreads = 1024;
+
+// Levels l0..L in one workgroup: restrictions down to the coarsest, `sweeps`
+// damped Jacobi sweeps there (in LDS), and the up phases back to level l0.
+// Phases are separated by workgroup barriers; the level data stay in L2.
+template <typename T, int W, int C>
+__global__ void __launch_bounds__(kMgTailThreads) k_mg_tail(const MgLevelDesc* __restrict__ lv,
+                                                            const int* __restrict__ nd, int l0, int L, T omega, T cf,
+                                                            int sweeps, const int* __restrict__ done) {
+#pragma clang fp contract(fast)
+    __shared__ T sx[2][kMgCoarse];
+    __shared__ T sb[kMgCoarse], sd[kMgCoarse], sdi[kMgCoarse];
+    __shared__ T sv[W][kMgCoarse];
+    __shared__ short sc[W][kMgCoarse];
+    if (mg_done(done)) return;
+    const int tid = threadIdx.x;
+    for (int l = l0; l < L; l++) {
+        const MgLevelDesc a = lv[l], p = lv[l + 1];
+        const int np = nd[l + 1];
+        const int items = np * C;
+        for (int base = 0; base < items; base += kMgTailThreads)
+            mg_down_item<T, W, C>(base + tid, np, p.child, p.cap, a.col, (const T*) a.val, (const T*) a.diag,
+                                  (const T*) a.dinv, a.cap, (const T*) a.b, (T*) p.b, omega);
+        __threadfence_block();
+        __syncthreads();
+    }
+    {
+        const MgLevelDesc c = lv[L];
+        const int n = nd[L];
+        if (tid < n) {
+            sb[tid] = ((const T*) c.b)[tid];
+            sd[tid] = ((const T*) c.diag)[tid];
+            sdi[tid] = ((const T*) c.dinv)[tid];
+            sx[0][tid] = (T) 0;
+            for (int j = 0; j < W; j++) {
+                sc[j][tid] = (short) c.col[(size_t) j * c.cap + tid];
+                sv[j][tid] = ((const T*) c.val)[(size_t) j * c.cap + tid];
+            }
+        }
+        __syncthreads();
+        int cur = 0;
+        for (int it = 0; it < sweeps; it++) {
+            if (tid < n) {
+                const T xt = sx[cur][tid];
+                T acc = sd[tid] * xt;
+                for (int j = 0; j < W; j++) {
+                    const int cc = sc[j][tid];
+                    if (cc < 0) break;
+                    acc += sv[j][tid] * sx[cur][cc];
+                }
+                sx[cur ^ 1][tid] = xt + omega * sdi[tid] * (sb[tid] - acc);
+            }
+            __syncthreads();
+            cur ^= 1;
+        }
+        if (tid < n) ((T*) c.x)[tid] = sx[cur][tid];
+        __threadfence_block();
+        __syncthreads();
+    }
+    for (int l = L - 1; l >= l0; l--) {
+        const MgLevelDesc a = lv[l];
+        const T* xc = (const T*) lv[l + 1].x;
+        const int n = nd[l];
+        for (int K = tid; K < n; K += kMgTailThreads)
+            mg_up_row<T, W>(K, a.col, (const T*) a.val, (const T*) a.diag, (const T*) a.dinv, a.cap, a.agg,
+                            (const T*) a.b, xc, (T*) a.x, omega, cf);
+        __threadfence_block();
+        __syncthreads();
+    }
+}
+
+// Fine level, up: xx = omega b + cf x1[agg0] ...
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_mg_prolong0(const int* __restrict__ nrow, const T* __restrict__ b,
+                                                        const int* __restrict__ agg0, const T* __restrict__ x1,
+                                                        T* __restrict__ xx, T omega, T cf,
+                                                        const int* __restrict__ done) {
+    if (mg_done(done)) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= *nrow) return;
+    xx[r] = omega * b[r] + cf * x1[agg0[r]];
+}
+
+// ... then u = xx + omega (b - Ahat xx).
+template <typename T, typename IDX>
+__global__ void __launch_bounds__(kBlock) k_mg_up0(const int* __restrict__ nrow, const int* __restrict__ slice_pos,
+                                                   const IDX* __restrict__ col, const T* __restrict__ val,
+                                                   const T* __restrict__ b, const T* __restrict__ xx,
+                                                   T* __restrict__ u, T omega, const int* __restrict__ done) {
+#pragma clang fp contract(fast)
+    if (mg_done(done)) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= *nrow) return;
+    const int sl = r >> 6, lane = r & 63;
+    const int s0 = slice_pos[sl], w = slice_pos[sl + 1] - s0;
+    const size_t base = (size_t) s0 * 64 + lane;
+    const T xr = xx[r];
+    T acc = xr;
+    for (int k = 0; k < w; k++) {
+        const size_t idx = base + (size_t) k * 64;
+        acc += val[idx] * xx[mg_col(col, idx, r)];
+    }
+    u[r] = xr + omega * (b[r] - acc);
+}
+
+// ---------------- host side ----------------
+
+struct MgWork {
+    int L = 0;                       // coarse levels 1..L (L = coarsest)
+    int W = 9, C = 4;
+    int esize = 8;                   // bytes per value the buffers are sized for
+    int n[kMgMaxLevels + 1] = {};    // rows per level (host copy of the last setup)
+    int cap[kMgMaxLevels + 1] = {};
+    MgDims dims[kMgMaxLevels + 1] = {};
+    int gen = 0;                     // bumped whenever a launch argument changes
+    int* nd = nullptr;               // device row counts, kMgMaxLevels + 1
+    int* err = nullptr;
+    // per level (index l): agg[l]: level l rows -> level l+1 rows
+    int* agg[kMgMaxLevels + 1] = {};
+    int* coord[kMgMaxLevels + 1] = {};
+    int* child[kMgMaxLevels + 1] = {};  // level l >= 2: C x cap children
+    int* col[kMgMaxLevels + 1] = {};
+    void* val[kMgMaxLevels + 1] = {};
+    void* diag[kMgMaxLevels + 1] = {};
+    void* dinv[kMgMaxLevels + 1] = {};
+    void* b[kMgMaxLevels + 1] = {};
+    void* x[kMgMaxLevels + 1] = {};
+    int* cst1 = nullptr;
+    int fcap = 0;                    // fine rows the fine scratch holds
+    int* head = nullptr;
+    int* hscan = nullptr;
+    int* rkey = nullptr;
+    void* res0 = nullptr;
+    void* xx0 = nullptr;
+    int gcap = 0;                    // dense grid scratch
+    int* flag = nullptr;
+    int* pscan = nullptr;
+    int* scan_tmp = nullptr;
+    size_t scan_cap = 0;
+    MgLevelDesc* desc = nullptr;     // device table of the levels
+    MgLevelDesc hdesc[kMgMaxLevels + 1] = {};
+    int tail = 1;                    // first level handled by the tail kernel
+    // measured on the 2M dam break (33 inner iterations per step at 1e-9):
+    // smoother weight 0.9, coarse correction 1.3, 8 coarsest sweeps
+    double omega = 0.9, cf = 1.3;
+    int sweeps = 8;
+    int tail_rows = kMgTailRows;
+};
+
+void mg_set_params(MgWork* m, double omega, double cf, int sweeps) {
+    if (!m) return;
+    if (omega > 0) m->omega = omega;
+    if (cf > 0) m->cf = cf;
+    if (sweeps > 0) m->sweeps = sweeps;
+    m->gen++;
+}
+
+// MPS_AMD_MG="omega,factor,sweeps[,tail_rows]" overrides the smoother
+// weight, the coarse-correction factor, the coarsest sweeps and the size up
+// to which levels run in the tail kernel (tuning aid).
+MgWork* mg_create() {
+    MgWork* m = new MgWork();
+    if (const char* v = getenv("MPS_AMD_MG")) {
+        double om = 0, cf = 0;
+        int sw = 0, tr = 0;
+        if (sscanf(v, "%lf,%lf,%d,%d", &om, &cf, &sw, &tr) >= 1) mg_set_params(m, om, cf, sw);
+        if (tr > 0) m->tail_rows = tr;
+    }
+    return m;
+}
+
+static void mg_free_level(MgWork* m, int l) {
+    void* ps[] = {m->agg[l], m->coord[l], m->child[l], m->col[l], m->val[l], m->diag[l], m->dinv[l], m->b[l], m->x[l]};
+    for (void* p : ps) hipFree(p);
+    m->agg[l] = m->coord[l] = m->child[l] = m->col[l] = nullptr;
+    m->val[l] = m->diag[l] = m->dinv[l] = m->b[l] = m->x[l] = nullptr;
+    m->cap[l] = 0;
+}
+
+void mg_destroy(MgWork* m) {
+    if (!m) return;
+    for (int l = 0; l <= kMgMaxLevels; l++) mg_free_level(m, l);
+    void* ps[] = {m->nd,  m->err,   m->cst1,  m->head,     m->hscan, m->rkey,
+                  m->res0, m->xx0, m->flag, m->pscan, m->scan_tmp, m->desc};
+    for (void* p : ps) hipFree(p);
+    delete m;
+}
+
+int mg_generation(const MgWork* m) { return m ? m->gen : -1; }
+int mg_levels(const MgWork* m) { return m ? m->L : 0; }
+
+template <typename P>
+static hipError_t mg_alloc(P** p, size_t bytes) {
+    return hipMalloc((void**) p, bytes < 8 ? 8 : bytes);
+}
+
+// Level l holds rows for up to `need`; level 0 holds only agg.
+static hipError_t mg_reserve_level(MgWork* m, int l, int need) {
+    if (need <= m->cap[l] && m->agg[l]) return hipSuccess;
+    mg_free_level(m, l);
+    const int cap = need + need / 4 + 64;
+    const size_t W = (size_t) m->W, C = (size_t) m->C, e = 8;
+    hipError_t er;
+    if ((er = mg_alloc(&m->agg[l], sizeof(int) * (size_t) cap))) return er;
+    if (l >= 1) {
+        if ((er = mg_alloc(&m->coord[l], sizeof(int) * (size_t) cap))) return er;
+        if ((er = mg_alloc(&m->child[l], sizeof(int) * C * cap))) return er;
+        if ((er = mg_alloc(&m->col[l], sizeof(int) * W * cap))) return er;
+        if ((er = mg_alloc(&m->val[l], e * W * cap))) return er;
+        if ((er = mg_alloc(&m->diag[l], e * cap))) return er;
+        if ((er = mg_alloc(&m->dinv[l], e * cap))) return er;
+        if ((er = mg_alloc(&m->b[l], e * cap))) return er;
+        if ((er = mg_alloc(&m->x[l], e * cap))) return er;
+    }
+    m->cap[l] = cap;
+    m->gen++;
+    return hipSuccess;
+}
+
+static hipError_t mg_scan(MgWork* m, const int* in, int* out, int len, hipStream_t st) {
+    const size_t need = scan_tmp_ints(len) + 16;
+    if (need > m->scan_cap) {
+        hipFree(m->scan_tmp);
+        m->scan_tmp = nullptr;
+        hipError_t e = mg_alloc(&m->scan_tmp, sizeof(int) * need);
+        if (e) return e;
+        m->scan_cap = need;
+    }
+    return scan_exclusive(in, out, len, m->scan_tmp, st);
+}
+
+static hipError_t mg_read_int(const int* dev, int* host, hipStream_t st) {
+    hipError_t e = hipMemcpyAsync(host, dev, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e) return e;
+    return hipStreamSynchronize(st);
+}
+
+template <typename T, typename IDX>
+static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, const GridView& g, int dim, const T* fval,
+                             const IDX* fcol, hipStream_t st) {
+    hipError_t e;
+    const int R = A.rows;
+    m->W = dim == 2 ? 9 : 27;
+    m->C = dim == 2 ? 4 : 8;
+    if (!m->nd) {
+        if ((e = mg_alloc(&m->nd, sizeof(int) * (kMgMaxLevels + 1)))) return e;
+        if ((e = mg_alloc(&m->err, sizeof(int)))) return e;
+        if ((e = mg_alloc(&m->desc, sizeof(MgLevelDesc) * (kMgMaxLevels + 1)))) return e;
+    }
+    if (R > m->fcap) {
+        void* ps[] = {m->head, m->hscan, m->rkey, m->res0, m->xx0, m->cst1};
+        for (void* p : ps) hipFree(p);
+        const int cap = R + R / 4 + 1024;
+        if ((e = mg_alloc(&m->head, sizeof(int) * (size_t) cap)) || (e = mg_alloc(&m->hscan, sizeof(int) * (cap + 1))) ||
+            (e = mg_alloc(&m->rkey, sizeof(int) * (size_t) cap)) || (e = mg_alloc(&m->res0, 8 * (size_t) cap)) ||
+            (e = mg_alloc(&m->xx0, 8 * (size_t) cap)) || (e = mg_alloc(&m->cst1, sizeof(int) * (cap + 1))))
+            return e;
+        m->fcap = cap;
+        m->gen++;
+    }
+    if ((e = mg_reserve_level(m, 0, R))) return e;
+    if ((e = fill_int(m->err, 0, 1, st))) return e;
+    // level 1: cells
+    hipLaunchKernelGGL(k_mg_heads, mg_grid(R), dim3(kBlock), 0, st, R, A.row_slot, h.key, m->head, m->rkey);
+    if ((e = mg_scan(m, m->head, m->hscan, R, st))) return e;
+    int n1 = 0;
+    if ((e = mg_read_int(m->hscan + R, &n1, st))) return e;
+    if ((e = mg_reserve_level(m, 1, n1))) return e;
+    hipLaunchKernelGGL(k_mg_level1, mg_grid(R), dim3(kBlock), 0, st, R, m->head, m->hscan, m->rkey, g.key0, m->agg[0],
+                       m->cst1, m->coord[1]);
+    if (dim == 2)
+        hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 9>), mg_grid(n1), dim3(kBlock), 0, st, n1, m->cst1, A.slice_off,
+                           fcol, fval, m->agg[0], m->cap[1], m->col[1], (T*) m->val[1], (T*) m->diag[1],
+                           (T*) m->dinv[1], m->err);
+    else
+        hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 27>), mg_grid(n1), dim3(kBlock), 0, st, n1, m->cst1, A.slice_off,
+                           fcol, fval, m->agg[0], m->cap[1], m->col[1], (T*) m->val[1], (T*) m->diag[1],
+                           (T*) m->dinv[1], m->err);
+    m->n[0] = R;
+    m->n[1] = n1;
+    m->dims[1] = MgDims{g.nx, g.ny, g.flat ? 1 : g.nz};
+    int l = 1;
+    while (m->n[l] > kMgCoarse && l < kMgMaxLevels) {
+        const MgDims d = m->dims[l];
+        const MgDims dp{(d.x + 1) / 2, (d.y + 1) / 2, (d.z + 1) / 2};
+        const int G = dp.x * dp.y * dp.z;
+        if (G + 1 > m->gcap) {
+            hipFree(m->flag);
+            hipFree(m->pscan);
+            m->flag = m->pscan = nullptr;
+            const int cap = G + 1024;
+            if ((e = mg_alloc(&m->flag, sizeof(int) * (size_t) cap)) ||
+                (e = mg_alloc(&m->pscan, sizeof(int) * (size_t) (cap + 1))))
+                return e;
+            m->gcap = cap;
+        }
+        if ((e = fill_int(m->flag, 0, (size_t) G, st))) return e;
+        hipLaunchKernelGGL(k_mg_mark, mg_grid(m->n[l]), dim3(kBlock), 0, st, m->n[l], m->coord[l], d, dp, m->flag);
+        if ((e = mg_scan(m, m->flag, m->pscan, G, st))) return e;
+        int np = 0;
+        if ((e = mg_read_int(m->pscan + G, &np, st))) return e;
+        if ((e = mg_reserve_level(m, l + 1, np))) return e;
+        hipLaunchKernelGGL(k_mg_parents, mg_grid(G), dim3(kBlock), 0, st, G, m->flag, m->pscan, m->coord[l + 1]);
+        if ((e = fill_int(m->child[l + 1], -1, (size_t) m->C * m->cap[l + 1], st))) return e;
+        hipLaunchKernelGGL(k_mg_link, mg_grid(m->n[l]), dim3(kBlock), 0, st, m->n[l], m->coord[l], d, dp, m->pscan,
+                           m->agg[l], m->child[l + 1], m->cap[l + 1]);
+        if (dim == 2)
+            hipLaunchKernelGGL((k_mg_assemble<T, 9, 4>), mg_grid(np), dim3(kBlock), 0, st, np, m->child[l + 1],
+                               m->cap[l + 1], m->col[l], (const T*) m->val[l], (const T*) m->diag[l], m->cap[l],
+                               m->agg[l], m->col[l + 1], (T*) m->val[l + 1], (T*) m->diag[l + 1],
+                               (T*) m->dinv[l + 1], m->err);
+        else
+            hipLaunchKernelGGL((k_mg_assemble<T, 27, 8>), mg_grid(np), dim3(kBlock), 0, st, np, m->child[l + 1],
+                               m->cap[l + 1], m->col[l], (const T*) m->val[l], (const T*) m->diag[l], m->cap[l],
+                               m->agg[l], m->col[l + 1], (T*) m->val[l + 1], (T*) m->diag[l + 1],
+                               (T*) m->dinv[l + 1], m->err);
+        m->n[l + 1] = np;
+        m->dims[l + 1] = dp;
+        l++;
+    }
+    if (m->L != l) m->gen++;
+    m->L = l;
+    if (m->n[l] > kMgCoarse) return hipErrorInvalidValue;  // the level cap was hit before the grid got small
+    int tail = l;
+    while (tail > 1 && m->n[tail - 1] <= m->tail_rows) tail--;
+    if (tail != m->tail) m->gen++;
+    m->tail = tail;
+    for (int k = 1; k <= l; k++)
+        m->hdesc[k] = MgLevelDesc{m->cap[k], 0,         m->col[k], m->val[k],   m->diag[k],
+                                  m->dinv[k], m->agg[k], m->child[k], m->b[k], m->x[k]};
+    if ((e = hipMemcpyAsync(m->desc, m->hdesc, sizeof(MgLevelDesc) * (kMgMaxLevels + 1), hipMemcpyHostToDevice, st)))
+        return e;
+    if ((e = hipMemcpyAsync(m->nd, m->n, sizeof(int) * (kMgMaxLevels + 1), hipMemcpyHostToDevice, st))) return e;
+    int bad = 0;
+    if ((e = mg_read_int(m->err, &bad, st))) return e;
+    if (bad) return hipErrorInvalidValue;  // a coarse row with more neighbours than the stencil allows
+    return hipGetLastError();
+}
+
+hipError_t mg_setup(MgWork* m, const Sell& A, const Hot& h, const GridView& g, int dim, int fp32, hipStream_t st) {
+    if (fp32)
+        return A.use16 ? mg_setup_t<float, short>(m, A, h, g, dim, A.val32, A.dcol, st)
+                       : mg_setup_t<float, int>(m, A, h, g, dim, A.val32, A.col, st);
+    return A.use16 ? mg_setup_t<double, short>(m, A, h, g, dim, A.val64, A.dcol, st)
+                   : mg_setup_t<double, int>(m, A, h, g, dim, A.val64, A.col, st);
+}
+
+template <typename T, typename IDX, int W, int C>
+static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval, const T* r, T* u, const int* done,
+                       hipStream_t st) {
+    const T om = (T) m->omega, cf = (T) m->cf;
+    const int L = m->L, lt = m->tail;
+    int* nd = m->nd;
+    T* res0 = (T*) m->res0;
+    T* xx0 = (T*) m->xx0;
+    hipLaunchKernelGGL((k_mg_down0<T, IDX>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, A.slice_off, fcol, fval,
+                       r, res0, om, done);
+    hipLaunchKernelGGL((k_mg_restrict1<T>), mg_grid(m->cap[1]), dim3(kBlock), 0, st, nd + 1, m->cst1, res0,
+                       (T*) m->b[1], done);
+    for (int l = 1; l < lt; l++)
+        hipLaunchKernelGGL((k_mg_down<T, W, C>), mg_grid(C * m->cap[l + 1]), dim3(kBlock), 0, st, nd + l + 1,
+                           m->child[l + 1], m->cap[l + 1], m->col[l], (const T*) m->val[l], (const T*) m->diag[l],
+                           (const T*) m->dinv[l], m->cap[l], (const T*) m->b[l], (T*) m->b[l + 1], om, done);
+    hipLaunchKernelGGL((k_mg_tail<T, W, C>), dim3(1), dim3(kMgTailThreads), 0, st, m->desc, nd, lt, L, om, cf,
+                       m->sweeps, done);
+    for (int l = lt - 1; l >= 1; l--)
+        hipLaunchKernelGGL((k_mg_up<T, W>), mg_grid(m->cap[l]), dim3(kBlock), 0, st, nd + l, m->col[l],
+                           (const T*) m->val[l], (const T*) m->diag[l], (const T*) m->dinv[l], m->cap[l], m->agg[l],
+                           (const T*) m->b[l], (const T*) m->x[l + 1], (T*) m->x[l], om, cf, done);
+    hipLaunchKernelGGL((k_mg_prolong0<T>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, r, m->agg[0],
+                       (const T*) m->x[1], xx0, om, cf, done);
+    hipLaunchKernelGGL((k_mg_up0<T, IDX>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, A.slice_off, fcol, fval, r,
+                       xx0, u, om, done);
+}
+
+template <typename T, typename IDX>
+void mg_cycle(MgWork* m, const Sell& A, const IDX* fcol, const T* fval, const T* r, T* u, const int* done, int dim,
+              hipStream_t st) {
+    if (dim == 2)
+        mg_cycle_t<T, IDX, 9, 4>(m, A, fcol, fval, r, u, done, st);
+    else
+        mg_cycle_t<T, IDX, 27, 8>(m, A, fcol, fval, r, u, done, st);
+}
+
+template void mg_cycle<float, short>(MgWork*, const Sell&, const short*, const float*, const float*, float*,
+                                     const int*, int, hipStream_t);
+template void mg_cycle<float, int>(MgWork*, const Sell&, const int*, const float*, const float*, float*, const int*,
+                                   int, hipStream_t);
+template void mg_cycle<double, short>(MgWork*, const Sell&, const short*, const double*, const double*, double*,
+                                      const int*, int, hipStream_t);
+template void mg_cycle<double, int>(MgWork*, const Sell&, const int*, const double*, const double*, double*,
+                                    const int*, int, hipStream_t);
+
+}  // namespace mps

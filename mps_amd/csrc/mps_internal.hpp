@@ -280,6 +280,8 @@ hipError_t launch_snapshot(int dim, int n, const Hot& h, const Cold& cold, doubl
 
 // ppe.hip — matrix storage and CG
 struct Sell {
+    int dim;             // 2 or 3
+    GridView grid;       // bucket grid of a single-scene handle (multigrid aggregation)
     int rows;            // Inner rows
     const int* rows_dev; // Inner rows, on the device
     int nseg;            // independent segments (scenes)
@@ -309,6 +311,21 @@ struct Sell {
 hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
                          const StepScratch& w, Sell& A, int want_val32, int want_raw, int* err,
                          int* host_rows, size_t* host_entries, hipStream_t st, int bs);
+
+// mg.hip — aggregation multigrid preconditioner (single segment)
+struct MgWork;
+MgWork* mg_create();
+void mg_destroy(MgWork* m);
+int mg_generation(const MgWork* m);
+int mg_levels(const MgWork* m);
+void mg_set_params(MgWork* m, double omega, double cf, int sweeps);
+// Builds the hierarchy for the current system; hipErrorInvalidValue when it
+// cannot (the caller then solves without it).
+hipError_t mg_setup(MgWork* m, const Sell& A, const Hot& h, const GridView& g, int dim, int fp32, hipStream_t st);
+// Enqueues u = M r (one V-cycle); kernels return at once while *done is set.
+template <typename T, typename IDX>
+void mg_cycle(MgWork* m, const Sell& A, const IDX* fcol, const T* fval, const T* r, T* u, const int* done, int dim,
+              hipStream_t st);
 
 struct CgWork;  // defined in ppe.hip
 CgWork* cg_create();

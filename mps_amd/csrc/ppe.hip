@@ -251,6 +251,10 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
 // apply gathers only r, once per stored entry; step is elementwise.
 
 constexpr int kMaxBlocks = 1024;  // per segment
+// Below this many rows the V-cycle's setup and launches cost more than the
+// iterations it saves (measured: 584-particle dam break), so small systems
+// keep the Jacobi scaling alone.
+constexpr int kMgMinRows = 8192;
 
 static int cg_blocks(int rows) {
     int nb = (rows + kBlock - 1) / kBlock;
@@ -395,27 +399,30 @@ __device__ __forceinline__ T sell_row(const int* __restrict__ slice_pos, const I
     return acc;
 }
 
-// apply: w = Ahat r (unit diagonal); partials r.r, w.r, stopping norm.
-template <typename T, typename IDX, bool WEIGHTED>
+// apply: w = Ahat u with u = M r (u = r without a preconditioner; unit
+// diagonal); partials r.u, w.u and the stopping norm of r.
+template <typename T, typename IDX, bool WEIGHTED, bool PRE>
 __global__ void __launch_bounds__(kBlock) k_cg_apply(Seg sg, const int* __restrict__ slice_pos,
                                                      const IDX* __restrict__ col, const T* __restrict__ val,
-                                                     const T* __restrict__ r, T* __restrict__ w,
-                                                     const double* __restrict__ dgl, double* __restrict__ part,
-                                                     CgScalars* scs) {
+                                                     const T* __restrict__ r, const T* __restrict__ u,
+                                                     T* __restrict__ w, const double* __restrict__ dgl,
+                                                     double* __restrict__ part, CgScalars* scs) {
 #pragma clang fp contract(fast)
     __shared__ double red[3 * kBlock / 64];
     const BlockRows br = block_rows(sg);
     CgScalars* sc = scs + br.seg;
     if (block_uniform_done(sc)) return;
     if (br.v == 0 && threadIdx.x == 0 && !sc->frozen) sc->iter += 1;
+    const T* x = PRE ? u : r;
     double g = 0.0, d = 0.0, rw = 0.0;
     for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) {
         const T rr = r[row];
-        const T acc = sell_row<T, IDX>(slice_pos, col, val, r, row, rr);
+        const T uu = PRE ? x[row] : rr;
+        const T acc = sell_row<T, IDX>(slice_pos, col, val, x, row, uu);
         w[row] = acc;
         const double r2 = (double) rr * (double) rr;
-        g += r2;
-        d += (double) acc * (double) rr;
+        g += (double) rr * (double) uu;
+        d += (double) acc * (double) uu;
         rw += WEIGHTED ? dgl[row] * r2 : r2;
     }
     double v[3] = {g, d, rw};
@@ -423,10 +430,11 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Seg sg, const int* __restri
 }
 
 // step: convergence test on r, then the vector recurrences.
-template <typename T, bool WEIGHTED>
+template <typename T, bool WEIGHTED, bool PRE>
 __global__ void __launch_bounds__(kBlock) k_cg_step(Seg sg, T* __restrict__ y, T* __restrict__ r, T* __restrict__ p,
                                                     T* __restrict__ s, const T* __restrict__ w,
-                                                    const double* __restrict__ part, CgScalars* scs, int par) {
+                                                    const T* __restrict__ u, const double* __restrict__ part,
+                                                    CgScalars* scs, int par) {
 #pragma clang fp contract(fast)
     __shared__ double red[3 * kBlock / 64];
     const BlockRows br = block_rows(sg);
@@ -465,7 +473,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_step(Seg sg, T* __restrict__ y, T
     }
     for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) {
         const T rr = r[row], ww = w[row];
-        const T pp = first ? rr : rr + beta * p[row];
+        const T uu = PRE ? u[row] : rr;
+        const T pp = first ? uu : uu + beta * p[row];
         const T ss = first ? ww : ww + beta * s[row];
         p[row] = pp;
         s[row] = ss;
@@ -589,18 +598,24 @@ struct CgWork {
     int* row_off = nullptr;   // device, nseg + 1
     int* h_blk = nullptr;     // pinned host mirrors
     int* h_row = nullptr;
+    double* u64 = nullptr;    // preconditioned residual M r
+    float* u32 = nullptr;
+    MgWork* mg = nullptr;     // multigrid preconditioner (single segment)
+    int mg_on = 0;            // the current solve uses it
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // chunk graphs: FP64 CG, FP32 inner CG
     int graph_chunk[2] = {0, 0};
     const void* graph_key[2] = {nullptr, nullptr};
     int graph_idx16[2] = {-1, -1};
     int graph_blocks[2] = {0, 0};
+    int graph_mg[2] = {-2, -2};  // multigrid generation captured (-1: none)
     Seg seg() const { return Seg{nseg, blk_off, row_off}; }
 };
 
 CgWork* cg_create() { return new CgWork(); }
 
 static void cg_free_vectors(CgWork* cw) {
-    void* ptrs[] = {cw->y, cw->bh, cw->r64, cw->p64, cw->s64, cw->w64, cw->r32, cw->p32, cw->s32, cw->w32, cw->e32};
+    void* ptrs[] = {cw->y,   cw->bh,  cw->r64, cw->p64, cw->s64, cw->w64, cw->r32,
+                    cw->p32, cw->s32, cw->w32, cw->e32, cw->u64, cw->u32};
     for (void* p : ptrs) hipFree(p);
 }
 
@@ -616,6 +631,7 @@ void cg_destroy(CgWork* cw) {
     hipHostFree(cw->h_row);
     for (int k = 0; k < 2; k++)
         if (cw->graph[k]) hipGraphExecDestroy(cw->graph[k]);
+    mg_destroy(cw->mg);
     delete cw;
 }
 
@@ -647,7 +663,8 @@ hipError_t cg_reserve(CgWork* cw, int rows) {
     if ((e = hipMalloc(&cw->y, d)) || (e = hipMalloc(&cw->bh, d)) || (e = hipMalloc(&cw->r64, d)) ||
         (e = hipMalloc(&cw->p64, d)) || (e = hipMalloc(&cw->s64, d)) || (e = hipMalloc(&cw->w64, d)) ||
         (e = hipMalloc(&cw->r32, f)) || (e = hipMalloc(&cw->p32, f)) || (e = hipMalloc(&cw->s32, f)) ||
-        (e = hipMalloc(&cw->w32, f)) || (e = hipMalloc(&cw->e32, f)))
+        (e = hipMalloc(&cw->w32, f)) || (e = hipMalloc(&cw->e32, f)) || (e = hipMalloc(&cw->u64, d)) ||
+        (e = hipMalloc(&cw->u32, f)))
         return e;
     cw->cap = cap;
     for (int k = 0; k < 2; k++) {
@@ -694,15 +711,29 @@ static hipError_t cg_layout(CgWork* cw, const Sell& A, hipStream_t st) {
 
 // Enqueue `chunk` CG iterations (two kernels each); chunk is even, so the
 // scalar parities line up from one chunk to the next.
+template <typename T> static T* u_vec(CgWork* cw);
+template <> double* u_vec<double>(CgWork* cw) { return cw->u64; }
+template <> float* u_vec<float>(CgWork* cw) { return cw->u32; }
+
 template <typename T, typename IDX, bool WEIGHTED>
 static void enqueue_chunk(CgWork* cw, const Sell& A, const IDX* col, const T* val, T* y, T* r, T* p, T* s, T* w,
                           int chunk, hipStream_t st) {
     const dim3 gr(cw->total_blocks), bl(kBlock);
     const Seg sg = cw->seg();
+    T* u = u_vec<T>(cw);
     for (int k = 0; k < chunk; k++) {
-        hipLaunchKernelGGL((k_cg_apply<T, IDX, WEIGHTED>), gr, bl, 0, st, sg, A.slice_off, col, val, r, w, A.dgl,
-                           cw->part, cw->sc);
-        hipLaunchKernelGGL((k_cg_step<T, WEIGHTED>), gr, bl, 0, st, sg, y, r, p, s, w, cw->part, cw->sc, k & 1);
+        if (cw->mg_on) {
+            mg_cycle<T, IDX>(cw->mg, A, col, val, r, u, &cw->sc[0].done, A.dim, st);
+            hipLaunchKernelGGL((k_cg_apply<T, IDX, WEIGHTED, true>), gr, bl, 0, st, sg, A.slice_off, col, val, r, u, w,
+                               A.dgl, cw->part, cw->sc);
+            hipLaunchKernelGGL((k_cg_step<T, WEIGHTED, true>), gr, bl, 0, st, sg, y, r, p, s, w, u, cw->part, cw->sc,
+                               k & 1);
+        } else {
+            hipLaunchKernelGGL((k_cg_apply<T, IDX, WEIGHTED, false>), gr, bl, 0, st, sg, A.slice_off, col, val, r, r,
+                               w, A.dgl, cw->part, cw->sc);
+            hipLaunchKernelGGL((k_cg_step<T, WEIGHTED, false>), gr, bl, 0, st, sg, y, r, p, s, w, r, cw->part, cw->sc,
+                               k & 1);
+        }
     }
 }
 
@@ -721,8 +752,10 @@ static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* c
     hipError_t e;
     for (;;) {
         if (o.use_graph) {
+            const int mgk = cw->mg_on ? mg_generation(cw->mg) : -1;
             if (!cw->graph[which] || cw->graph_chunk[which] != chunk || cw->graph_key[which] != (const void*) val ||
-                cw->graph_idx16[which] != idx16 || cw->graph_blocks[which] != cw->total_blocks) {
+                cw->graph_idx16[which] != idx16 || cw->graph_blocks[which] != cw->total_blocks ||
+                cw->graph_mg[which] != mgk) {
                 if (cw->graph[which]) hipGraphExecDestroy(cw->graph[which]);
                 cw->graph[which] = nullptr;
                 hipGraph_t gph;
@@ -736,6 +769,7 @@ static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* c
                 cw->graph_key[which] = val;
                 cw->graph_idx16[which] = idx16;
                 cw->graph_blocks[which] = cw->total_blocks;
+                cw->graph_mg[which] = mgk;
             }
             if ((e = hipGraphLaunch(cw->graph[which], st))) return e;
         } else {
@@ -798,6 +832,18 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
     if ((e = cg_layout(cw, A, st))) return e;
     const dim3 gr(cw->total_blocks), bl(kBlock);
     const Seg sg = cw->seg();
+    // multigrid preconditioner on single-scene systems, built in the precision
+    // of the CG it serves; if the hierarchy cannot be built the solve runs
+    // with the Jacobi scaling alone
+    cw->mg_on = 0;
+    if (o.preconditioner == MPS_PRECOND_MULTIGRID && nseg == 1 && rows >= kMgMinRows) {
+        if (!cw->mg) cw->mg = mg_create();
+        const hipError_t me = mg_setup(cw->mg, A, h, A.grid, A.dim, o.precision == MPS_PREC_MIXED, st);
+        if (me == hipSuccess)
+            cw->mg_on = 1;
+        else if (me != hipErrorInvalidValue)
+            return me;
+    }
 
     hipLaunchKernelGGL(k_cg_init, gr, bl, 0, st, sg, A.row_slot, h.id, b_s, o.warm_start ? pprev_id : nullptr,
                        A.sdiag, cw->y, cw->bh, cw->part);
@@ -920,20 +966,20 @@ static void time_launch(CgWork* cw, const Sell& A, const IDX* col, int kind, hip
     const Seg sg = cw->seg();
     switch (kind) {
         case 0:
-            hipLaunchKernelGGL((k_cg_apply<double, IDX, true>), gr, bl, 0, st, sg, A.slice_off, col, A.val64, cw->r64,
-                               cw->w64, A.dgl, cw->part, cw->sc);
+            hipLaunchKernelGGL((k_cg_apply<double, IDX, true, false>), gr, bl, 0, st, sg, A.slice_off, col, A.val64,
+                               cw->r64, cw->r64, cw->w64, A.dgl, cw->part, cw->sc);
             break;
         case 1:
-            hipLaunchKernelGGL((k_cg_step<double, true>), gr, bl, 0, st, sg, cw->y, cw->r64, cw->p64, cw->s64,
-                               cw->w64, cw->part, cw->sc, 0);
+            hipLaunchKernelGGL((k_cg_step<double, true, false>), gr, bl, 0, st, sg, cw->y, cw->r64, cw->p64, cw->s64,
+                               cw->w64, cw->r64, cw->part, cw->sc, 0);
             break;
         case 2:
-            hipLaunchKernelGGL((k_cg_apply<float, IDX, false>), gr, bl, 0, st, sg, A.slice_off, col, A.val32, cw->r32,
-                               cw->w32, A.dgl, cw->part, cw->sc);
+            hipLaunchKernelGGL((k_cg_apply<float, IDX, false, false>), gr, bl, 0, st, sg, A.slice_off, col, A.val32,
+                               cw->r32, cw->r32, cw->w32, A.dgl, cw->part, cw->sc);
             break;
         default:
-            hipLaunchKernelGGL((k_cg_step<float, false>), gr, bl, 0, st, sg, cw->e32, cw->r32, cw->p32, cw->s32,
-                               cw->w32, cw->part, cw->sc, 0);
+            hipLaunchKernelGGL((k_cg_step<float, false, false>), gr, bl, 0, st, sg, cw->e32, cw->r32, cw->p32, cw->s32,
+                               cw->w32, cw->r32, cw->part, cw->sc, 0);
             break;
     }
 }

@@ -64,6 +64,10 @@ int main(int argc, char** argv) {
         o.warm_start = std::atoi(w) ? 1 : 0;
         custom = true;
     }
+    if (const char* m = std::getenv("MPS_AMD_PRECONDITIONER")) {
+        o.preconditioner = std::strcmp(m, "multigrid") == 0 ? MPS_PRECOND_MULTIGRID : MPS_PRECOND_JACOBI;
+        custom = true;
+    }
     if (custom) simulation.setSolver(o);
     try {
         simulation.run(fs::path(setting));

@@ -1,0 +1,76 @@
+"""Multigrid-preconditioned pressure CG (MPS_PRECOND_MULTIGRID).
+
+The preconditioner changes how many iterations the CG takes, not what it
+solves: at the same tolerance the solution must match the Jacobi path's to
+the accuracy the tolerance fixes, the true FP64 residual must meet it, and
+the path must be deterministic (graph replay and stream launches identical,
+a rerun bit-identical)."""
+import numpy as np
+import pytest
+
+from mps_amd import capi, configs
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(name, precision, precond, steps, tol=1e-9, use_graph=True, check_interval=8):
+    s, sc, _ = configs.workload(name)
+    sim = capi.Simulation(s, sc)
+    sim.set_solver(precision=precision, tolerance=tol, warm_start=True, preconditioner=precond,
+                   use_graph=use_graph, check_interval=check_interval)
+    sim.load_initial_state()
+    stats = []
+    for _ in range(steps):
+        sim.step()
+        stats.append(sim.solver_stats())
+    st = sim.state()
+    sim.close()
+    return st, stats
+
+
+@pytest.mark.parametrize("precision", [capi.MPS_PREC_FP64, capi.MPS_PREC_MIXED])
+def test_multigrid_solves_the_same_system(precision):
+    # 35k particles: 32k Inner rows, a three-level hierarchy
+    jac, sj = _run("dambreak2d_35k", precision, capi.MPS_PRECOND_JACOBI, 5)
+    mg, sm = _run("dambreak2d_35k", precision, capi.MPS_PRECOND_MULTIGRID, 5)
+    for a, b in zip(sj, sm):
+        assert b["converged"] == 1 and b["rel_residual"] <= 1e-9
+        assert b["iterations"] * 5 < a["iterations"]  # the point of it
+    # after five steps the pressures differ only by what the 1e-9 residual
+    # leaves free (cond ~1e4) and its effect on the trajectory
+    p_j, p_m = jac["p"], mg["p"]
+    assert np.abs(p_j - p_m).max() <= 1e-5 * np.abs(p_j).max()
+
+
+def test_multigrid_is_deterministic():
+    a, _ = _run("dambreak2d_35k", capi.MPS_PREC_MIXED, capi.MPS_PRECOND_MULTIGRID, 4)
+    b, _ = _run("dambreak2d_35k", capi.MPS_PREC_MIXED, capi.MPS_PRECOND_MULTIGRID, 4)
+    c, _ = _run("dambreak2d_35k", capi.MPS_PREC_MIXED, capi.MPS_PRECOND_MULTIGRID, 4, use_graph=False,
+                check_interval=3)
+    for k in ("p", "pos", "vel"):
+        assert (a[k] == b[k]).all() and (a[k] == c[k]).all(), k
+
+
+def test_multigrid_3d():
+    s = configs.dambreak_settings(0.01, 1e-4, 1.0, dim=3)
+    s.domain_x, s.domain_y, s.domain_z = (-0.1, 0.5), (-0.1, 0.5), (-0.1, 0.3)
+    sc = configs.lattice_3d(0.01, (0.4, 0.4, 0.2), (0.0, 0.15), (0.0, 0.3), (0.0, 0.2), 2, 2)
+    out = {}
+    for pre in (capi.MPS_PRECOND_JACOBI, capi.MPS_PRECOND_MULTIGRID):
+        sim = capi.Simulation(s, sc)
+        sim.set_solver(precision=capi.MPS_PREC_FP64, tolerance=1e-9, warm_start=True, preconditioner=pre)
+        sim.load_initial_state()
+        sim.step()
+        out[pre] = (sim.state()["p"], sim.solver_stats())
+        sim.close()
+    (pj, sj), (pm, sm) = out[0], out[1]
+    assert sm["rows"] >= 8192 and sm["rel_residual"] <= 1e-9
+    assert sm["iterations"] * 3 < sj["iterations"]
+    assert np.abs(pj - pm).max() <= 1e-5 * np.abs(pj).max()
+
+
+def test_small_systems_keep_jacobi():
+    # below the size threshold the multigrid option changes nothing
+    a, _ = _run("dambreak2d_584", capi.MPS_PREC_FP64, capi.MPS_PRECOND_JACOBI, 3)
+    b, _ = _run("dambreak2d_584", capi.MPS_PREC_FP64, capi.MPS_PRECOND_MULTIGRID, 3)
+    assert (a["p"] == b["p"]).all()

@@ -346,76 +346,180 @@ struct MgLevelDesc {
     void* x;
 };
 
-constexpr int kMgTailRows = 1024;  // levels this small run inside the tail kernel (measured best at 2M)
+constexpr int kMgTailRows = 1024;       // levels this small may run inside the tail kernel
 constexpr int kMgTailThreads = 1024;
+constexpr int kMgTailLds = 156 * 1024;  // dynamic LDS of the tail kernel
 
-// Levels l0..L in one workgroup: restrictions down to the coarsest, `sweeps`
-// damped Jacobi sweeps there (in LDS), and the up phases back to level l0.
-// Phases are separated by workgroup barriers; the level data stay in L2.
+// LDS bytes one row of a tail level takes: ELL columns (16-bit) and values,
+// diagonal, inverse diagonal, b, x, a second x (coarsest sweeps), the parent
+// index and the children indices (16-bit).
+__host__ __device__ constexpr int mg_tail_row_bytes(int W, int C, int tsize) {
+    return W * (2 + tsize) + 5 * tsize + 2 + 2 * C;
+}
+
+// One tail level's arrays in LDS for n rows (16-byte aligned pieces).
+template <typename T, int W, int C>
+struct MgLdsLevel {
+    short* col;  // W x n
+    T* val;      // W x n
+    T* diag;
+    T* dinv;
+    T* b;
+    T* x;
+    T* x2;
+    short* agg;
+    short* child;  // C x n
+    __device__ static int align16(int v) { return (v + 15) & ~15; }
+    __device__ static int bytes(int n) {
+        return align16(W * n * 2) + align16(W * n * (int) sizeof(T)) + 5 * align16(n * (int) sizeof(T)) +
+               align16(n * 2) + align16(C * n * 2);
+    }
+    __device__ void carve(unsigned char* base, int n) {
+        int o = 0;
+        col = (short*) (base + o);
+        o += align16(W * n * 2);
+        val = (T*) (base + o);
+        o += align16(W * n * (int) sizeof(T));
+        diag = (T*) (base + o);
+        o += align16(n * (int) sizeof(T));
+        dinv = (T*) (base + o);
+        o += align16(n * (int) sizeof(T));
+        b = (T*) (base + o);
+        o += align16(n * (int) sizeof(T));
+        x = (T*) (base + o);
+        o += align16(n * (int) sizeof(T));
+        x2 = (T*) (base + o);
+        o += align16(n * (int) sizeof(T));
+        agg = (short*) (base + o);
+        o += align16(n * 2);
+        child = (short*) (base + o);
+    }
+};
+
+// Levels l0..L in one workgroup, out of LDS: every level's operator, transfer
+// maps and b[l0] are loaded once (one round of independent loads), then the
+// restrictions down to the coarsest, `sweeps` damped Jacobi sweeps there and
+// the prolongation + smoothing phases back to level l0 run on LDS, separated
+// by workgroup barriers; x[l0] goes back to global memory for the next kernel.
 template <typename T, int W, int C>
 __global__ void __launch_bounds__(kMgTailThreads) k_mg_tail(const MgLevelDesc* __restrict__ lv,
                                                             const int* __restrict__ nd, int l0, int L, T omega, T cf,
                                                             int sweeps, const int* __restrict__ done) {
 #pragma clang fp contract(fast)
-    __shared__ T sx[2][kMgCoarse];
-    __shared__ T sb[kMgCoarse], sd[kMgCoarse], sdi[kMgCoarse];
-    __shared__ T sv[W][kMgCoarse];
-    __shared__ short sc[W][kMgCoarse];
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int s_off[kMgMaxLevels + 2];
+    __shared__ int s_n[kMgMaxLevels + 1];
     if (mg_done(done)) return;
     const int tid = threadIdx.x;
-    for (int l = l0; l < L; l++) {
-        const MgLevelDesc a = lv[l], p = lv[l + 1];
-        const int np = nd[l + 1];
-        const int items = np * C;
-        for (int base = 0; base < items; base += kMgTailThreads)
-            mg_down_item<T, W, C>(base + tid, np, p.child, p.cap, a.col, (const T*) a.val, (const T*) a.diag,
-                                  (const T*) a.dinv, a.cap, (const T*) a.b, (T*) p.b, omega);
-        __threadfence_block();
-        __syncthreads();
+    if (tid == 0) {
+        int o = 0;
+        for (int l = l0; l <= L; l++) {
+            s_n[l] = nd[l];
+            s_off[l] = o;
+            o += MgLdsLevel<T, W, C>::bytes(s_n[l]);
+        }
     }
-    {
-        const MgLevelDesc c = lv[L];
-        const int n = nd[L];
-        if (tid < n) {
-            sb[tid] = ((const T*) c.b)[tid];
-            sd[tid] = ((const T*) c.diag)[tid];
-            sdi[tid] = ((const T*) c.dinv)[tid];
-            sx[0][tid] = (T) 0;
+    __syncthreads();
+    auto level = [&](int l) {
+        MgLdsLevel<T, W, C> q;
+        q.carve(smem + s_off[l], s_n[l]);
+        return q;
+    };
+    // load
+    for (int l = l0; l <= L; l++) {
+        const MgLevelDesc d = lv[l];
+        const int n = s_n[l];
+        const MgLdsLevel<T, W, C> q = level(l);
+        for (int K = tid; K < n; K += kMgTailThreads) {
             for (int j = 0; j < W; j++) {
-                sc[j][tid] = (short) c.col[(size_t) j * c.cap + tid];
-                sv[j][tid] = ((const T*) c.val)[(size_t) j * c.cap + tid];
+                q.col[j * n + K] = (short) d.col[(size_t) j * d.cap + K];
+                q.val[j * n + K] = ((const T*) d.val)[(size_t) j * d.cap + K];
             }
+            q.diag[K] = ((const T*) d.diag)[K];
+            q.dinv[K] = ((const T*) d.dinv)[K];
+            if (l == l0) q.b[K] = ((const T*) d.b)[K];
+            if (l < L) q.agg[K] = (short) d.agg[K];
+            if (l > l0)
+                for (int c = 0; c < C; c++) q.child[c * n + K] = (short) d.child[(size_t) c * d.cap + K];
+        }
+    }
+    __syncthreads();
+    // down
+    for (int l = l0; l < L; l++) {
+        const MgLdsLevel<T, W, C> a = level(l), p = level(l + 1);
+        const int n = s_n[l], np = s_n[l + 1];
+        const int items = np * C;
+        for (int base = 0; base < items; base += kMgTailThreads) {
+            const int t = base + tid;
+            const int P = t / C, qs = t % C;
+            T v = (T) 0;
+            if (P < np) {
+                const int K = p.child[qs * np + P];
+                if (K >= 0) {
+                    const T bK = a.b[K];
+                    T acc = a.diag[K] * (omega * bK * a.dinv[K]);
+                    for (int j = 0; j < W; j++) {
+                        const int c = a.col[j * n + K];
+                        if (c < 0) break;
+                        acc += a.val[j * n + K] * (omega * a.b[c] * a.dinv[c]);
+                    }
+                    v = bK - acc;
+                }
+            }
+            for (int o = C / 2; o > 0; o >>= 1) v += __shfl_down(v, o, C);
+            if (P < np && qs == 0) p.b[P] = v;
         }
         __syncthreads();
-        int cur = 0;
+    }
+    // coarsest
+    {
+        const MgLdsLevel<T, W, C> c = level(L);
+        const int n = s_n[L];
+        T* xa = c.x;
+        T* xb = c.x2;
+        for (int K = tid; K < n; K += kMgTailThreads) xa[K] = (T) 0;
+        __syncthreads();
         for (int it = 0; it < sweeps; it++) {
-            if (tid < n) {
-                const T xt = sx[cur][tid];
-                T acc = sd[tid] * xt;
+            for (int K = tid; K < n; K += kMgTailThreads) {
+                const T xt = xa[K];
+                T acc = c.diag[K] * xt;
                 for (int j = 0; j < W; j++) {
-                    const int cc = sc[j][tid];
+                    const int cc = c.col[j * n + K];
                     if (cc < 0) break;
-                    acc += sv[j][tid] * sx[cur][cc];
+                    acc += c.val[j * n + K] * xa[cc];
                 }
-                sx[cur ^ 1][tid] = xt + omega * sdi[tid] * (sb[tid] - acc);
+                xb[K] = xt + omega * c.dinv[K] * (c.b[K] - acc);
             }
             __syncthreads();
-            cur ^= 1;
+            T* t = xa;
+            xa = xb;
+            xb = t;
         }
-        if (tid < n) ((T*) c.x)[tid] = sx[cur][tid];
-        __threadfence_block();
+        if (xa != c.x)
+            for (int K = tid; K < n; K += kMgTailThreads) c.x[K] = xa[K];
         __syncthreads();
     }
+    // up (a level's own x is written, never read, in its phase)
     for (int l = L - 1; l >= l0; l--) {
-        const MgLevelDesc a = lv[l];
-        const T* xc = (const T*) lv[l + 1].x;
-        const int n = nd[l];
-        for (int K = tid; K < n; K += kMgTailThreads)
-            mg_up_row<T, W>(K, a.col, (const T*) a.val, (const T*) a.diag, (const T*) a.dinv, a.cap, a.agg,
-                            (const T*) a.b, xc, (T*) a.x, omega, cf);
-        __threadfence_block();
+        const MgLdsLevel<T, W, C> a = level(l);
+        const T* xc = level(l + 1).x;
+        const int n = s_n[l];
+        for (int K = tid; K < n; K += kMgTailThreads) {
+            const T bK = a.b[K], dK = a.dinv[K];
+            const T xx = omega * bK * dK + cf * xc[a.agg[K]];
+            T acc = a.diag[K] * xx;
+            for (int j = 0; j < W; j++) {
+                const int c = a.col[j * n + K];
+                if (c < 0) break;
+                acc += a.val[j * n + K] * (omega * a.b[c] * a.dinv[c] + cf * xc[a.agg[c]]);
+            }
+            a.x[K] = xx + omega * dK * (bK - acc);
+        }
         __syncthreads();
     }
+    T* xo = (T*) lv[l0].x;
+    const MgLdsLevel<T, W, C> top = level(l0);
+    for (int K = tid; K < s_n[l0]; K += kMgTailThreads) xo[K] = top.x[K];
 }
 
 // Fine level, up: xx = omega b + cf x1[agg0] ...
@@ -584,10 +688,20 @@ static hipError_t mg_read_int(const int* dev, int* host, hipStream_t st) {
     return hipStreamSynchronize(st);
 }
 
+template <typename T, int W, int C>
+static hipError_t mg_allow_tail_lds() {
+    static hipError_t once = hipFuncSetAttribute((const void*) k_mg_tail<T, W, C>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, kMgTailLds);
+    return once;
+}
+
 template <typename T, typename IDX>
 static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, const GridView& g, int dim, const T* fval,
                              const IDX* fcol, hipStream_t st) {
     hipError_t e;
+    // without the large LDS allowance the cycle cannot run: solve without it
+    if ((dim == 2 ? mg_allow_tail_lds<T, 9, 4>() : mg_allow_tail_lds<T, 27, 8>()) != hipSuccess)
+        return hipErrorInvalidValue;
     const int R = A.rows;
     m->W = dim == 2 ? 9 : 27;
     m->C = dim == 2 ? 4 : 8;
@@ -671,7 +785,16 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, const GridV
     m->L = l;
     if (m->n[l] > kMgCoarse) return hipErrorInvalidValue;  // the level cap was hit before the grid got small
     int tail = l;
-    while (tail > 1 && m->n[tail - 1] <= m->tail_rows) tail--;
+    {
+        const int rb = mg_tail_row_bytes(m->W, m->C, (int) sizeof(T)) + 16;
+        long long used = (long long) m->n[l] * rb + 256;
+        while (tail > 1 && m->n[tail - 1] <= m->tail_rows &&
+               used + (long long) m->n[tail - 1] * rb + 256 <= kMgTailLds) {
+            tail--;
+            used += (long long) m->n[tail] * rb + 256;
+        }
+        if (used > kMgTailLds) return hipErrorInvalidValue;
+    }
     if (tail != m->tail) m->gen++;
     m->tail = tail;
     for (int k = 1; k <= l; k++)
@@ -710,8 +833,8 @@ static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval,
         hipLaunchKernelGGL((k_mg_down<T, W, C>), mg_grid(C * m->cap[l + 1]), dim3(kBlock), 0, st, nd + l + 1,
                            m->child[l + 1], m->cap[l + 1], m->col[l], (const T*) m->val[l], (const T*) m->diag[l],
                            (const T*) m->dinv[l], m->cap[l], (const T*) m->b[l], (T*) m->b[l + 1], om, done);
-    hipLaunchKernelGGL((k_mg_tail<T, W, C>), dim3(1), dim3(kMgTailThreads), 0, st, m->desc, nd, lt, L, om, cf,
-                       m->sweeps, done);
+    hipLaunchKernelGGL((k_mg_tail<T, W, C>), dim3(1), dim3(kMgTailThreads), kMgTailLds, st, m->desc, nd, lt, L, om,
+                       cf, m->sweeps, done);
     for (int l = lt - 1; l >= 1; l--)
         hipLaunchKernelGGL((k_mg_up<T, W>), mg_grid(m->cap[l]), dim3(kBlock), 0, st, nd + l, m->col[l],
                            (const T*) m->val[l], (const T*) m->diag[l], (const T*) m->dinv[l], m->cap[l], m->agg[l],

@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_down(const int* __restrict__ npar
                                                     const T* __restrict__ b, T* __restrict__ bp, T omega,
                                                     const int* __restrict__ done) {
     if (mg_done(done)) return;
-    mg_down_item<T, W, C>(blockIdx.x * blockDim.x + threadIdx.x, *nparent, child, capp, col, val, diag, dinv, cap, b,
+    mg_down_item<T, W, C>((int) (blockIdx.x * blockDim.x + threadIdx.x), *nparent, child, capp, col, val, diag, dinv, cap, b,
                           bp, omega);
 }
 

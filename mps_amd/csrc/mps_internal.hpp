@@ -111,6 +111,15 @@ struct Events {
 
 // ---------------- small device helpers ----------------
 
+// blockIdx -> logical block so that each of the 8 XCDs (which take the blocks
+// of a dispatch round robin) works on one contiguous range of the index
+// space and keeps that range's neighbourhood in its own L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int xcd = b & 7, idx = b >> 3;
+    const int per = nb >> 3, rem = nb & 7;
+    return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
 // Cell coordinate as src/bucket.cpp:40-42 computes it: (int)((x-min)/len)+1.
 // Returns false if the value cannot index the grid.
 __device__ __forceinline__ bool cell_coord(double x, double xmin, double len, int n, int& ic) {

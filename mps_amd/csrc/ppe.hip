@@ -41,13 +41,6 @@ namespace mps {
 
 static inline dim3 grid_for(int n) { return dim3((unsigned) ((n + kBlock - 1) / kBlock)); }
 
-// blockIdx -> logical block so that each of the 8 XCDs (dispatched round
-// robin) works on one contiguous range of rows and keeps its halo in its L2.
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-    const int xcd = b & 7, idx = b >> 3;
-    const int per = nb >> 3, rem = nb & 7;
-    return xcd * per + (xcd < rem ? xcd : rem) + idx;
-}
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);

@@ -65,64 +65,94 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_rows(int n, const int* __rest
     atomicMax(&slice_w[r >> 6], nnz_s[s]);
 }
 
+// SELL assembly: one wavefront per block and per slice.  A row's first
+// kFillStage entries are staged in LDS while its neighbours are walked (each
+// lane finds them in a different loop trip), then all 64 lanes write entry k
+// of their rows together, so every store is one coalesced 64-lane line.
+// Entries past kFillStage (dense 3D rows) are written directly.  The 32-bit
+// column array is written only when asked (some offset does not fit 16 bits).
+constexpr int kFillStage = 24;
+
 template <int DIM, bool MULTI>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_fill_sell(GridView g0, Consts c0, SceneTab tab, Hot h,
-                                                             const int* __restrict__ rows_p,
-                                                      const int* __restrict__ row_slot, const int* __restrict__ row_of,
-                                                      const int8_t* __restrict__ bc_s,
-                                                      const double* __restrict__ diag_s,
-                                                      const int* __restrict__ slice_pos, int* __restrict__ col,
-                                                      short* __restrict__ dcol, int* __restrict__ maxdelta,
-                                                      double* __restrict__ val64, float* __restrict__ val32,
-                                                      double* __restrict__ sdiag, double* __restrict__ dgl,
-                                                      const int* __restrict__ raw_off, int* __restrict__ raw_col,
-                                                      double* __restrict__ raw_val) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= *rows_p) return;
-    const int s = row_slot[r];
-    GridView g = g0;
-    Consts c = c0;
-    scene_of<MULTI>(tab, h.id[s], g, c);
-    const double di = diag_s[s];
-    const double si = 1.0 / sqrt(di);
-    sdiag[r] = si;
-    dgl[r] = di;
-    const int sl = r >> 6, lane = r & 63;
-    const size_t base = (size_t) slice_pos[sl] * 64;
+__global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneTab tab, Hot h,
+                                                  const int* __restrict__ rows_p, const int* __restrict__ row_slot,
+                                                  const int* __restrict__ row_of, const int8_t* __restrict__ bc_s,
+                                                  const double* __restrict__ diag_s,
+                                                  const int* __restrict__ slice_pos, int* __restrict__ col,
+                                                  short* __restrict__ dcol, int* __restrict__ maxdelta,
+                                                  double* __restrict__ val64, float* __restrict__ val32,
+                                                  double* __restrict__ sdiag, double* __restrict__ dgl,
+                                                  const int* __restrict__ raw_off, int* __restrict__ raw_col,
+                                                  double* __restrict__ raw_val, int write_col) {
+    __shared__ double s_val[kFillStage][64];
+    __shared__ int s_dl[kFillStage][64];
+    const int lane = threadIdx.x;
+    const int sl = blockIdx.x;
+    const int r = sl * 64 + lane;
+    const int rows = *rows_p;
+    if (sl * 64 >= rows) return;  // block-uniform
+    const size_t base = (size_t) slice_pos[sl] * 64 + lane;
     const int width = slice_pos[sl + 1] - slice_pos[sl];
     int k = 0;
     int md = 0;
-    int ro = raw_off ? raw_off[s] : 0;
-    const double re = c.re_p;
-    const double zi = DIM == 3 ? h.pz[s] : 0.0;
-    for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
-                       [&](int q, double dist, double, double, double) {
-                           if (bc_s[q] != MPS_BC_INNER) return;
-                           // the off-diagonal value of src/mps.cpp:343-348
-                           const double a = c.coefA * wfun(dist, re) / c.rho;
-                           const double v = -a;
-                           const double sj = 1.0 / sqrt(diag_s[q]);
-                           const double sv = v * (si * sj);
-                           const size_t idx = base + (size_t) k * 64 + lane;
-                           const int cq = row_of[q];
-                           const int dl = cq - r;
-                           const int ad = dl < 0 ? -dl : dl;
-                           md = ad > md ? ad : md;
-                           col[idx] = cq;
-                           dcol[idx] = (short) dl;
-                           val64[idx] = sv;
-                           if (val32) val32[idx] = (float) sv;
-                           if (raw_col) {
-                               raw_col[ro] = q;
-                               raw_val[ro] = v;
-                               ro++;
-                           }
-                           k++;
-                       });
-    for (; k < width; k++) {
-        const size_t idx = base + (size_t) k * 64 + lane;
-        col[idx] = r;
+    if (r < rows) {
+        const int s = row_slot[r];
+        GridView g = g0;
+        Consts c = c0;
+        scene_of<MULTI>(tab, h.id[s], g, c);
+        const double di = diag_s[s];
+        const double si = 1.0 / sqrt(di);
+        sdiag[r] = si;
+        dgl[r] = di;
+        int ro = raw_off ? raw_off[s] : 0;
+        const double re = c.re_p;
+        const double zi = DIM == 3 ? h.pz[s] : 0.0;
+        for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
+                           [&](int q, double dist, double, double, double) {
+                               if (bc_s[q] != MPS_BC_INNER) return;
+                               // the off-diagonal value of src/mps.cpp:343-348
+                               const double a = c.coefA * wfun(dist, re) / c.rho;
+                               const double v = -a;
+                               const double sj = 1.0 / sqrt(diag_s[q]);
+                               const double sv = v * (si * sj);
+                               const int cq = row_of[q];
+                               const int dl = cq - r;
+                               const int ad = dl < 0 ? -dl : dl;
+                               md = ad > md ? ad : md;
+                               if (k < kFillStage) {
+                                   s_val[k][lane] = sv;
+                                   s_dl[k][lane] = dl;
+                               } else {
+                                   const size_t idx = base + (size_t) k * 64;
+                                   dcol[idx] = (short) dl;
+                                   if (write_col) col[idx] = cq;
+                                   val64[idx] = sv;
+                                   if (val32) val32[idx] = (float) sv;
+                               }
+                               if (raw_col) {
+                                   raw_col[ro] = q;
+                                   raw_val[ro] = v;
+                                   ro++;
+                               }
+                               k++;
+                           });
+    }
+    // coalesced write-out of the staged entries; padding (value 0, own row) to the slice width
+    const int ws = width < kFillStage ? width : kFillStage;
+    for (int e = 0; e < ws; e++) {
+        const size_t idx = base + (size_t) e * 64;
+        const bool have = e < k;
+        const int dl = have ? s_dl[e][lane] : 0;
+        const double sv = have ? s_val[e][lane] : 0.0;
+        dcol[idx] = (short) dl;
+        if (write_col) col[idx] = r + dl;
+        val64[idx] = sv;
+        if (val32) val32[idx] = (float) sv;
+    }
+    for (int e = k > kFillStage ? k : kFillStage; e < width; e++) {
+        const size_t idx = base + (size_t) e * 64;
         dcol[idx] = 0;
+        if (write_col) col[idx] = r;
         val64[idx] = 0.0;
         if (val32) val32[idx] = 0.0f;
     }
@@ -200,24 +230,29 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
     if (rows == 0) return hipSuccess;
     const int* rows_p = w.row_of + n;
     if ((e = fill_int(A.maxdelta, 0, 1, st)) != hipSuccess) return e;
-    const dim3 fg((rows + bs - 1) / bs), fb(bs);
+    // one wavefront per slice; the 32-bit columns only if some offset needs them
+    const dim3 fg((rows + 63) / 64), fb(64);
+    int md = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        const int write_col = pass;
 #define FILL_ARGS                                                                                                  \
     g, c, tab, h, rows_p, A.row_slot, w.row_of, w.bc_s, w.diag_s, A.slice_off, A.col, A.dcol, A.maxdelta, A.val64, \
-        want_val32 ? A.val32 : nullptr, A.sdiag, A.dgl, want_raw ? A.raw_off : nullptr,                           \
-        want_raw ? A.raw_col : nullptr, want_raw ? A.raw_val : nullptr
-    if (dim == 2 && tab.g)
-        hipLaunchKernelGGL((k_fill_sell<2, true>), fg, fb, 0, st, FILL_ARGS);
-    else if (dim == 2)
-        hipLaunchKernelGGL((k_fill_sell<2, false>), fg, fb, 0, st, FILL_ARGS);
-    else if (tab.g)
-        hipLaunchKernelGGL((k_fill_sell<3, true>), fg, fb, 0, st, FILL_ARGS);
-    else
-        hipLaunchKernelGGL((k_fill_sell<3, false>), fg, fb, 0, st, FILL_ARGS);
+        want_val32 ? A.val32 : nullptr, A.sdiag, A.dgl, (want_raw && !pass) ? A.raw_off : nullptr,                \
+        (want_raw && !pass) ? A.raw_col : nullptr, (want_raw && !pass) ? A.raw_val : nullptr, write_col
+        if (dim == 2 && tab.g)
+            hipLaunchKernelGGL((k_fill_sell<2, true>), fg, fb, 0, st, FILL_ARGS);
+        else if (dim == 2)
+            hipLaunchKernelGGL((k_fill_sell<2, false>), fg, fb, 0, st, FILL_ARGS);
+        else if (tab.g)
+            hipLaunchKernelGGL((k_fill_sell<3, true>), fg, fb, 0, st, FILL_ARGS);
+        else
+            hipLaunchKernelGGL((k_fill_sell<3, false>), fg, fb, 0, st, FILL_ARGS);
 #undef FILL_ARGS
-    // 16-bit column offsets when every row's neighbours lie within +-32767 rows
-    int md = 0;
-    if ((e = hipMemcpyAsync(&md, A.maxdelta, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
-    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+        // 16-bit column offsets when every row's neighbours lie within +-32767 rows
+        if ((e = hipMemcpyAsync(&md, A.maxdelta, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+        if (md <= 32767) break;
+    }
     A.use16 = md <= 32767 ? 1 : 0;
     return hipGetLastError();
 }

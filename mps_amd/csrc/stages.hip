@@ -306,34 +306,43 @@ hipError_t launch_collision(int dim, const GridView& g, const SceneTab& tab, int
 
 // ---------------- boundary condition + source term (src/mps.cpp:262-320) ----------------
 
+// BC and source term, and the start of the Dirichlet connectivity forest:
+// parent = the smallest slot among the particle and its neighbours within
+// re_p (every non-ghost particle, dummy walls included, src/mps.cpp:387-397).
 template <int DIM, bool MULTI>
 __global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g0, Consts c0, SceneTab tab, int n, Hot h, Cold cold,
-                                                      int8_t* __restrict__ bc_s, double* __restrict__ b_s) {
+                                                      int8_t* __restrict__ bc_s, double* __restrict__ b_s,
+                                                      int* __restrict__ parent) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     GridView g = g0;
     Consts c = c0;
     scene_of<MULTI>(tab, h.id[s], g, c);
     const int t = h.type[s];
-    int8_t bc;
+    int8_t bc = MPS_BC_IGNORED;
     double b = 0.0;
-    if (t == MPS_GHOST || t == MPS_DUMMY_WALL) {
-        bc = MPS_BC_IGNORED;
-    } else {
+    int m = s;
+    if (t != MPS_GHOST) {
         const double rs = c.re_s, rp = c.re_p;
         const double re = rs > rp ? rs : rp;
         double ns = 0.0, np = 0.0;
         const double zi = DIM == 3 ? h.pz[s] : 0.0;
         for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
-                           [&](int, double dist, double, double, double) {
+                           [&](int q, double dist, double, double, double) {
                                if (dist < rs) ns += wfun(dist, rs);
-                               if (dist < rp) np += wfun(dist, rp);
+                               if (dist < rp) {
+                                   np += wfun(dist, rp);
+                                   if (q < m) m = q;
+                               }
                            });
-        bc = ns < c.beta * c.n0_s ? MPS_BC_SURFACE : MPS_BC_INNER;
-        if (bc == MPS_BC_INNER && !c.hos_on) b = c.src_scale * (np - c.n0_p) / c.n0_p;
+        if (t != MPS_DUMMY_WALL) {
+            bc = ns < c.beta * c.n0_s ? MPS_BC_SURFACE : MPS_BC_INNER;
+            if (bc == MPS_BC_INNER && !c.hos_on) b = c.src_scale * (np - c.n0_p) / c.n0_p;
+        }
     }
     bc_s[s] = bc;
     b_s[s] = b;
+    parent[s] = m;
     cold.bc[h.id[s]] = bc;
 }
 
@@ -364,45 +373,19 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_source_hos(GridView g0, Const
 hipError_t launch_bc_source(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
                             Cold& cold, StepScratch& w, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
-    LAUNCH_DM(k_bc_source, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, cold, w.bc_s, w.b_s);
+    LAUNCH_DM(k_bc_source, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, cold, w.bc_s, w.b_s, w.parent);
     if (c.hos_on) LAUNCH_DM(k_source_hos, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, cold, w.bc_s, w.b_s);
     return hipGetLastError();
 }
 
 // ---------------- matrix diagonal + Dirichlet connectivity (src/mps.cpp:322-415) ----------------
 
-template <int DIM, bool MULTI>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_diag(GridView g0, Consts c0, SceneTab tab, int n, Hot h, const int8_t* __restrict__ bc_s,
-                                                 double* __restrict__ diag, int* __restrict__ nnz) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    GridView g = g0;
-    Consts c = c0;
-    scene_of<MULTI>(tab, h.id[s], g, c);
-    double d = 0.0;
-    int k = 0;
-    if (bc_s[s] == MPS_BC_INNER) {
-        const double re = c.re_p;
-        const double zi = DIM == 3 ? h.pz[s] : 0.0;
-        for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
-                           [&](int q, double dist, double, double, double) {
-                               const int8_t bq = bc_s[q];
-                               if (bq == MPS_BC_IGNORED) return;
-                               const double a = c.coefA * wfun(dist, re) / c.rho;
-                               d += a;
-                               if (bq == MPS_BC_INNER) k++;
-                           });
-        if (c.qc_on) d += c.qc_add;
-    }
-    diag[s] = d;
-    nnz[s] = k;
-}
-
 // Connectivity: the reference runs a multi-source BFS from every Surface
 // particle over edges dist < re_p through any particle (src/mps.cpp:373-400).
 // The visited set is the union of connected components that hold a Surface
 // particle, found here with lock-free union-find (hook the larger root under
-// the smaller with atomicCAS, path halving on find).
+// the smaller with atomicCAS, path halving on find), started from the
+// parents k_bc_source wrote.
 __device__ __forceinline__ int cc_find(volatile int* parent, int v) {
     int curr = parent[v];
     if (curr != v) {
@@ -416,51 +399,54 @@ __device__ __forceinline__ int cc_find(volatile int* parent, int v) {
     return curr;
 }
 
+// Matrix diagonal (src/mps.cpp:336-356) and off-diagonal count of Inner
+// rows, and the union-find hooking over the same re_p neighbours.
 template <int DIM, bool MULTI>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_cc_init(GridView g0, Consts c0, SceneTab tab, int n, Hot h, int* __restrict__ parent) {
+__global__ void __launch_bounds__(kMaxTuneBlock) k_diag_join(GridView g0, Consts c0, SceneTab tab, int n, Hot h,
+                                                             const int8_t* __restrict__ bc_s,
+                                                             double* __restrict__ diag, int* __restrict__ nnz,
+                                                             int* parent) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     GridView g = g0;
     Consts c = c0;
     scene_of<MULTI>(tab, h.id[s], g, c);
-    int m = s;
+    double d = 0.0;
+    int k = 0;
     if (h.type[s] != MPS_GHOST) {
+        const bool inner = bc_s[s] == MPS_BC_INNER;
+        const double re = c.re_p;
         const double zi = DIM == 3 ? h.pz[s] : 0.0;
-        for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, c.re_p,
-                           [&](int q, double, double, double, double) {
-                               if (q < m) m = q;
-                           });
-    }
-    parent[s] = m;
-}
-
-template <int DIM, bool MULTI>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_cc_join(GridView g0, Consts c0, SceneTab tab, int n, Hot h, int* parent) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    GridView g = g0;
-    Consts c = c0;
-    scene_of<MULTI>(tab, h.id[s], g, c);
-    if (h.type[s] == MPS_GHOST) return;
-    volatile int* vp = parent;
-    const double zi = DIM == 3 ? h.pz[s] : 0.0;
-    for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, c.re_p,
-                       [&](int q, double, double, double, double) {
-                           if (q <= s) return;
-                           int a = cc_find(vp, s);
-                           int b = cc_find(vp, q);
-                           while (a != b) {
-                               if (a < b) {
-                                   const int r = atomicCAS(&parent[b], b, a);
-                                   if (r == b) break;
-                                   b = r;
-                               } else {
-                                   const int r = atomicCAS(&parent[a], a, b);
-                                   if (r == a) break;
-                                   a = r;
+        volatile int* vp = parent;
+        for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
+                           [&](int q, double dist, double, double, double) {
+                               if (inner) {
+                                   const int8_t bq = bc_s[q];
+                                   if (bq != MPS_BC_IGNORED) {
+                                       const double a = c.coefA * wfun(dist, re) / c.rho;
+                                       d += a;
+                                       if (bq == MPS_BC_INNER) k++;
+                                   }
                                }
-                           }
-                       });
+                               if (q <= s) return;
+                               int a = cc_find(vp, s);
+                               int b = cc_find(vp, q);
+                               while (a != b) {
+                                   if (a < b) {
+                                       const int r = atomicCAS(&parent[b], b, a);
+                                       if (r == b) break;
+                                       b = r;
+                                   } else {
+                                       const int r = atomicCAS(&parent[a], a, b);
+                                       if (r == a) break;
+                                       a = r;
+                                   }
+                               }
+                           });
+        if (inner && c.qc_on) d += c.qc_add;
+    }
+    diag[s] = d;
+    nnz[s] = k;
 }
 
 __global__ void __launch_bounds__(kMaxTuneBlock) k_cc_finalize(int n, int* parent, const int8_t* __restrict__ bc_s,
@@ -492,9 +478,7 @@ hipError_t launch_diag_cc(int dim, const GridView& g, const SceneTab& tab, int n
                           StepScratch& w, Events& ev, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
     const dim3 gr = grid_for(n, bs), bl(bs);
-    LAUNCH_DM(k_diag, gr, bl, st, g, c, tab, n, h, w.bc_s, w.diag_s, w.nnz_s);
-    LAUNCH_DM(k_cc_init, gr, bl, st, g, c, tab, n, h, w.parent);
-    LAUNCH_DM(k_cc_join, gr, bl, st, g, c, tab, n, h, w.parent);
+    LAUNCH_DM(k_diag_join, gr, bl, st, g, c, tab, n, h, w.bc_s, w.diag_s, w.nnz_s, w.parent);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // surf[] is a byte array: clear it with the int fill over its rounded-up size
@@ -505,7 +489,7 @@ hipError_t launch_diag_cc(int dim, const GridView& g, const SceneTab& tab, int n
     return hipGetLastError();
 }
 
-// ---------------- pressure out, minimum pressure (src/mps.cpp:466-503) ----------------
+// ---------------- pressure out, clamp (src/mps.cpp:466-479) ----------------
 
 __global__ void __launch_bounds__(kMaxTuneBlock) k_pressure_out(int n, Hot h, const int8_t* __restrict__ bc_s,
                                                          const double* __restrict__ x_s, Cold cold,
@@ -520,46 +504,26 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_pressure_out(int n, Hot h, co
     cold.p[id] = p;
 }
 
-template <int DIM, bool MULTI>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_min_pressure(GridView g0, Consts c0, SceneTab tab, int n, Hot h,
-                                                         const int8_t* __restrict__ bc_s,
-                                                         const double* __restrict__ p_s, Cold cold,
-                                                         double* __restrict__ pmin_s) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    GridView g = g0;
-    Consts c = c0;
-    scene_of<MULTI>(tab, h.id[s], g, c);
-    const int id = h.id[s];
-    if (bc_s[s] == MPS_BC_IGNORED) {
-        pmin_s[s] = cold.pmin[id];
-        return;
-    }
-    double pm = p_s[s];
-    const double zi = DIM == 3 ? h.pz[s] : 0.0;
-    for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, c.re_p,
-                       [&](int q, double, double, double, double) {
-                           if (h.type[q] == MPS_DUMMY_WALL) return;
-                           if (pm > p_s[q]) pm = p_s[q];
-                       });
-    pmin_s[s] = pm;
-    cold.pmin[id] = pm;
-}
-
 hipError_t launch_pressure_out(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
                                const double* x_s, Cold& cold, StepScratch& w, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_pressure_out, grid_for(n, bs), dim3(bs), 0, st, n, h, w.bc_s, x_s, cold, w.p_s);
-    LAUNCH_DM(k_min_pressure, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, w.bc_s, w.p_s, cold, w.pmin_s);
     return hipGetLastError();
 }
 
-// ---------------- pressure gradient + implicit move (src/mps.cpp:505-545) ----------------
+// ---------------- minimum pressure, pressure gradient, implicit move (src/mps.cpp:481-545) ----------------
+//
+// One thread per slot walks its re_p neighbours twice: first for the minimum
+// pressure (src/mps.cpp:487-500: p_i itself and every neighbour that is not a
+// dummy wall; Ignored particles keep their old value), which needs only the
+// clamped pressures of the pass before, then for the gradient, which needs
+// only its own minimum.  The second walk finds the neighbours in cache.
 
 template <int DIM, bool MULTI>
 __global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g0, Consts c0, SceneTab tab, int n, Hot cur, Hot out, Cold cold,
+                                                      const int8_t* __restrict__ bc_s,
                                                       const double* __restrict__ p_s,
-                                                      const double* __restrict__ pmin_s, Events ev) {
+                                                      double* __restrict__ pmin_s, Events ev) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     GridView g = g0;
@@ -567,9 +531,23 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g0, Consts
     scene_of<MULTI>(tab, cur.id[s], g, c);
     double x = cur.px[s], y = cur.py[s], z = DIM == 3 ? cur.pz[s] : 0.0;
     double vx = cur.vx[s], vy = cur.vy[s], vz = DIM == 3 ? cur.vz[s] : 0.0;
-    if (cur.type[s] == MPS_FLUID) {
+    const int t = cur.type[s];
+    const int id = cur.id[s];
+    double pmi;
+    if (bc_s[s] == MPS_BC_IGNORED) {
+        pmi = cold.pmin[id];
+    } else {
+        pmi = p_s[s];
+        for_neighbors<DIM>(g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, c.re_p,
+                           [&](int q, double, double, double, double) {
+                               if (cur.type[q] == MPS_DUMMY_WALL) return;
+                               if (pmi > p_s[q]) pmi = p_s[q];
+                           });
+        cold.pmin[id] = pmi;
+    }
+    pmin_s[s] = pmi;
+    if (t == MPS_FLUID) {
         const double re = c.re_p;
-        const double pmi = pmin_s[s];
         double gx = 0.0, gy = 0.0, gz = 0.0;
         for_neighbors<DIM>(g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, re,
                            [&](int q, double dist, double dx, double dy, double dz) {
@@ -594,7 +572,6 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g0, Consts
         if (DIM == 3) z += az * c.dt * c.dt;
         if (!in_domain(c, x, y, DIM == 3 ? z : c.dmin[2])) {
             cur.type[s] = MPS_GHOST;
-            const int id = cur.id[s];
             push_event(ev.n_ghost, ev.ghost_ids, ev.cap, id);
             cold.ghost_n[id] = ndens<DIM>(g, cur, s, x0, y0, z0, c.re_p);
         }
@@ -612,7 +589,7 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g0, Consts
 hipError_t launch_grad_move(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& cur,
                             Hot& out, Cold& cold, StepScratch& w, Events& ev, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
-    LAUNCH_DM(k_grad_move, grid_for(n, bs), dim3(bs), st, g, c, tab, n, cur, out, cold, w.p_s, w.pmin_s, ev);
+    LAUNCH_DM(k_grad_move, grid_for(n, bs), dim3(bs), st, g, c, tab, n, cur, out, cold, w.bc_s, w.p_s, w.pmin_s, ev);
     return hipGetLastError();
 }
 

@@ -6,6 +6,8 @@ state entering the pressure stage.  Within stated FP64 tolerances: the
 pressure solution (two different CG summation orders, both run to near
 machine precision) and the state after the step.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -260,3 +262,36 @@ def test_autotuner_changes_speed_not_results():
         runs.append(g.state())
     for k in ("pos", "vel", "p", "ndr", "bc", "type"):
         assert (runs[0][k] == runs[1][k]).all() and (runs[0][k] == runs[2][k]).all(), k
+
+
+def _f3_case(**kw):
+    s, sc, _ = configs.workload("dambreak2d_584")
+    s = dataclasses.replace(s, **kw)
+    return s, _jitter(s, sc, 21)
+
+
+@pytest.mark.parametrize("opts", [dict(hos_on=True, hos_gamma=1e-3), dict(qc_on=False),
+                                  dict(hos_on=True, hos_gamma=5e-3, qc_on=False)],
+                         ids=["hos", "qc_off", "hos_qc_off"])
+def test_pressure_stabilisation_options(opts):
+    """The higher-order source term (src/mps.cpp:288-308, which reads the
+    previous step's number density ratio and the neighbours' velocities) and
+    the quasi-compressibility-off branch (src/mps.cpp:352-355): the system of
+    every step is bit-exact, the state within the solver tolerance."""
+    s, sc = _f3_case(**opts)
+    o, g = _pair(s, sc)
+    o.load_initial_state()
+    g.load_initial_state()
+    g.set_snapshot(True)
+    for k in range(4):
+        o.step(False)
+        g.step(False)
+        op, gp = o.pressure_system(), g.pressure_system()
+        assert (gp["b"] == op["b"]).all(), k
+        assert (gp["diag"] == op["diag"]).all(), k
+        assert (gp["val"] == op["val"]).all(), k
+        nx = np.linalg.norm(op["x"])
+        assert np.linalg.norm(gp["x"] - op["x"]) <= P_RTOL * max(nx, 1e-300), k
+        # keep the two runs on the same state so later steps compare the same system
+        st = o.state()
+        g.upload(pos=st["pos"], vel=st["vel"], p=st["p"], ndr=st["ndr"])

@@ -290,6 +290,7 @@ static int alloc_all(mps_handle* h) {
     HIPCHK(h, dalloc(&w.bc_s, n));
     HIPCHK(h, dalloc(&w.b_s, n));
     HIPCHK(h, dalloc(&w.diag_s, n));
+    HIPCHK(h, dalloc(&w.isq_s, n));
     HIPCHK(h, dalloc(&w.nnz_s, n));
     HIPCHK(h, dalloc(&w.parent, n));
     HIPCHK(h, dalloc(&w.surf, n + 8));
@@ -360,7 +361,7 @@ static void free_all(mps_handle* h) {
     }
     void* ptrs[] = {h->bb.cnt,      h->bb.cs,         h->bb.bin_id,  h->bb.dst,       h->bb.scan_tmp, h->bb.slot_of,
                     h->cold.p,      h->cold.pmin,     h->cold.ndr,   h->cold.bc,      h->cold.pprev,  h->cold.ghost_n,
-                    h->cold.z,      h->cold.vz,       h->w.bc_s,     h->w.b_s,        h->w.diag_s,    h->w.nnz_s,
+                    h->cold.z,      h->cold.vz,       h->w.bc_s,     h->w.b_s,        h->w.diag_s,    h->w.isq_s,    h->w.nnz_s,
                     h->w.parent,    h->w.surf,        h->w.inner_flag, h->w.row_of,   h->w.p_s,       h->w.pmin_s,
                     h->w.vmax,      h->w.coll_cnt,    h->w.coll_off, h->w.coll_pairs, h->w.coll_dist, h->w.coll_level,
                     h->w.coll_last, h->x_s,           h->dbg_id,     h->tmp3a,        h->tmp3b,       h->tmp8,

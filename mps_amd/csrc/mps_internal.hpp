@@ -151,21 +151,38 @@ __device__ __forceinline__ void for_neighbors(const GridView& g, const double* _
     if constexpr (DIM == 2) {
         const int ix = key % g.nx;
         const int iy = key / g.nx;
-        for (int jx = ix - 1; jx <= ix + 1; jx++) {
-            if (jx < 0 || jx >= g.nx) continue;
-            for (int jy = iy - 1; jy <= iy + 1; jy++) {
-                if (jy < 0 || jy >= g.ny) continue;
+        // the 9 cells' slot ranges first (independent loads, one round trip),
+        // then the walk in the reference's order.  Candidates with
+        // d2 > re^2 (1 + 1e-12) are skipped before the square root: sqrt is
+        // correctly rounded and monotone, so for them sqrt(d2) >= re and the
+        // reference's test dist < re fails too.
+        const double re2 = re * re * (1.0 + 1e-12);
+        int cb[9], ce[9];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                const int jx = ix - 1 + a, jy = iy - 1 + b;
+                const bool ok = jx >= 0 && jx < g.nx && jy >= 0 && jy < g.ny;
                 const int k = jx + jy * g.nx;
-                const int e = cs[k + 1];
-                for (int q = cs[k]; q < e; q++) {
-                    const double dx = px[q] - xi;
-                    const double dy = py[q] - yi;
-                    const double dist = sqrt(dx * dx + dy * dy);
-                    if (q != s && dist < re) f(q, dist, dx, dy, 0.0);
-                }
+                cb[3 * a + b] = ok ? cs[k] : 0;
+                ce[3 * a + b] = ok ? cs[k + 1] : 0;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+            const int e = ce[c];
+            for (int q = cb[c]; q < e; q++) {
+                const double dx = px[q] - xi;
+                const double dy = py[q] - yi;
+                const double d2 = dx * dx + dy * dy;
+                if (d2 > re2) continue;
+                const double dist = sqrt(d2);
+                if (q != s && dist < re) f(q, dist, dx, dy, 0.0);
             }
         }
     } else {
+        const double re2 = re * re * (1.0 + 1e-12);  // see the 2D branch
         int ix, iy, iz;
         if (g.flat) {
             ix = key % g.nx;
@@ -192,7 +209,9 @@ __device__ __forceinline__ void for_neighbors(const GridView& g, const double* _
                         const double dx = px[q] - xi;
                         const double dy = py[q] - yi;
                         const double dz = pz[q] - zi;
-                        const double dist = sqrt((dx * dx + dy * dy) + dz * dz);
+                        const double d2 = (dx * dx + dy * dy) + dz * dz;
+                        if (d2 > re2) continue;
+                        const double dist = sqrt(d2);
                         if (q != s && dist < re) f(q, dist, dx, dy, dz);
                     }
                 }
@@ -240,6 +259,7 @@ struct StepScratch {
     int8_t* bc_s;       // slot order
     double* b_s;        // source term
     double* diag_s;     // matrix diagonal
+    double* isq_s;      // 1 / sqrt(diagonal) of Inner slots (the Jacobi scaling)
     int* nnz_s;         // off-diagonal count per slot (Inner rows)
     int* parent;        // connectivity forest
     int8_t* surf;       // component holds a Surface particle

@@ -78,6 +78,7 @@ __global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneT
                                                   const int* __restrict__ rows_p, const int* __restrict__ row_slot,
                                                   const int* __restrict__ row_of, const int8_t* __restrict__ bc_s,
                                                   const double* __restrict__ diag_s,
+                                                  const double* __restrict__ isq_s,
                                                   const int* __restrict__ slice_pos, int* __restrict__ col,
                                                   short* __restrict__ dcol, int* __restrict__ maxdelta,
                                                   double* __restrict__ val64, float* __restrict__ val32,
@@ -101,7 +102,7 @@ __global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneT
         Consts c = c0;
         scene_of<MULTI>(tab, h.id[s], g, c);
         const double di = diag_s[s];
-        const double si = 1.0 / sqrt(di);
+        const double si = isq_s[s];  // 1 / sqrt(di), from the connectivity pass
         sdiag[r] = si;
         dgl[r] = di;
         int ro = raw_off ? raw_off[s] : 0;
@@ -113,7 +114,7 @@ __global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneT
                                // the off-diagonal value of src/mps.cpp:343-348
                                const double a = c.coefA * wfun(dist, re) / c.rho;
                                const double v = -a;
-                               const double sj = 1.0 / sqrt(diag_s[q]);
+                               const double sj = isq_s[q];  // 1 / sqrt(diag_s[q])
                                const double sv = v * (si * sj);
                                const int cq = row_of[q];
                                const int dl = cq - r;
@@ -236,7 +237,7 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
     for (int pass = 0; pass < 2; pass++) {
         const int write_col = pass;
 #define FILL_ARGS                                                                                                  \
-    g, c, tab, h, rows_p, A.row_slot, w.row_of, w.bc_s, w.diag_s, A.slice_off, A.col, A.dcol, A.maxdelta, A.val64, \
+    g, c, tab, h, rows_p, A.row_slot, w.row_of, w.bc_s, w.diag_s, w.isq_s, A.slice_off, A.col, A.dcol, A.maxdelta, A.val64, \
         want_val32 ? A.val32 : nullptr, A.sdiag, A.dgl, (want_raw && !pass) ? A.raw_off : nullptr,                \
         (want_raw && !pass) ? A.raw_col : nullptr, (want_raw && !pass) ? A.raw_val : nullptr, write_col
         if (dim == 2 && tab.g)

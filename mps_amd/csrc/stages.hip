@@ -460,18 +460,23 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_cc_finalize(int n, int* paren
     if (bc_s[s] == MPS_BC_SURFACE) surf[r] = 1;
 }
 
+// Dirichlet doubling (src/mps.cpp:403-414) and, for the Jacobi scaling of
+// the solve, 1/sqrt of each Inner row's final diagonal.
 __global__ void __launch_bounds__(kMaxTuneBlock) k_cc_apply(int n, Hot h, const int* __restrict__ parent,
                                                      const int8_t* __restrict__ bc_s, const int8_t* __restrict__ surf,
-                                                     double* __restrict__ diag, int* __restrict__ inner_flag,
-                                                     Events ev) {
+                                                     double* __restrict__ diag, double* __restrict__ isq,
+                                                     int* __restrict__ inner_flag, Events ev) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const bool inner = bc_s[s] == MPS_BC_INNER;
     inner_flag[s] = inner ? 1 : 0;
+    double d = diag[s];
     if (inner && !surf[parent[s]]) {
-        diag[s] *= 2.0;
+        d *= 2.0;
+        diag[s] = d;
         push_event(ev.n_unconn, ev.unconn_ids, ev.cap, h.id[s]);
     }
+    isq[s] = inner ? 1.0 / sqrt(d) : 0.0;
 }
 
 hipError_t launch_diag_cc(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
@@ -485,7 +490,7 @@ hipError_t launch_diag_cc(int dim, const GridView& g, const SceneTab& tab, int n
     e = fill_int(reinterpret_cast<int*>(w.surf), 0, ((size_t) n + 3) / 4, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_cc_finalize, gr, bl, 0, st, n, w.parent, w.bc_s, w.surf);
-    hipLaunchKernelGGL(k_cc_apply, gr, bl, 0, st, n, h, w.parent, w.bc_s, w.surf, w.diag_s, w.inner_flag, ev);
+    hipLaunchKernelGGL(k_cc_apply, gr, bl, 0, st, n, h, w.parent, w.bc_s, w.surf, w.diag_s, w.isq_s, w.inner_flag, ev);
     return hipGetLastError();
 }
 

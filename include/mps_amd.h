@@ -260,6 +260,24 @@ int mps_adjoint_record(mps_handle* h, int32_t steps, int32_t checkpoint_interval
 int mps_adjoint_backward(mps_handle* h, const double* pos_bar, const double* vel_bar, double* pos_bar0,
                          double* vel_bar0, double* param_bar, mps_solver_stats* adjoint_stats);
 
+/* Asynchronous export snapshot (no counterpart in the reference, whose
+ * exporter reads the particle vector synchronously, src/simulation.cpp:231-245).
+ * mps_snapshot_begin queues, on the handle's stream, the gather of type,
+ * position, velocity, pressure, number density ratio and boundary condition
+ * into id order and their copy into library-owned pinned host memory of slot
+ * 0 or 1, and returns without waiting: later steps queue behind the copy, so
+ * the state copied is the one of the call.  mps_snapshot_end waits for that
+ * copy and returns pointers into the slot (N int8 types, N x 3 positions and
+ * velocities, N pressures and number density ratios, N int8 boundary
+ * conditions; any pointer argument may be NULL).  They stay valid until the
+ * slot is begun again or the handle is destroyed.  mps_snapshot_end touches
+ * only the slot, so another host thread may call it while the handle's owner
+ * keeps stepping; it reports failure by status only (mps_last_error is not
+ * updated). */
+int mps_snapshot_begin(mps_handle* h, int32_t slot);
+int mps_snapshot_end(mps_handle* h, int32_t slot, const int8_t** type, const double** pos, const double** vel,
+                     const double** p, const double** ndr, const int8_t** bc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,8 +20,8 @@ INCLUDE = os.path.join(ROOT, "include")
 
 SOURCES = ["grid.hip", "stages.hip", "ppe.hip", "mg.hip", "adjoint.hip", "capi.hip"]
 HOST = os.path.join(HERE, "host")
-HOST_SOURCES = ["particle.cpp", "logger.cpp", "settings.cpp", "exporter.cpp", "bucket.cpp", "mps.cpp",
-                "simulation.cpp", "hooks.cpp"]
+HOST_SOURCES = ["particle.cpp", "logger.cpp", "settings.cpp", "exporter.cpp", "async_export.cpp", "bucket.cpp",
+                "mps.cpp", "simulation.cpp", "hooks.cpp"]
 HOST_LIB = os.path.join(HERE, "libmps_host.so")
 CLI = os.path.join(HERE, "mps")
 CXX = os.environ.get("CXX", "g++")

@@ -31,7 +31,7 @@ EXPORTED = [
     "mps_dump_neighbors", "mps_get_pressure_system", "mps_set_snapshot", "mps_get_snapshot",
     "mps_time_pressure_kernels", "mps_set_block_sizes", "mps_autotune",
     "mps_create_ensemble", "mps_scene_count", "mps_ensemble_stats",
-    "mps_adjoint_record", "mps_adjoint_backward",
+    "mps_adjoint_record", "mps_adjoint_backward", "mps_snapshot_begin", "mps_snapshot_end",
 ]
 
 
@@ -113,6 +113,8 @@ def lib():
     L.mps_ensemble_stats.argtypes = [vp, vp, vp]
     L.mps_adjoint_record.argtypes = [vp, i32, i32, vp]
     L.mps_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.POINTER(SolverStats)]
+    L.mps_snapshot_begin.argtypes = [vp, i32]
+    L.mps_snapshot_end.argtypes = [vp, i32] + [ctypes.POINTER(vp)] * 6
     for name in EXPORTED:
         if name not in ("mps_last_error",):
             getattr(L, name).restype = i32

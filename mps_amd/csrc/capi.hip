@@ -48,6 +48,21 @@ struct AdjStore {
     std::vector<void*> owned;          // everything above, for release
 };
 
+// Pinned host copy of the export fields (id order), filled asynchronously.
+struct ExportSlot {
+    int8_t* type = nullptr;
+    double* pos = nullptr;
+    double* vel = nullptr;
+    double* p = nullptr;
+    double* ndr = nullptr;
+    int8_t* bc = nullptr;
+    hipEvent_t done = nullptr;
+    double* dpos = nullptr;   // device staging (id order)
+    double* dvel = nullptr;
+    int8_t* dtype = nullptr;
+    bool pending = false;
+};
+
 struct mps_handle {
     std::string err;
     mps_settings s{};  // scene 0
@@ -105,6 +120,7 @@ struct mps_handle {
     int nsites = 0;
     double tacc[9] = {};
     AdjStore adj;
+    ExportSlot xs[2];
 };
 
 // Kernel families the autotuner sizes independently.
@@ -372,6 +388,15 @@ static void free_all(mps_handle* h) {
     for (void* p : ptrs) hipFree(p);
     for (void* p : h->adj.owned) hipFree(p);
     h->adj.owned.clear();
+    for (ExportSlot& x : h->xs) {
+        if (x.done) hipEventDestroy(x.done);
+        void* hp[] = {x.type, x.pos, x.vel, x.p, x.ndr, x.bc};
+        for (void* q : hp)
+            if (q) hipHostFree(q);
+        hipFree(x.dpos);
+        hipFree(x.dvel);
+        hipFree(x.dtype);
+    }
     if (h->pinned_i) hipHostFree(h->pinned_i);
     if (h->pinned_d) hipHostFree(h->pinned_d);
     if (h->cg) cg_destroy(h->cg);
@@ -1482,6 +1507,55 @@ int mps_adjoint_backward(mps_handle* h, const double* pos_bar, const double* vel
     }
     HIPCHK(h, hipMemcpy(param_bar, S.a.acc, 3 * sizeof(double), hipMemcpyDeviceToHost));
     if (adjoint_stats) *adjoint_stats = agg;
+    return MPS_OK;
+}
+
+int mps_snapshot_begin(mps_handle* h, int32_t slot) {
+    if (!h || slot < 0 || slot > 1) return MPS_E_INVALID;
+    ExportSlot& x = h->xs[slot];
+    const size_t N = (size_t) h->n;
+    if (!x.done) {
+        HIPCHK(h, hipEventCreateWithFlags(&x.done, hipEventDisableTiming));
+        const size_t n1 = N ? N : 1;
+        HIPCHK(h, hipHostMalloc((void**) &x.type, n1, hipHostMallocDefault));
+        HIPCHK(h, hipHostMalloc((void**) &x.bc, n1, hipHostMallocDefault));
+        HIPCHK(h, hipHostMalloc((void**) &x.pos, 3 * n1 * sizeof(double), hipHostMallocDefault));
+        HIPCHK(h, hipHostMalloc((void**) &x.vel, 3 * n1 * sizeof(double), hipHostMallocDefault));
+        HIPCHK(h, hipHostMalloc((void**) &x.p, n1 * sizeof(double), hipHostMallocDefault));
+        HIPCHK(h, hipHostMalloc((void**) &x.ndr, n1 * sizeof(double), hipHostMallocDefault));
+        HIPCHK(h, dalloc(&x.dpos, 3 * n1));
+        HIPCHK(h, dalloc(&x.dvel, 3 * n1));
+        HIPCHK(h, dalloc(&x.dtype, n1));
+    }
+    if (N) {
+        const Hot c = cur(h);
+        HIPCHK(h, launch_snapshot(h->dim, h->n, c, h->cold, x.dpos, x.dvel, h->st));
+        HIPCHK(h, launch_gather_type(h->n, c, x.dtype, h->st));
+        HIPCHK(h, hipMemcpyAsync(x.pos, x.dpos, 3 * N * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(h, hipMemcpyAsync(x.vel, x.dvel, 3 * N * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(h, hipMemcpyAsync(x.type, x.dtype, N, hipMemcpyDeviceToHost, h->st));
+        HIPCHK(h, hipMemcpyAsync(x.p, h->cold.p, N * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(h, hipMemcpyAsync(x.ndr, h->cold.ndr, N * sizeof(double), hipMemcpyDeviceToHost, h->st));
+        HIPCHK(h, hipMemcpyAsync(x.bc, h->cold.bc, N, hipMemcpyDeviceToHost, h->st));
+    }
+    HIPCHK(h, hipEventRecord(x.done, h->st));
+    x.pending = true;
+    return MPS_OK;
+}
+
+int mps_snapshot_end(mps_handle* h, int32_t slot, const int8_t** type, const double** pos, const double** vel,
+                     const double** p, const double** ndr, const int8_t** bc) {
+    if (!h || slot < 0 || slot > 1) return MPS_E_INVALID;
+    // may run on another host thread while the owner steps: touches only the slot
+    ExportSlot& x = h->xs[slot];
+    if (!x.pending) return MPS_E_INVALID;
+    if (hipEventSynchronize(x.done) != hipSuccess) return MPS_E_HIP;
+    if (type) *type = x.type;
+    if (pos) *pos = x.pos;
+    if (vel) *vel = x.vel;
+    if (p) *p = x.p;
+    if (ndr) *ndr = x.ndr;
+    if (bc) *bc = x.bc;
     return MPS_OK;
 }
 

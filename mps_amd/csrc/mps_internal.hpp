@@ -306,6 +306,7 @@ hipError_t launch_courant(int dim, int n, const Hot& h, const SceneTab& tab, int
                           hipStream_t st);
 hipError_t launch_snapshot(int dim, int n, const Hot& h, const Cold& cold, double* pos_id, double* vel_id,
                            hipStream_t st);
+hipError_t launch_gather_type(int n, const Hot& h, int8_t* type_id, hipStream_t st);
 
 // ppe.hip — matrix storage and CG
 struct Sell {

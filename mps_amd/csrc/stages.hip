@@ -687,6 +687,18 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_gather_id(int n, int dim, Hot
     vel[3 * (size_t) id + 2] = dim == 3 ? h.vz[s] : cold.vz[id];
 }
 
+__global__ void __launch_bounds__(kBlock) k_gather_type(int n, Hot h, int8_t* __restrict__ type_id) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    type_id[h.id[s]] = h.type[s];
+}
+
+hipError_t launch_gather_type(int n, const Hot& h, int8_t* type_id, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_type, grid_for(n, kBlock), dim3(kBlock), 0, st, n, h, type_id);
+    return hipGetLastError();
+}
+
 hipError_t launch_snapshot(int dim, int n, const Hot& h, const Cold& cold, double* pos_id, double* vel_id,
                            hipStream_t st) {
     if (n == 0) return hipSuccess;

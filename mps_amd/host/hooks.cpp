@@ -62,3 +62,43 @@ int mps_host_bucket_from_csv(const char* csv, double re_max, const double* dmin,
 }
 
 }
+
+#include <fstream>
+
+#include "async_export.hpp"
+
+extern "C" {
+
+// Writes one particle table (id order; type/bc int8, pos/vel n x 3) through
+// the synchronous writers (Exporter on a Particle vector) and through the
+// threaded snapshot writers (csvText / vtuText with `threads` threads):
+// out_prefix + "sync.csv", "sync.vtu", "async.csv", "async.vtu".
+int mps_host_export_both(int n, const int8_t* type, const double* pos, const double* vel, const double* p,
+                         const double* ndr, const int8_t* bc, double time, const char* out_prefix, int threads) {
+    const std::string pre(out_prefix);
+    std::vector<Particle> ps;
+    ps.reserve((size_t) n);
+    for (int i = 0; i < n; i++) {
+        ps.emplace_back(i, static_cast<ParticleType>(type[i]), Vec3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]),
+                        Vec3(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]), 1000.0);
+        ps.back().pressure = p[i];
+        ps.back().numberDensityRatio = ndr[i];
+        ps.back().boundaryCondition = static_cast<BoundaryCondition>(bc[i]);
+    }
+    Exporter ex;
+    ex.toCsv(pre + "sync.csv", ps, time);
+    ex.toVtu(pre + "sync.vtu", ps, time);
+    ExportView v;
+    v.n = (size_t) n;
+    v.type = type;
+    v.pos = pos;
+    v.vel = vel;
+    v.p = p;
+    v.ndr = ndr;
+    v.bc = bc;
+    std::ofstream(pre + "async.csv", std::ios::binary) << csvText(v, time, threads);
+    std::ofstream(pre + "async.vtu", std::ios::binary) << vtuText(v, time, threads);
+    return 0;
+}
+
+}

@@ -142,7 +142,17 @@ double MPS::stepForward(const bool isTimeToExport) {
     double courant = 0.0;
     check(mps_step(h, isTimeToExport ? 1 : 0, &courant), "mps_step");
     reportEvents(courant);
-    if (isTimeToExport) syncParticles();
+    if (isTimeToExport && syncOnExport) syncParticles();
     lastCourant = courant;
     return courant;
+}
+
+void MPS::snapshotBegin(int slot) { check(mps_snapshot_begin(h, slot), "mps_snapshot_begin"); }
+
+ExportView MPS::snapshotEnd(int slot) const {
+    ExportView v;
+    v.n = particles.size();
+    const int rc = mps_snapshot_end(h, slot, &v.type, &v.pos, &v.vel, &v.p, &v.ndr, &v.bc);
+    if (rc) throw std::runtime_error("mps_snapshot_end failed with status " + std::to_string(rc));
+    return v;
 }

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "../../include/mps_amd.h"
+#include "async_export.hpp"
 #include "bucket.hpp"
 #include "particle.hpp"
 #include "settings.hpp"
@@ -39,12 +40,20 @@ public:
     mps_handle* handle() const { return h; }
     // Copies the device state into `particles`.
     void syncParticles();
+    // false: stepForward(true) leaves `particles` as they are (the caller
+    // exports from snapshots instead).
+    void setSyncOnExport(bool on) { syncOnExport = on; }
+    // Asynchronous snapshot of the export fields (mps_snapshot_begin/_end);
+    // snapshotEnd may run on another thread.
+    void snapshotBegin(int slot);
+    ExportView snapshotEnd(int slot) const;
 
 private:
     mps_handle* h = nullptr;
     mps_solver_options opt{};
     bool haveOpt = false;
     double lastCourant = 0.0;
+    bool syncOnExport = true;
 
     double importInitialCondition();
     void reportEvents(double courant);

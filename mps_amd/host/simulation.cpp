@@ -59,7 +59,16 @@ void Simulation::run(const fs::path& inputYamlPath) {
     mps = MPS(inputYamlPath);
     if (haveOpt) mps.setSolver(opt);
     time = mps.loadInitialState();
-    exportParticles(mps.particles);
+    // asynchronous export unless MPS_AMD_SYNC_EXPORT=1 (same files either way)
+    const char* se = std::getenv("MPS_AMD_SYNC_EXPORT");
+    if (!(se && std::atoi(se))) {
+        asyncExporter = std::make_unique<AsyncExporter>();
+        mps.setSyncOnExport(false);
+    }
+    if (asyncExporter)
+        exportAsync();
+    else
+        exportParticles(mps.particles);
 
     mpslog::info("START SIMULATION");
     simulationStartTime = system_clock::now();
@@ -71,13 +80,19 @@ void Simulation::run(const fs::path& inputYamlPath) {
 
         double courantNumber = mps.stepForward(isTimeToExport());
 
-        if (isTimeToExport()) exportParticles(mps.particles);
+        if (isTimeToExport()) {
+            if (asyncExporter)
+                exportAsync();
+            else
+                exportParticles(mps.particles);
+        }
 
         auto timeStepEndTime = system_clock::now();
         timeStepReport(timeStepStartTime, timeStepEndTime, courantNumber, mps.debugLogCount > initialDebugLogCount);
 
         timeStep++;
     }
+    if (asyncExporter) asyncExporter->drain();
     simulationEndTime = system_clock::now();
 
     endSimulation();
@@ -164,5 +179,20 @@ void Simulation::exportParticles(const std::vector<Particle>& particles) {
     std::snprintf(name, sizeof name, "output_%04d", outFileNum);
     exporter.toCsv(resultDirectory / "csv" / (std::string(name) + ".csv"), particles, time);
     exporter.toVtu(resultDirectory / "vtu" / (std::string(name) + ".vtu"), particles, time);
+    outFileNum++;
+}
+
+// exportParticles, from an asynchronous snapshot: slot k of the library holds
+// one snapshot until its files are written, two alternate
+void Simulation::exportAsync() {
+    const int slot = outFileNum & 1;
+    asyncExporter->waitSlot(slot);
+    mps.snapshotBegin(slot);
+    char name[64];
+    std::snprintf(name, sizeof name, "output_%04d", outFileNum);
+    MPS* m = &mps;
+    asyncExporter->submit(
+        slot, [m, slot] { return m->snapshotEnd(slot); }, resultDirectory / "csv" / (std::string(name) + ".csv"),
+        resultDirectory / "vtu" / (std::string(name) + ".vtu"), time);
     outFileNum++;
 }

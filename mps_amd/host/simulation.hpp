@@ -5,6 +5,7 @@
 #include <chrono>
 #include <filesystem>
 #include <fstream>
+#include <memory>
 #include <vector>
 
 #include "mps.hpp"
@@ -40,5 +41,9 @@ private:
                         const bool& isDebugLogAdded);
     bool isTimeToExport();
     void exportParticles(const std::vector<Particle>& particles);
+    // beyond the reference: the same files from an asynchronous device
+    // snapshot, formatted and written on a background thread
+    std::unique_ptr<AsyncExporter> asyncExporter;
+    void exportAsync();
     void endSimulation();
 };

@@ -93,3 +93,34 @@ def test_bucket_class_matches_reference_lists():
     np.testing.assert_array_equal(first, ob["first"])
     np.testing.assert_array_equal(last, ob["last"])
     np.testing.assert_array_equal(nxt, ob["next"])
+
+
+def test_async_and_sync_export_write_the_same_files(tmp_path):
+    """The CLI's default export (asynchronous snapshot, threaded formatting,
+    background writes) and the reference's synchronous path
+    (MPS_AMD_SYNC_EXPORT=1) produce byte-identical result files."""
+    csv = os.path.join(GOLD, "inputs", "dambreak_fine", "input.csv")
+    outs = {}
+    for mode in ("async", "sync"):
+        d = tmp_path / mode
+        d.mkdir()
+        yml = d / "settings.yml"
+        src = open(os.path.join(GOLD, "tutorials", "dambreak_fine", "settings.yml")).read()
+        src = src.replace("finish time: 2.0", "finish time: 0.0105")
+        src = src.replace("output interval: 0.025", "output interval: 0.002")
+        src = src.replace("inputCsvPath: inputs/dambreak_fine/input.csv", f"inputCsvPath: {csv}")
+        yml.write_text(src)
+        env = dict(os.environ)
+        env.pop("MPS_AMD_SYNC_EXPORT", None)
+        if mode == "sync":
+            env["MPS_AMD_SYNC_EXPORT"] = "1"
+        r = subprocess.run([CLI, "-s", str(yml)], cwd=d, capture_output=True, text=True, timeout=120, env=env)
+        assert r.returncode == 0, r.stdout + r.stderr
+        res = d / "result"
+        files = sorted(os.listdir(res / "csv")) + sorted(os.listdir(res / "vtu"))
+        outs[mode] = {f: ((res / "csv" / f) if f.endswith(".csv") else (res / "vtu" / f)).read_bytes()
+                      for f in files}
+    assert len(outs["async"]) == 12  # t = 0 and every 2 ms to 10 ms, CSV and VTU
+    assert outs["async"].keys() == outs["sync"].keys()
+    for f in outs["async"]:
+        assert outs["async"][f] == outs["sync"][f], f

@@ -116,3 +116,30 @@ def test_vtu_writer_reproduces_reference_file(tmp_path):
             assert len(mv) == len(rv)
         else:
             assert mv == rv
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_threaded_snapshot_writers_match_the_writers(threads, tmp_path):
+    """The asynchronous export path formats the snapshot on several threads
+    (ranges joined in order); its files must be byte-identical to the
+    Exporter's for the same particles, ghosts (skipped in the CSV rows,
+    counted in its header, kept in the VTU) and negative values included."""
+    rng = np.random.default_rng(threads)
+    n = 1001
+    t = rng.choice([-1, 0, 1, 2], size=n, p=[0.05, 0.6, 0.2, 0.15]).astype(np.int8)
+    pos = rng.normal(0, 0.3, size=(n, 3))
+    vel = rng.normal(0, 2.0, size=(n, 3))
+    p = rng.normal(0, 3000.0, size=n)
+    ndr = rng.uniform(0.5, 1.2, size=n)
+    bc = rng.integers(0, 3, size=n).astype(np.int8)
+    L = _host()
+    L.mps_host_export_both.restype = ctypes.c_int
+    L.mps_host_export_both.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 6 + [ctypes.c_double, ctypes.c_char_p,
+                                                                                ctypes.c_int]
+    arrs = [np.ascontiguousarray(a) for a in (t, pos, vel, p, ndr, bc)]
+    pre = str(tmp_path) + "/"
+    assert L.mps_host_export_both(n, *[a.ctypes.data for a in arrs], 0.125, pre.encode(), threads) == 0
+    for ext in ("csv", "vtu"):
+        a = (tmp_path / f"sync.{ext}").read_bytes()
+        b = (tmp_path / f"async.{ext}").read_bytes()
+        assert len(a) > 1000 and a == b, ext

@@ -278,6 +278,18 @@ int mps_snapshot_begin(mps_handle* h, int32_t slot);
 int mps_snapshot_end(mps_handle* h, int32_t slot, const int8_t** type, const double** pos, const double** vel,
                      const double** p, const double** ndr, const int8_t** bc);
 
+/* Lossless checkpoint (the reference restarts from its 6-decimal CSV,
+ * src/mps.cpp:38-94): the full particle state as a byte image, every field in
+ * FP64 plus types, boundary conditions and the solver's warm start, so a run
+ * resumed from it continues bit for bit as if uninterrupted.  The image is
+ * mps_checkpoint_size() bytes (a 64-byte header, then 11 doubles and 2 bytes
+ * per particle); mps_checkpoint_load accepts only an image of a handle of the
+ * same shape (particle count, dim, scenes) and validates it first.  Debug
+ * events and solver statistics are not part of the state. */
+int64_t mps_checkpoint_size(const mps_handle* h);
+int mps_checkpoint_save(mps_handle* h, void* buf, int64_t cap);
+int mps_checkpoint_load(mps_handle* h, const void* buf, int64_t size);
+
 #ifdef __cplusplus
 }
 #endif

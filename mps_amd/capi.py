@@ -32,6 +32,7 @@ EXPORTED = [
     "mps_time_pressure_kernels", "mps_set_block_sizes", "mps_autotune",
     "mps_create_ensemble", "mps_scene_count", "mps_ensemble_stats",
     "mps_adjoint_record", "mps_adjoint_backward", "mps_snapshot_begin", "mps_snapshot_end",
+    "mps_checkpoint_size", "mps_checkpoint_save", "mps_checkpoint_load",
 ]
 
 
@@ -115,8 +116,12 @@ def lib():
     L.mps_adjoint_backward.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.POINTER(SolverStats)]
     L.mps_snapshot_begin.argtypes = [vp, i32]
     L.mps_snapshot_end.argtypes = [vp, i32] + [ctypes.POINTER(vp)] * 6
+    L.mps_checkpoint_size.restype = ctypes.c_int64
+    L.mps_checkpoint_size.argtypes = [vp]
+    L.mps_checkpoint_save.argtypes = [vp, vp, ctypes.c_int64]
+    L.mps_checkpoint_load.argtypes = [vp, vp, ctypes.c_int64]
     for name in EXPORTED:
-        if name not in ("mps_last_error",):
+        if name not in ("mps_last_error", "mps_checkpoint_size"):
             getattr(L, name).restype = i32
     _lib = L
     return L
@@ -189,6 +194,17 @@ class Simulation:
         self._check(self.L.mps_adjoint_backward(self.h, _p(pb), _p(vb), _p(pos0), _p(vel0), _p(par),
                                                 ctypes.byref(st)))
         return pos0, vel0, par, st.as_dict()
+
+    def checkpoint(self) -> bytes:
+        """Lossless image of the full state (mps_checkpoint_save)."""
+        size = self.L.mps_checkpoint_size(self.h)
+        buf = ctypes.create_string_buffer(size)
+        self._check(self.L.mps_checkpoint_save(self.h, ctypes.cast(buf, ctypes.c_void_p), size))
+        return buf.raw
+
+    def restore(self, image: bytes):
+        buf = ctypes.create_string_buffer(image, len(image))
+        self._check(self.L.mps_checkpoint_load(self.h, ctypes.cast(buf, ctypes.c_void_p), len(image)))
 
     def scene_state(self, k, state=None):
         """The fields of scene k (local ids) out of a state() dict."""

@@ -1559,4 +1559,102 @@ int mps_snapshot_end(mps_handle* h, int32_t slot, const int8_t** type, const dou
     return MPS_OK;
 }
 
+// ---- lossless checkpoints ----
+// Image: 64-byte header {magic "MPSAMDCK", version 1, n, dim, nscene, 0...},
+// then the saved state (kStateDoubles x n doubles, kStateBytes x n bytes) in
+// the layout of save_state: per particle id x y z vx vy vz, warm-start
+// solution, pressure, minimum pressure, number density ratio, a ghost's
+// stale number density; then type and boundary condition.
+struct CkHeader {
+    char magic[8];
+    int32_t version, n, dim, nscene;
+    int64_t doubles, bytes;
+    char pad[24];
+};
+static_assert(sizeof(CkHeader) == 64, "checkpoint header is 64 bytes");
+
+int64_t mps_checkpoint_size(const mps_handle* h) {
+    if (!h) return -1;
+    return (int64_t) sizeof(CkHeader) + (int64_t) h->n * (kStateDoubles * (int64_t) sizeof(double) + kStateBytes);
+}
+
+static int ck_staging(mps_handle* h, double** f, int8_t** t) {
+    const size_t N = (size_t) h->n;
+    HIPCHK(h, dalloc(f, (size_t) kStateDoubles * N));
+    if (hipError_t e = dalloc(t, (size_t) kStateBytes * N)) {
+        hipFree(*f);
+        return hip_fail(h, e, "checkpoint staging");
+    }
+    return MPS_OK;
+}
+
+int mps_checkpoint_save(mps_handle* h, void* buf, int64_t cap) {
+    if (!h || !buf) return MPS_E_INVALID;
+    const int64_t need = mps_checkpoint_size(h);
+    if (cap < need) return fail(h, MPS_E_INVALID, "checkpoint buffer too small");
+    int rc = drain(h);
+    if (rc) return rc;
+    CkHeader hd{};
+    memcpy(hd.magic, "MPSAMDCK", 8);
+    hd.version = 1;
+    hd.n = h->n;
+    hd.dim = h->dim;
+    hd.nscene = h->nscene;
+    hd.doubles = (int64_t) kStateDoubles * h->n;
+    hd.bytes = (int64_t) kStateBytes * h->n;
+    memcpy(buf, &hd, sizeof hd);
+    if (h->n == 0) return MPS_OK;
+    double* f = nullptr;
+    int8_t* t = nullptr;
+    if ((rc = ck_staging(h, &f, &t))) return rc;
+    hipError_t e = save_state(h->dim, h->n, cur(h), h->cold, f, t, h->st);
+    char* out = (char*) buf + sizeof hd;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, f, (size_t) hd.doubles * sizeof(double), hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out + hd.doubles * sizeof(double), t, (size_t) hd.bytes, hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    hipFree(f);
+    hipFree(t);
+    if (e != hipSuccess) return hip_fail(h, e, "mps_checkpoint_save");
+    return MPS_OK;
+}
+
+int mps_checkpoint_load(mps_handle* h, const void* buf, int64_t size) {
+    if (!h || !buf) return MPS_E_INVALID;
+    if (size < (int64_t) sizeof(CkHeader)) return fail(h, MPS_E_INVALID, "checkpoint image too short");
+    CkHeader hd;
+    memcpy(&hd, buf, sizeof hd);
+    if (memcmp(hd.magic, "MPSAMDCK", 8) != 0 || hd.version != 1)
+        return fail(h, MPS_E_INVALID, "not a checkpoint image of this library");
+    if (hd.n != h->n || hd.dim != h->dim || hd.nscene != h->nscene)
+        return fail(h, MPS_E_INVALID, "checkpoint was taken from a handle of another shape");
+    if (size < mps_checkpoint_size(h) || hd.doubles != (int64_t) kStateDoubles * h->n ||
+        hd.bytes != (int64_t) kStateBytes * h->n)
+        return fail(h, MPS_E_INVALID, "checkpoint image truncated");
+    const int8_t* tb = (const int8_t*) buf + sizeof hd + hd.doubles * (int64_t) sizeof(double);
+    for (int64_t i = 0; i < h->n; i++) {
+        if (tb[i] < -1 || tb[i] > 2) return fail(h, MPS_E_INVALID, "checkpoint holds an invalid particle type");
+        if (tb[h->n + i] < 0 || tb[h->n + i] > 2)
+            return fail(h, MPS_E_INVALID, "checkpoint holds an invalid boundary condition");
+    }
+    int rc = drain(h);
+    if (rc) return rc;
+    if (h->n == 0) return MPS_OK;
+    double* f = nullptr;
+    int8_t* t = nullptr;
+    if ((rc = ck_staging(h, &f, &t))) return rc;
+    const char* in = (const char*) buf + sizeof hd;
+    hipError_t e = hipMemcpyAsync(f, in, (size_t) hd.doubles * sizeof(double), hipMemcpyHostToDevice, h->st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(t, in + hd.doubles * sizeof(double), (size_t) hd.bytes, hipMemcpyHostToDevice, h->st);
+    Hot c = cur(h);
+    if (e == hipSuccess) e = restore_state(h->dim, h->n, c, h->cold, h->bb.slot_of, f, t, h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    hipFree(f);
+    hipFree(t);
+    if (e != hipSuccess) return hip_fail(h, e, "mps_checkpoint_load");
+    return MPS_OK;
+}
+
 }  // extern "C"

@@ -295,3 +295,36 @@ def test_pressure_stabilisation_options(opts):
         # keep the two runs on the same state so later steps compare the same system
         st = o.state()
         g.upload(pos=st["pos"], vel=st["vel"], p=st["p"], ndr=st["ndr"])
+
+
+def test_checkpoint_resume_is_bit_identical():
+    """A run resumed from mps_checkpoint_save's image (in a fresh handle built
+    from the initial scene) continues bit for bit; a corrupted or foreign
+    image is refused."""
+    s, sc, _ = configs.workload("dambreak2d_584")
+    sc = _jitter(s, sc, 31, vamp=0.5)
+    a = capi.Simulation(s, sc)
+    a.set_solver(precision=capi.MPS_PREC_FP64, tolerance=1e-12, warm_start=True)
+    a.load_initial_state()
+    for _ in range(7):
+        a.step()
+    img = a.checkpoint()
+    for _ in range(9):
+        a.step()
+    b = capi.Simulation(s, sc)
+    b.set_solver(precision=capi.MPS_PREC_FP64, tolerance=1e-12, warm_start=True)
+    b.restore(img)
+    for _ in range(9):
+        b.step()
+    sa, sb = a.state(), b.state()
+    for k in sa:
+        assert (sa[k] == sb[k]).all(), k
+    bad = bytearray(img)
+    bad[0:8] = b"NOTACKPT"
+    with pytest.raises(capi.MpsError):
+        b.restore(bytes(bad))
+    with pytest.raises(capi.MpsError):
+        b.restore(img[:-10])
+    other = capi.Simulation(*configs.workload("dambreak2d_35k")[:2])
+    with pytest.raises(capi.MpsError):
+        other.restore(img)

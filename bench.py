@@ -92,7 +92,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="enqueue CG chunks directly (for profilers)")
     ap.add_argument("--precond", choices=("multigrid", "jacobi"), default="multigrid",
-                    help="CG preconditioner of both paths (ensembles always use jacobi)")
+                    help="CG preconditioner of both paths")
     ap.add_argument("--paths", default="fp64,mixed", help="paths to run (mixed sets value)")
     a = ap.parse_args()
     precond = capi.MPS_PRECOND_MULTIGRID if a.precond == "multigrid" else capi.MPS_PRECOND_JACOBI
@@ -176,7 +176,7 @@ def main():
             "dt": settings.dt if scene is not None else "per scene: 0.2 l0 / 3.5",
             "tolerance": TOL,
             "warm_start": True,
-            "preconditioner": a.precond if scene is not None else "jacobi",
+            "preconditioner": a.precond,
             "per_particle_value_basis": "n_fluid",
         },
         "fp64": {

@@ -94,8 +94,8 @@ typedef struct mps_solver_options {
 /* Preconditioners of the pressure CG. */
 #define MPS_PRECOND_JACOBI 0    /* diagonal (Eigen's DiagonalPreconditioner) */
 #define MPS_PRECOND_MULTIGRID 1 /* aggregation multigrid V-cycle on the bucket
-                                   cells; single-scene handles (ensembles use
-                                   JACOBI) */
+                                   cells, one hierarchy per scene; systems
+                                   under 8192 rows use JACOBI */
 
 /* Statistics of the last pressure solve. */
 typedef struct mps_solver_stats {

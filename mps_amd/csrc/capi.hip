@@ -73,6 +73,7 @@ struct mps_handle {
     int iz0 = 1;
     int nscene = 1;
     std::vector<SceneInfo> scenes;
+    std::vector<GridView> sgrids;   // each scene's bucket grid (host)
     GridView* d_g = nullptr;   // device scene table (ensembles)
     Consts* d_c = nullptr;
     int16_t* d_sid = nullptr;
@@ -971,7 +972,9 @@ static int create_impl(int32_t nscene, const mps_settings* sset, const int32_t* 
     h->opt.use_graph = 1;
     h->opt.preconditioner = MPS_PRECOND_JACOBI;
     h->A.dim = h->dim;
-    h->A.grid = h->g;
+    h->sgrids.resize((size_t) nscene);
+    for (int k = 0; k < nscene; k++) h->sgrids[k] = h->scenes[k].g;
+    h->A.scene_grids = h->sgrids.data();
     *out = h;
     return MPS_OK;
 }

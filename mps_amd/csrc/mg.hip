@@ -38,11 +38,31 @@ constexpr int kMgCoarse = 64;  // rows of the coarsest level at most
 
 static inline dim3 mg_grid(int n) { return dim3((unsigned) std::max(1, (n + kBlock - 1) / kBlock)); }
 
-__device__ __forceinline__ bool mg_done(const int* done) {
+// The cycle is skipped once every segment's CG has converged (a segment
+// still iterating needs it; converged segments ignore its output).
+__device__ __forceinline__ bool mg_done(MgDone d) {
     __shared__ int s_done;
-    if (threadIdx.x == 0) s_done = done ? *(volatile const int*) done : 0;
+    if (threadIdx.x == 0) {
+        int all = d.flag ? 1 : 0;
+        for (int i = 0; i < d.count && all; i++) all = *(volatile const int*) (d.flag + (size_t) i * d.stride);
+        s_done = all;
+    }
     __syncthreads();
     return s_done != 0;
+}
+
+// Scene of a key / a dense index: the last i with first[i] <= v (first has
+// count + 1 ascending entries).
+__device__ __forceinline__ int mg_bucket(const int* __restrict__ first, int count, int v) {
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (first[mid] <= v)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
 }
 
 template <typename IDX>
@@ -66,18 +86,22 @@ __global__ void __launch_bounds__(kBlock) k_mg_heads(int R, const int* __restric
     head[r] = (r == 0 || key_s[row_slot[r - 1]] != k) ? 1 : 0;
 }
 
-// Level 1: fine row -> aggregate, aggregate -> first fine row and its cell.
+// Level 1: fine row -> aggregate, aggregate -> first fine row, its scene and
+// its cell on the scene's grid.
 __global__ void __launch_bounds__(kBlock) k_mg_level1(int R, const int* __restrict__ head,
                                                       const int* __restrict__ hscan, const int* __restrict__ rkey,
-                                                      int key0, int* __restrict__ agg0, int* __restrict__ cst1,
-                                                      int* __restrict__ coord1) {
+                                                      const int* __restrict__ key0, int nscene,
+                                                      int* __restrict__ agg0, int* __restrict__ cst1,
+                                                      int* __restrict__ coord1, int* __restrict__ scene1) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     const int K = hscan[r] + head[r] - 1;
     agg0[r] = K;
     if (head[r]) {
+        const int sc = mg_bucket(key0, nscene, rkey[r]);
         cst1[K] = r;
-        coord1[K] = rkey[r] - key0;  // the cell's key is its place on the level-1 grid
+        coord1[K] = rkey[r] - key0[sc];  // the cell's key is its place on the scene's grid
+        scene1[K] = sc;
     }
     if (r == R - 1) cst1[K + 1] = R;
 }
@@ -143,33 +167,51 @@ __device__ __forceinline__ void mg_decode(int key, MgDims d, int& cx, int& cy, i
     cz = t / d.y;
 }
 
-__global__ void __launch_bounds__(kBlock) k_mg_mark(int n, const int* __restrict__ coord, MgDims d, MgDims dp,
+// Parent (on the next level's dense index space) of level-l row K.
+__device__ __forceinline__ int mg_parent_key(int K, const int* __restrict__ coord, const int* __restrict__ scene,
+                                             const MgDims* __restrict__ d, const MgDims* __restrict__ dp,
+                                             const int* __restrict__ goffp, int& slot) {
+    const int sc = scene[K];
+    int cx, cy, cz;
+    mg_decode(coord[K], d[sc], cx, cy, cz);
+    slot = (cx & 1) + 2 * (cy & 1) + 4 * (cz & 1);
+    const MgDims q = dp[sc];
+    return goffp[sc] + (cx >> 1) + q.x * ((cy >> 1) + q.y * (cz >> 1));
+}
+
+__global__ void __launch_bounds__(kBlock) k_mg_mark(int n, const int* __restrict__ coord,
+                                                    const int* __restrict__ scene, const MgDims* __restrict__ d,
+                                                    const MgDims* __restrict__ dp, const int* __restrict__ goffp,
                                                     int* __restrict__ flag) {
     const int K = blockIdx.x * blockDim.x + threadIdx.x;
     if (K >= n) return;
-    int cx, cy, cz;
-    mg_decode(coord[K], d, cx, cy, cz);
-    flag[(cx >> 1) + dp.x * ((cy >> 1) + dp.y * (cz >> 1))] = 1;
+    int slot;
+    flag[mg_parent_key(K, coord, scene, d, dp, goffp, slot)] = 1;
 }
 
 __global__ void __launch_bounds__(kBlock) k_mg_parents(int G, const int* __restrict__ flag,
-                                                       const int* __restrict__ pscan, int* __restrict__ coordp) {
+                                                       const int* __restrict__ pscan, const int* __restrict__ goffp,
+                                                       int nscene, int* __restrict__ coordp,
+                                                       int* __restrict__ scenep) {
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= G || !flag[P]) return;
-    coordp[pscan[P]] = P;
+    const int sc = mg_bucket(goffp, nscene, P);
+    coordp[pscan[P]] = P - goffp[sc];
+    scenep[pscan[P]] = sc;
 }
 
 // agg[K] = parent row; the parent's child slot (cx&1) + 2(cy&1) + 4(cz&1) = K.
-__global__ void __launch_bounds__(kBlock) k_mg_link(int n, const int* __restrict__ coord, MgDims d, MgDims dp,
+__global__ void __launch_bounds__(kBlock) k_mg_link(int n, const int* __restrict__ coord,
+                                                    const int* __restrict__ scene, const MgDims* __restrict__ d,
+                                                    const MgDims* __restrict__ dp, const int* __restrict__ goffp,
                                                     const int* __restrict__ pscan, int* __restrict__ agg,
                                                     int* __restrict__ child, int capp) {
     const int K = blockIdx.x * blockDim.x + threadIdx.x;
     if (K >= n) return;
-    int cx, cy, cz;
-    mg_decode(coord[K], d, cx, cy, cz);
-    const int pr = pscan[(cx >> 1) + dp.x * ((cy >> 1) + dp.y * (cz >> 1))];
+    int slot;
+    const int pr = pscan[mg_parent_key(K, coord, scene, d, dp, goffp, slot)];
     agg[K] = pr;
-    child[(size_t) ((cx & 1) + 2 * (cy & 1) + 4 * (cz & 1)) * capp + pr] = K;
+    child[(size_t) slot * capp + pr] = K;
 }
 
 template <typename T, int W, int C>
@@ -226,7 +268,7 @@ template <typename T, typename IDX>
 __global__ void __launch_bounds__(kBlock) k_mg_down0(const int* __restrict__ nrow, const int* __restrict__ slice_pos,
                                                      const IDX* __restrict__ col, const T* __restrict__ val,
                                                      const T* __restrict__ b, T* __restrict__ res, T omega,
-                                                     const int* __restrict__ done) {
+                                                     MgDone done) {
 #pragma clang fp contract(fast)
     if (mg_done(done)) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -247,7 +289,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_down0(const int* __restrict__ nro
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_mg_restrict1(const int* __restrict__ nrow, const int* __restrict__ cst1,
                                                          const T* __restrict__ res, T* __restrict__ b1,
-                                                         const int* __restrict__ done) {
+                                                         MgDone done) {
     if (mg_done(done)) return;
     const int K = blockIdx.x * blockDim.x + threadIdx.x;
     if (K >= *nrow) return;
@@ -295,7 +337,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_down(const int* __restrict__ npar
                                                     int capp, const int* __restrict__ col, const T* __restrict__ val,
                                                     const T* __restrict__ diag, const T* __restrict__ dinv, int cap,
                                                     const T* __restrict__ b, T* __restrict__ bp, T omega,
-                                                    const int* __restrict__ done) {
+                                                    MgDone done) {
     if (mg_done(done)) return;
     mg_down_item<T, W, C>((int) (blockIdx.x * blockDim.x + threadIdx.x), *nparent, child, capp, col, val, diag, dinv, cap, b,
                           bp, omega);
@@ -325,7 +367,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_up(const int* __restrict__ nrow, 
                                                   const T* __restrict__ val, const T* __restrict__ diag,
                                                   const T* __restrict__ dinv, int cap, const int* __restrict__ agg,
                                                   const T* __restrict__ b, const T* __restrict__ xc,
-                                                  T* __restrict__ x, T omega, T cf, const int* __restrict__ done) {
+                                                  T* __restrict__ x, T omega, T cf, MgDone done) {
     if (mg_done(done)) return;
     const int K = blockIdx.x * blockDim.x + threadIdx.x;
     if (K >= *nrow) return;
@@ -404,7 +446,7 @@ struct MgLdsLevel {
 template <typename T, int W, int C>
 __global__ void __launch_bounds__(kMgTailThreads) k_mg_tail(const MgLevelDesc* __restrict__ lv,
                                                             const int* __restrict__ nd, int l0, int L, T omega, T cf,
-                                                            int sweeps, const int* __restrict__ done) {
+                                                            int sweeps, MgDone done) {
 #pragma clang fp contract(fast)
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_off[kMgMaxLevels + 2];
@@ -527,7 +569,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_mg_prolong0(const int* __restrict__ nrow, const T* __restrict__ b,
                                                         const int* __restrict__ agg0, const T* __restrict__ x1,
                                                         T* __restrict__ xx, T omega, T cf,
-                                                        const int* __restrict__ done) {
+                                                        MgDone done) {
     if (mg_done(done)) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= *nrow) return;
@@ -539,7 +581,7 @@ template <typename T, typename IDX>
 __global__ void __launch_bounds__(kBlock) k_mg_up0(const int* __restrict__ nrow, const int* __restrict__ slice_pos,
                                                    const IDX* __restrict__ col, const T* __restrict__ val,
                                                    const T* __restrict__ b, const T* __restrict__ xx,
-                                                   T* __restrict__ u, T omega, const int* __restrict__ done) {
+                                                   T* __restrict__ u, T omega, MgDone done) {
 #pragma clang fp contract(fast)
     if (mg_done(done)) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -564,7 +606,13 @@ struct MgWork {
     int esize = 8;                   // bytes per value the buffers are sized for
     int n[kMgMaxLevels + 1] = {};    // rows per level (host copy of the last setup)
     int cap[kMgMaxLevels + 1] = {};
-    MgDims dims[kMgMaxLevels + 1] = {};
+    int nscene = 1;
+    std::vector<MgDims> sdims[kMgMaxLevels + 1];  // per level: each scene's grid
+    std::vector<int> goff[kMgMaxLevels + 1];      // per level: each scene's first dense index (+ total)
+    MgDims* d_sdims[kMgMaxLevels + 1] = {};
+    int* d_goff[kMgMaxLevels + 1] = {};
+    int scap = 0;                    // scenes the device tables hold
+    int* scene[kMgMaxLevels + 1] = {};  // per level: scene of each row
     int gen = 0;                     // bumped whenever a launch argument changes
     int* nd = nullptr;               // device row counts, kMgMaxLevels + 1
     int* err = nullptr;
@@ -623,8 +671,10 @@ MgWork* mg_create() {
 }
 
 static void mg_free_level(MgWork* m, int l) {
-    void* ps[] = {m->agg[l], m->coord[l], m->child[l], m->col[l], m->val[l], m->diag[l], m->dinv[l], m->b[l], m->x[l]};
+    void* ps[] = {m->agg[l],  m->coord[l], m->child[l], m->col[l], m->val[l],
+                  m->diag[l], m->dinv[l],  m->b[l],     m->x[l],   m->scene[l]};
     for (void* p : ps) hipFree(p);
+    m->scene[l] = nullptr;
     m->agg[l] = m->coord[l] = m->child[l] = m->col[l] = nullptr;
     m->val[l] = m->diag[l] = m->dinv[l] = m->b[l] = m->x[l] = nullptr;
     m->cap[l] = 0;
@@ -632,7 +682,11 @@ static void mg_free_level(MgWork* m, int l) {
 
 void mg_destroy(MgWork* m) {
     if (!m) return;
-    for (int l = 0; l <= kMgMaxLevels; l++) mg_free_level(m, l);
+    for (int l = 0; l <= kMgMaxLevels; l++) {
+        mg_free_level(m, l);
+        hipFree(m->d_sdims[l]);
+        hipFree(m->d_goff[l]);
+    }
     void* ps[] = {m->nd,  m->err,   m->cst1,  m->head,     m->hscan, m->rkey,
                   m->res0, m->xx0, m->flag, m->pscan, m->scan_tmp, m->desc};
     for (void* p : ps) hipFree(p);
@@ -657,6 +711,7 @@ static hipError_t mg_reserve_level(MgWork* m, int l, int need) {
     if ((er = mg_alloc(&m->agg[l], sizeof(int) * (size_t) cap))) return er;
     if (l >= 1) {
         if ((er = mg_alloc(&m->coord[l], sizeof(int) * (size_t) cap))) return er;
+        if ((er = mg_alloc(&m->scene[l], sizeof(int) * (size_t) cap))) return er;
         if ((er = mg_alloc(&m->child[l], sizeof(int) * C * cap))) return er;
         if ((er = mg_alloc(&m->col[l], sizeof(int) * W * cap))) return er;
         if ((er = mg_alloc(&m->val[l], e * W * cap))) return er;
@@ -696,8 +751,8 @@ static hipError_t mg_allow_tail_lds() {
 }
 
 template <typename T, typename IDX>
-static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, const GridView& g, int dim, const T* fval,
-                             const IDX* fcol, hipStream_t st) {
+static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, const T* fval, const IDX* fcol,
+                             hipStream_t st) {
     hipError_t e;
     // without the large LDS allowance the cycle cannot run: solve without it
     if ((dim == 2 ? mg_allow_tail_lds<T, 9, 4>() : mg_allow_tail_lds<T, 27, 8>()) != hipSuccess)
@@ -723,14 +778,31 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, const GridV
     }
     if ((e = mg_reserve_level(m, 0, R))) return e;
     if ((e = fill_int(m->err, 0, 1, st))) return e;
+    const int ns = A.nseg;
+    if (ns > m->scap) {
+        for (int k = 0; k <= kMgMaxLevels; k++) {
+            hipFree(m->d_sdims[k]);
+            hipFree(m->d_goff[k]);
+            m->d_sdims[k] = nullptr;
+            m->d_goff[k] = nullptr;
+        }
+        for (int k = 0; k <= kMgMaxLevels; k++)
+            if ((e = mg_alloc(&m->d_sdims[k], sizeof(MgDims) * (size_t) ns)) ||
+                (e = mg_alloc(&m->d_goff[k], sizeof(int) * (size_t) (ns + 1))))
+                return e;
+        m->scap = ns;
+        m->gen++;
+    }
+    if (m->nscene != ns) m->gen++;
+    m->nscene = ns;
     // level 1: cells
     hipLaunchKernelGGL(k_mg_heads, mg_grid(R), dim3(kBlock), 0, st, R, A.row_slot, h.key, m->head, m->rkey);
     if ((e = mg_scan(m, m->head, m->hscan, R, st))) return e;
     int n1 = 0;
     if ((e = mg_read_int(m->hscan + R, &n1, st))) return e;
     if ((e = mg_reserve_level(m, 1, n1))) return e;
-    hipLaunchKernelGGL(k_mg_level1, mg_grid(R), dim3(kBlock), 0, st, R, m->head, m->hscan, m->rkey, g.key0, m->agg[0],
-                       m->cst1, m->coord[1]);
+    hipLaunchKernelGGL(k_mg_level1, mg_grid(R), dim3(kBlock), 0, st, R, m->head, m->hscan, m->rkey, A.scene_key0, ns,
+                       m->agg[0], m->cst1, m->coord[1], m->scene[1]);
     if (dim == 2)
         hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 9>), mg_grid(n1), dim3(kBlock), 0, st, n1, m->cst1, A.slice_off,
                            fcol, fval, m->agg[0], m->cap[1], m->col[1], (T*) m->val[1], (T*) m->diag[1],
@@ -741,12 +813,31 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, const GridV
                            (T*) m->dinv[1], m->err);
     m->n[0] = R;
     m->n[1] = n1;
-    m->dims[1] = MgDims{g.nx, g.ny, g.flat ? 1 : g.nz};
+    m->sdims[1].resize(ns);
+    for (int k = 0; k < ns; k++) {
+        const GridView& gk = A.scene_grids[k];
+        m->sdims[1][k] = MgDims{gk.nx, gk.ny, gk.flat ? 1 : gk.nz};
+    }
+    if ((e = hipMemcpyAsync(m->d_sdims[1], m->sdims[1].data(), sizeof(MgDims) * (size_t) ns, hipMemcpyHostToDevice,
+                            st)))
+        return e;
+    // stop at <= 64 rows, or once every scene's grid is a single cell
+    const int coarse = std::max(kMgCoarse, ns);
     int l = 1;
-    while (m->n[l] > kMgCoarse && l < kMgMaxLevels) {
-        const MgDims d = m->dims[l];
-        const MgDims dp{(d.x + 1) / 2, (d.y + 1) / 2, (d.z + 1) / 2};
-        const int G = dp.x * dp.y * dp.z;
+    while (m->n[l] > coarse && l < kMgMaxLevels) {
+        std::vector<MgDims>& dp = m->sdims[l + 1];
+        std::vector<int>& go = m->goff[l + 1];
+        dp.resize(ns);
+        go.assign(ns + 1, 0);
+        bool shrinks = false;
+        for (int k = 0; k < ns; k++) {
+            const MgDims d = m->sdims[l][k];
+            dp[k] = MgDims{(d.x + 1) / 2, (d.y + 1) / 2, (d.z + 1) / 2};
+            shrinks = shrinks || d.x > 1 || d.y > 1 || d.z > 1;
+            go[k + 1] = go[k] + dp[k].x * dp[k].y * dp[k].z;
+        }
+        if (!shrinks) break;
+        const int G = go[ns];
         if (G + 1 > m->gcap) {
             hipFree(m->flag);
             hipFree(m->pscan);
@@ -757,16 +848,24 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, const GridV
                 return e;
             m->gcap = cap;
         }
+        if ((e = hipMemcpyAsync(m->d_sdims[l + 1], dp.data(), sizeof(MgDims) * (size_t) ns, hipMemcpyHostToDevice,
+                                st)) ||
+            (e = hipMemcpyAsync(m->d_goff[l + 1], go.data(), sizeof(int) * (size_t) (ns + 1), hipMemcpyHostToDevice,
+                                st)))
+            return e;
         if ((e = fill_int(m->flag, 0, (size_t) G, st))) return e;
-        hipLaunchKernelGGL(k_mg_mark, mg_grid(m->n[l]), dim3(kBlock), 0, st, m->n[l], m->coord[l], d, dp, m->flag);
+        hipLaunchKernelGGL(k_mg_mark, mg_grid(m->n[l]), dim3(kBlock), 0, st, m->n[l], m->coord[l], m->scene[l],
+                           m->d_sdims[l], m->d_sdims[l + 1], m->d_goff[l + 1], m->flag);
         if ((e = mg_scan(m, m->flag, m->pscan, G, st))) return e;
         int np = 0;
         if ((e = mg_read_int(m->pscan + G, &np, st))) return e;
         if ((e = mg_reserve_level(m, l + 1, np))) return e;
-        hipLaunchKernelGGL(k_mg_parents, mg_grid(G), dim3(kBlock), 0, st, G, m->flag, m->pscan, m->coord[l + 1]);
+        hipLaunchKernelGGL(k_mg_parents, mg_grid(G), dim3(kBlock), 0, st, G, m->flag, m->pscan, m->d_goff[l + 1], ns,
+                           m->coord[l + 1], m->scene[l + 1]);
         if ((e = fill_int(m->child[l + 1], -1, (size_t) m->C * m->cap[l + 1], st))) return e;
-        hipLaunchKernelGGL(k_mg_link, mg_grid(m->n[l]), dim3(kBlock), 0, st, m->n[l], m->coord[l], d, dp, m->pscan,
-                           m->agg[l], m->child[l + 1], m->cap[l + 1]);
+        hipLaunchKernelGGL(k_mg_link, mg_grid(m->n[l]), dim3(kBlock), 0, st, m->n[l], m->coord[l], m->scene[l],
+                           m->d_sdims[l], m->d_sdims[l + 1], m->d_goff[l + 1], m->pscan, m->agg[l],
+                           m->child[l + 1], m->cap[l + 1]);
         if (dim == 2)
             hipLaunchKernelGGL((k_mg_assemble<T, 9, 4>), mg_grid(np), dim3(kBlock), 0, st, np, m->child[l + 1],
                                m->cap[l + 1], m->col[l], (const T*) m->val[l], (const T*) m->diag[l], m->cap[l],
@@ -778,12 +877,11 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, const GridV
                                m->agg[l], m->col[l + 1], (T*) m->val[l + 1], (T*) m->diag[l + 1],
                                (T*) m->dinv[l + 1], m->err);
         m->n[l + 1] = np;
-        m->dims[l + 1] = dp;
         l++;
     }
     if (m->L != l) m->gen++;
     m->L = l;
-    if (m->n[l] > kMgCoarse) return hipErrorInvalidValue;  // the level cap was hit before the grid got small
+    if (m->n[l] > coarse) return hipErrorInvalidValue;  // the level cap was hit before the grids got small
     int tail = l;
     {
         const int rb = mg_tail_row_bytes(m->W, m->C, (int) sizeof(T)) + 16;
@@ -809,16 +907,16 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, const GridV
     return hipGetLastError();
 }
 
-hipError_t mg_setup(MgWork* m, const Sell& A, const Hot& h, const GridView& g, int dim, int fp32, hipStream_t st) {
+hipError_t mg_setup(MgWork* m, const Sell& A, const Hot& h, int dim, int fp32, hipStream_t st) {
     if (fp32)
-        return A.use16 ? mg_setup_t<float, short>(m, A, h, g, dim, A.val32, A.dcol, st)
-                       : mg_setup_t<float, int>(m, A, h, g, dim, A.val32, A.col, st);
-    return A.use16 ? mg_setup_t<double, short>(m, A, h, g, dim, A.val64, A.dcol, st)
-                   : mg_setup_t<double, int>(m, A, h, g, dim, A.val64, A.col, st);
+        return A.use16 ? mg_setup_t<float, short>(m, A, h, dim, A.val32, A.dcol, st)
+                       : mg_setup_t<float, int>(m, A, h, dim, A.val32, A.col, st);
+    return A.use16 ? mg_setup_t<double, short>(m, A, h, dim, A.val64, A.dcol, st)
+                   : mg_setup_t<double, int>(m, A, h, dim, A.val64, A.col, st);
 }
 
 template <typename T, typename IDX, int W, int C>
-static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval, const T* r, T* u, const int* done,
+static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval, const T* r, T* u, MgDone done,
                        hipStream_t st) {
     const T om = (T) m->omega, cf = (T) m->cf;
     const int L = m->L, lt = m->tail;
@@ -846,7 +944,7 @@ static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval,
 }
 
 template <typename T, typename IDX>
-void mg_cycle(MgWork* m, const Sell& A, const IDX* fcol, const T* fval, const T* r, T* u, const int* done, int dim,
+void mg_cycle(MgWork* m, const Sell& A, const IDX* fcol, const T* fval, const T* r, T* u, MgDone done, int dim,
               hipStream_t st) {
     if (dim == 2)
         mg_cycle_t<T, IDX, 9, 4>(m, A, fcol, fval, r, u, done, st);
@@ -854,13 +952,13 @@ void mg_cycle(MgWork* m, const Sell& A, const IDX* fcol, const T* fval, const T*
         mg_cycle_t<T, IDX, 27, 8>(m, A, fcol, fval, r, u, done, st);
 }
 
-template void mg_cycle<float, short>(MgWork*, const Sell&, const short*, const float*, const float*, float*,
-                                     const int*, int, hipStream_t);
-template void mg_cycle<float, int>(MgWork*, const Sell&, const int*, const float*, const float*, float*, const int*,
-                                   int, hipStream_t);
+template void mg_cycle<float, short>(MgWork*, const Sell&, const short*, const float*, const float*, float*, MgDone,
+                                     int, hipStream_t);
+template void mg_cycle<float, int>(MgWork*, const Sell&, const int*, const float*, const float*, float*, MgDone, int,
+                                   hipStream_t);
 template void mg_cycle<double, short>(MgWork*, const Sell&, const short*, const double*, const double*, double*,
-                                      const int*, int, hipStream_t);
-template void mg_cycle<double, int>(MgWork*, const Sell&, const int*, const double*, const double*, double*,
-                                    const int*, int, hipStream_t);
+                                      MgDone, int, hipStream_t);
+template void mg_cycle<double, int>(MgWork*, const Sell&, const int*, const double*, const double*, double*, MgDone,
+                                    int, hipStream_t);
 
 }  // namespace mps

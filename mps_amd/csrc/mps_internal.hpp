@@ -311,7 +311,7 @@ hipError_t launch_gather_type(int n, const Hot& h, int8_t* type_id, hipStream_t 
 // ppe.hip — matrix storage and CG
 struct Sell {
     int dim;             // 2 or 3
-    GridView grid;       // bucket grid of a single-scene handle (multigrid aggregation)
+    const GridView* scene_grids;  // host: each segment's bucket grid (multigrid aggregation)
     int rows;            // Inner rows
     const int* rows_dev; // Inner rows, on the device
     int nseg;            // independent segments (scenes)
@@ -349,12 +349,19 @@ void mg_destroy(MgWork* m);
 int mg_generation(const MgWork* m);
 int mg_levels(const MgWork* m);
 void mg_set_params(MgWork* m, double omega, double cf, int sweeps);
-// Builds the hierarchy for the current system; hipErrorInvalidValue when it
-// cannot (the caller then solves without it).
-hipError_t mg_setup(MgWork* m, const Sell& A, const Hot& h, const GridView& g, int dim, int fp32, hipStream_t st);
-// Enqueues u = M r (one V-cycle); kernels return at once while *done is set.
+// Builds the hierarchy for the current system (every segment its own
+// grids); hipErrorInvalidValue when it cannot (the caller then solves
+// without it).
+hipError_t mg_setup(MgWork* m, const Sell& A, const Hot& h, int dim, int fp32, hipStream_t st);
+// count flags at stride ints from flag: the cycle is skipped once all are set
+struct MgDone {
+    const int* flag;
+    int stride;
+    int count;
+};
+// Enqueues u = M r (one V-cycle).
 template <typename T, typename IDX>
-void mg_cycle(MgWork* m, const Sell& A, const IDX* fcol, const T* fval, const T* r, T* u, const int* done, int dim,
+void mg_cycle(MgWork* m, const Sell& A, const IDX* fcol, const T* fval, const T* r, T* u, MgDone done, int dim,
               hipStream_t st);
 
 struct CgWork;  // defined in ppe.hip

@@ -752,7 +752,8 @@ static void enqueue_chunk(CgWork* cw, const Sell& A, const IDX* col, const T* va
     T* u = u_vec<T>(cw);
     for (int k = 0; k < chunk; k++) {
         if (cw->mg_on) {
-            mg_cycle<T, IDX>(cw->mg, A, col, val, r, u, &cw->sc[0].done, A.dim, st);
+            const MgDone dn{&cw->sc[0].done, (int) (sizeof(CgScalars) / sizeof(int)), cw->nseg};
+            mg_cycle<T, IDX>(cw->mg, A, col, val, r, u, dn, A.dim, st);
             hipLaunchKernelGGL((k_cg_apply<T, IDX, WEIGHTED, true>), gr, bl, 0, st, sg, A.slice_off, col, val, r, u, w,
                                A.dgl, cw->part, cw->sc);
             hipLaunchKernelGGL((k_cg_step<T, WEIGHTED, true>), gr, bl, 0, st, sg, y, r, p, s, w, u, cw->part, cw->sc,
@@ -861,13 +862,13 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
     if ((e = cg_layout(cw, A, st))) return e;
     const dim3 gr(cw->total_blocks), bl(kBlock);
     const Seg sg = cw->seg();
-    // multigrid preconditioner on single-scene systems, built in the precision
-    // of the CG it serves; if the hierarchy cannot be built the solve runs
+    // multigrid preconditioner (each segment its own hierarchy), built in the
+    // precision of the CG it serves; if it cannot be built the solve runs
     // with the Jacobi scaling alone
     cw->mg_on = 0;
-    if (o.preconditioner == MPS_PRECOND_MULTIGRID && nseg == 1 && rows >= kMgMinRows) {
+    if (o.preconditioner == MPS_PRECOND_MULTIGRID && rows >= kMgMinRows) {
         if (!cw->mg) cw->mg = mg_create();
-        const hipError_t me = mg_setup(cw->mg, A, h, A.grid, A.dim, o.precision == MPS_PREC_MIXED, st);
+        const hipError_t me = mg_setup(cw->mg, A, h, A.dim, o.precision == MPS_PREC_MIXED, st);
         if (me == hipSuccess)
             cw->mg_on = 1;
         else if (me != hipErrorInvalidValue)

@@ -53,3 +53,38 @@ def test_sampler_is_reproducible_and_in_range():
     nus = [m[0].kinematic_viscosity for m in a]
     assert min(nus) >= 1e-6 and max(nus) <= 1e-3
     assert sum(1 for m in a if m[2] == "dambreak") == 32
+
+
+@pytest.mark.parametrize("precision", [capi.MPS_PREC_FP64, capi.MPS_PREC_MIXED])
+def test_multigrid_ensemble(precision):
+    """With the multigrid preconditioner every scene keeps its own coarse
+    grids (aggregates never cross scenes) and its own convergence test: each
+    member meets 1e-9 every step and matches its solo run within what the
+    tolerance leaves free (the hierarchy's depth depends on the whole
+    ensemble, so the iterates are not bit-identical)."""
+    members = configs.ensemble_members(seed=11, count=5, l0_choices=(0.004, 0.005))
+    steps = 6
+    ens = capi.Simulation(members)
+    ens.set_solver(precision=precision, tolerance=1e-9, warm_start=True, preconditioner=capi.MPS_PRECOND_MULTIGRID)
+    ens.load_initial_state()
+    for _ in range(steps):
+        ens.step()
+        stats, _ = ens.scene_stats()
+        assert all(x["converged"] == 1 and x["rel_residual"] <= 1e-9 for x in stats)
+    assert sum(x["rows"] for x in stats) >= 8192
+    jac_iters = []
+    est = ens.state()
+    for i, (s, sc, kind) in enumerate(members):
+        solo = capi.Simulation(s, sc)
+        solo.set_solver(precision=precision, tolerance=1e-9, warm_start=True)
+        solo.load_initial_state()
+        for _ in range(steps):
+            solo.step()
+        jac_iters.append(solo.solver_stats()["iterations"])
+        ss = solo.state()
+        mine = ens.scene_state(i, est)
+        assert (mine["type"] == ss["type"]).all()
+        assert np.abs(mine["pos"] - ss["pos"]).max() <= 1e-9
+        assert np.abs(mine["p"] - ss["p"]).max() <= 1e-5 * max(np.abs(ss["p"]).max(), 1.0)
+    # the hierarchy pays: the worst scene needs far fewer iterations than Jacobi
+    assert max(x["iterations"] for x in stats) * 3 < max(jac_iters)

@@ -231,11 +231,14 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
     if (rows == 0) return hipSuccess;
     const int* rows_p = w.row_of + n;
     if ((e = fill_int(A.maxdelta, 0, 1, st)) != hipSuccess) return e;
-    // one wavefront per slice; the 32-bit columns only if some offset needs them
+    // one wavefront per slice; the 32-bit columns only if some offset needs
+    // them.  The first pass writes them when the previous assembly needed
+    // them (the particle order changes little between steps); a second pass
+    // walks the neighbours again only when that guess was wrong.
     const dim3 fg((rows + 63) / 64), fb(64);
     int md = 0;
     for (int pass = 0; pass < 2; pass++) {
-        const int write_col = pass;
+        const int write_col = pass || A.wide_hint;
 #define FILL_ARGS                                                                                                  \
     g, c, tab, h, rows_p, A.row_slot, w.row_of, w.bc_s, w.diag_s, w.isq_s, A.slice_off, A.col, A.dcol, A.maxdelta, A.val64, \
         want_val32 ? A.val32 : nullptr, A.sdiag, A.dgl, (want_raw && !pass) ? A.raw_off : nullptr,                \
@@ -252,9 +255,10 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
         // 16-bit column offsets when every row's neighbours lie within +-32767 rows
         if ((e = hipMemcpyAsync(&md, A.maxdelta, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
         if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-        if (md <= 32767) break;
+        if (md <= 32767 || write_col) break;
     }
     A.use16 = md <= 32767 ? 1 : 0;
+    A.wide_hint = A.use16 ? 0 : 1;
     return hipGetLastError();
 }
 

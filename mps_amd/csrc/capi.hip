@@ -8,6 +8,7 @@
 #include <float.h>
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -330,6 +331,28 @@ static int alloc_all(mps_handle* h) {
     HIPCHK(h, dalloc(&w.coll_level, (size_t) w.coll_cap));
     HIPCHK(h, dalloc(&w.coll_last, n));
     w.scan_tmp = h->bb.scan_tmp;
+    // pressure-stage neighbour lists: room for 1.5x the neighbours a lattice
+    // of spacing l0 has within max(re_s, re_p), plus 8 (slots beyond it are
+    // walked again by the readers), at most 256 per slot and 24 GB in all
+    {
+        double est = 0.0;
+        for (const SceneInfo& sc : h->scenes) {
+            const double r = std::max(sc.c.re_s, sc.c.re_p) / sc.c.l0;
+            const double v = h->dim == 3 ? 4.0 / 3.0 * M_PI * r * r * r : M_PI * r * r;
+            est = std::max(est, v);
+        }
+        size_t cap = (size_t) (1.5 * est) + 8;
+        cap = std::min<size_t>(cap, 256);
+        const size_t budget = (size_t) 24 << 30;
+        if (n > 0 && cap * n * 12 > budget) cap = budget / (12 * n);
+        // MPS_AMD_NB_CAP overrides the capacity (0: every reader walks the cells)
+        if (const char* e = getenv("MPS_AMD_NB_CAP")) cap = std::min<size_t>((size_t) std::max(atoi(e), 0), cap);
+        w.nb.cap = (int) cap;
+        w.nb.stride = n;
+        HIPCHK(h, dalloc(&w.nb.cnt, n));
+        HIPCHK(h, dalloc(&w.nb.q, std::max<size_t>(cap * n, 1)));
+        HIPCHK(h, dalloc(&w.nb.d, std::max<size_t>(cap * n, 1)));
+    }
     HIPCHK(h, dalloc(&h->x_s, n));
     HIPCHK(h, dalloc(&h->dbg_id, n));
     HIPCHK(h, dalloc(&h->tmp3a, 3 * n));
@@ -385,7 +408,7 @@ static void free_all(mps_handle* h) {
                     h->A.slice_off, h->A.slice_w,     h->A.row_slot, h->A.sdiag,      h->A.dgl,       h->A.raw_off,
                     h->A.col,       h->A.dcol,        h->A.maxdelta, h->A.val64,       h->A.val32,    h->A.raw_col,    h->A.raw_val,   h->dev_counters,
                     h->ev.ghost_ids, h->ev.unconn_ids, h->snap_pos,  h->snap_vel,     h->d_g,         h->d_c,
-                    h->d_sid,       h->d_key0};
+                    h->d_sid,       h->d_key0,        h->w.nb.cnt,   h->w.nb.q,       h->w.nb.d};
     for (void* p : ptrs) hipFree(p);
     for (void* p : h->adj.owned) hipFree(p);
     h->adj.owned.clear();

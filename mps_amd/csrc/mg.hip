@@ -106,21 +106,103 @@ __global__ void __launch_bounds__(kBlock) k_mg_level1(int R, const int* __restri
     if (r == R - 1) cst1[K + 1] = R;
 }
 
+struct MgDims {
+    int x, y, z;
+};
+
+__device__ __forceinline__ void mg_decode(int key, MgDims d, int& cx, int& cy, int& cz) {
+    cx = key % d.x;
+    const int t = key / d.x;
+    cy = t % d.y;
+    cz = t / d.y;
+}
+
+// Stencil position of a neighbouring cell: dx + 3 dy (+ 9 dz in 3D), each
+// offset shifted to 0..2, from the key difference on a grid of d.x * d.y (*
+// d.z) cells (keys x fastest).  Needs d.x, d.y >= 3 (then the offsets are the
+// nearest multiples of the strides); -1 if dkey is no neighbour offset.
+template <int W>
+__device__ __forceinline__ int mg_stencil_of_dkey(int dkey, MgDims d) {
+    int dz = 0;
+    if (W == 27 && d.z > 1) {
+        const int nxy = d.x * d.y;
+        dz = dkey > nxy / 2 ? 1 : (dkey < -(nxy / 2) ? -1 : 0);
+        dkey -= dz * nxy;
+    }
+    const int dy = dkey > d.x / 2 ? 1 : (dkey < -(d.x / 2) ? -1 : 0);
+    const int dx = dkey - dy * d.x;
+    if (dx < -1 || dx > 1) return -1;
+    int j = (dx + 1) + 3 * (dy + 1);
+    if (W == 27) j += 9 * (dz + 1);
+    return j;
+}
+
+// The same from decoded cell coordinates (any grid size).
+template <int W>
+__device__ __forceinline__ int mg_stencil_of_cells(int k0, int k1, MgDims d) {
+    int x0, y0, z0, x1, y1, z1;
+    mg_decode(k0, d, x0, y0, z0);
+    mg_decode(k1, d, x1, y1, z1);
+    const int dx = x1 - x0, dy = y1 - y0, dz = z1 - z0;
+    if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1 || (W == 9 && dz != 0)) return -1;
+    int j = (dx + 1) + 3 * (dy + 1);
+    if (W == 27) j += 9 * (dz + 1);
+    return j;
+}
+
+// Coarse-row accumulator of one thread: the neighbours' columns and sums in
+// order of first appearance, found through their stencil positions (every
+// neighbour of an aggregate lies in the 3 x 3 (x 3) block of cells around
+// it).  Kept in LDS, indexed [slot][thread]; blocks of kMgAsmBlock threads.
+constexpr int kMgAsmBlock = 64;
+
+template <int W>
+struct MgAcc {
+    signed char (*slot)[kMgAsmBlock];
+    int (*col)[kMgAsmBlock];
+    double (*val)[kMgAsmBlock];
+    int t, m;
+    __device__ void init() {
+        for (int j = 0; j < W; j++) slot[j][t] = -1;
+        m = 0;
+    }
+    __device__ void add(int j, int c, double v) {
+        const int sl = slot[j][t];
+        if (sl >= 0) {
+            val[sl][t] += v;
+        } else {
+            slot[j][t] = (signed char) m;
+            col[m][t] = c;
+            val[m][t] = v;
+            m++;
+        }
+    }
+};
+
 // Galerkin level 1 from the fine SELL matrix (unit diagonal not stored;
-// padding entries point at their own row with value 0).
+// padding entries point at their own row with value 0).  Aggregate K is the
+// bucket cell coord1[K] of scene scene1[K]; sums per neighbour in row, entry
+// order.
 template <typename T, typename IDX, int W>
-__global__ void __launch_bounds__(kBlock) k_mg_assemble1(int n1, const int* __restrict__ cst1,
-                                                         const int* __restrict__ slice_pos,
-                                                         const IDX* __restrict__ col, const T* __restrict__ val,
-                                                         const int* __restrict__ agg0, int cap1,
-                                                         int* __restrict__ col1, T* __restrict__ val1,
-                                                         T* __restrict__ diag1, T* __restrict__ dinv1,
-                                                         int* __restrict__ err) {
+__global__ void __launch_bounds__(kMgAsmBlock) k_mg_assemble1(int n1, const int* __restrict__ cst1,
+                                                              const int* __restrict__ slice_pos,
+                                                              const IDX* __restrict__ col, const T* __restrict__ val,
+                                                              const int* __restrict__ agg0,
+                                                              const int* __restrict__ coord1,
+                                                              const int* __restrict__ scene1,
+                                                              const MgDims* __restrict__ dims, int cap1,
+                                                              int* __restrict__ col1, T* __restrict__ val1,
+                                                              T* __restrict__ diag1, T* __restrict__ dinv1,
+                                                              int* __restrict__ err) {
+    __shared__ signed char s_slot[W][kMgAsmBlock];
+    __shared__ int s_col[W][kMgAsmBlock];
+    __shared__ double s_val[W][kMgAsmBlock];
     const int K = blockIdx.x * blockDim.x + threadIdx.x;
     if (K >= n1) return;
-    int cs[W];
-    double vs[W];
-    int m = 0;
+    MgAcc<W> acc{s_slot, s_col, s_val, (int) threadIdx.x, 0};
+    acc.init();
+    const MgDims d = dims[scene1[K]];
+    const int key = coord1[K];
     double dg = 0.0;
     for (int r = cst1[K]; r < cst1[K + 1]; r++) {
         dg += 1.0;
@@ -135,36 +217,20 @@ __global__ void __launch_bounds__(kBlock) k_mg_assemble1(int n1, const int* __re
                 dg += v;
                 continue;
             }
-            int j = 0;
-            while (j < m && cs[j] != K2) j++;
-            if (j < m) {
-                vs[j] += v;
-            } else if (m < W) {
-                cs[m] = K2;
-                vs[m] = v;
-                m++;
-            } else {
+            const int j = mg_stencil_of_dkey<W>(coord1[K2] - key, d);
+            if (j < 0) {
                 atomicOr(err, 1);
+                continue;
             }
+            acc.add(j, K2, v);
         }
     }
     for (int j = 0; j < W; j++) {
-        col1[(size_t) j * cap1 + K] = j < m ? cs[j] : -1;
-        val1[(size_t) j * cap1 + K] = j < m ? (T) vs[j] : (T) 0;
+        col1[(size_t) j * cap1 + K] = j < acc.m ? s_col[j][threadIdx.x] : -1;
+        val1[(size_t) j * cap1 + K] = j < acc.m ? (T) s_val[j][threadIdx.x] : (T) 0;
     }
     diag1[K] = (T) dg;
     dinv1[K] = (T) (1.0 / dg);
-}
-
-struct MgDims {
-    int x, y, z;
-};
-
-__device__ __forceinline__ void mg_decode(int key, MgDims d, int& cx, int& cy, int& cz) {
-    cx = key % d.x;
-    const int t = key / d.x;
-    cy = t % d.y;
-    cz = t / d.y;
 }
 
 // Parent (on the next level's dense index space) of level-l row K.
@@ -214,18 +280,27 @@ __global__ void __launch_bounds__(kBlock) k_mg_link(int n, const int* __restrict
     child[(size_t) slot * capp + pr] = K;
 }
 
+// Galerkin level l + 1 from level l: parent P sums its children's rows (child
+// slots in order, then their entries in order).
 template <typename T, int W, int C>
-__global__ void __launch_bounds__(kBlock) k_mg_assemble(int np, const int* __restrict__ child, int capp,
-                                                        const int* __restrict__ col, const T* __restrict__ val,
-                                                        const T* __restrict__ diag, int cap,
-                                                        const int* __restrict__ agg, int* __restrict__ colp,
-                                                        T* __restrict__ valp, T* __restrict__ diagp,
-                                                        T* __restrict__ dinvp, int* __restrict__ err) {
+__global__ void __launch_bounds__(kMgAsmBlock) k_mg_assemble(int np, const int* __restrict__ child, int capp,
+                                                             const int* __restrict__ col, const T* __restrict__ val,
+                                                             const T* __restrict__ diag, int cap,
+                                                             const int* __restrict__ agg,
+                                                             const int* __restrict__ coordp,
+                                                             const int* __restrict__ scenep,
+                                                             const MgDims* __restrict__ dp, int* __restrict__ colp,
+                                                             T* __restrict__ valp, T* __restrict__ diagp,
+                                                             T* __restrict__ dinvp, int* __restrict__ err) {
+    __shared__ signed char s_slot[W][kMgAsmBlock];
+    __shared__ int s_col[W][kMgAsmBlock];
+    __shared__ double s_val[W][kMgAsmBlock];
     const int P = blockIdx.x * blockDim.x + threadIdx.x;
     if (P >= np) return;
-    int cs[W];
-    double vs[W];
-    int m = 0;
+    MgAcc<W> acc{s_slot, s_col, s_val, (int) threadIdx.x, 0};
+    acc.init();
+    const MgDims d = dp[scenep[P]];
+    const int key = coordp[P];
     double dg = 0.0;
     for (int q = 0; q < C; q++) {
         const int K = child[(size_t) q * capp + P];
@@ -240,22 +315,17 @@ __global__ void __launch_bounds__(kBlock) k_mg_assemble(int np, const int* __res
                 dg += v;
                 continue;
             }
-            int i = 0;
-            while (i < m && cs[i] != P2) i++;
-            if (i < m) {
-                vs[i] += v;
-            } else if (m < W) {
-                cs[m] = P2;
-                vs[m] = v;
-                m++;
-            } else {
+            const int sj = mg_stencil_of_cells<W>(key, coordp[P2], d);
+            if (sj < 0) {
                 atomicOr(err, 1);
+                continue;
             }
+            acc.add(sj, P2, v);
         }
     }
     for (int j = 0; j < W; j++) {
-        colp[(size_t) j * capp + P] = j < m ? cs[j] : -1;
-        valp[(size_t) j * capp + P] = j < m ? (T) vs[j] : (T) 0;
+        colp[(size_t) j * capp + P] = j < acc.m ? s_col[j][threadIdx.x] : -1;
+        valp[(size_t) j * capp + P] = j < acc.m ? (T) s_val[j][threadIdx.x] : (T) 0;
     }
     diagp[P] = (T) dg;
     dinvp[P] = (T) (1.0 / dg);
@@ -803,24 +873,26 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
     if ((e = mg_reserve_level(m, 1, n1))) return e;
     hipLaunchKernelGGL(k_mg_level1, mg_grid(R), dim3(kBlock), 0, st, R, m->head, m->hscan, m->rkey, A.scene_key0, ns,
                        m->agg[0], m->cst1, m->coord[1], m->scene[1]);
-    if (dim == 2)
-        hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 9>), mg_grid(n1), dim3(kBlock), 0, st, n1, m->cst1, A.slice_off,
-                           fcol, fval, m->agg[0], m->cap[1], m->col[1], (T*) m->val[1], (T*) m->diag[1],
-                           (T*) m->dinv[1], m->err);
-    else
-        hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 27>), mg_grid(n1), dim3(kBlock), 0, st, n1, m->cst1, A.slice_off,
-                           fcol, fval, m->agg[0], m->cap[1], m->col[1], (T*) m->val[1], (T*) m->diag[1],
-                           (T*) m->dinv[1], m->err);
     m->n[0] = R;
     m->n[1] = n1;
     m->sdims[1].resize(ns);
     for (int k = 0; k < ns; k++) {
         const GridView& gk = A.scene_grids[k];
         m->sdims[1][k] = MgDims{gk.nx, gk.ny, gk.flat ? 1 : gk.nz};
+        if (gk.nx < 3 || gk.ny < 3 || (!gk.flat && dim == 3 && gk.nz < 3)) return hipErrorInvalidValue;
     }
     if ((e = hipMemcpyAsync(m->d_sdims[1], m->sdims[1].data(), sizeof(MgDims) * (size_t) ns, hipMemcpyHostToDevice,
                             st)))
         return e;
+    const dim3 ab(kMgAsmBlock);
+    if (dim == 2)
+        hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 9>), dim3((n1 + kMgAsmBlock - 1) / kMgAsmBlock), ab, 0, st, n1,
+                           m->cst1, A.slice_off, fcol, fval, m->agg[0], m->coord[1], m->scene[1], m->d_sdims[1],
+                           m->cap[1], m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1], m->err);
+    else
+        hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 27>), dim3((n1 + kMgAsmBlock - 1) / kMgAsmBlock), ab, 0, st, n1,
+                           m->cst1, A.slice_off, fcol, fval, m->agg[0], m->coord[1], m->scene[1], m->d_sdims[1],
+                           m->cap[1], m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1], m->err);
     // stop at <= 64 rows, or once every scene's grid is a single cell
     const int coarse = std::max(kMgCoarse, ns);
     int l = 1;
@@ -866,16 +938,17 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
         hipLaunchKernelGGL(k_mg_link, mg_grid(m->n[l]), dim3(kBlock), 0, st, m->n[l], m->coord[l], m->scene[l],
                            m->d_sdims[l], m->d_sdims[l + 1], m->d_goff[l + 1], m->pscan, m->agg[l],
                            m->child[l + 1], m->cap[l + 1]);
+        const dim3 pg((np + kMgAsmBlock - 1) / kMgAsmBlock);
         if (dim == 2)
-            hipLaunchKernelGGL((k_mg_assemble<T, 9, 4>), mg_grid(np), dim3(kBlock), 0, st, np, m->child[l + 1],
-                               m->cap[l + 1], m->col[l], (const T*) m->val[l], (const T*) m->diag[l], m->cap[l],
-                               m->agg[l], m->col[l + 1], (T*) m->val[l + 1], (T*) m->diag[l + 1],
-                               (T*) m->dinv[l + 1], m->err);
+            hipLaunchKernelGGL((k_mg_assemble<T, 9, 4>), pg, ab, 0, st, np, m->child[l + 1], m->cap[l + 1], m->col[l],
+                               (const T*) m->val[l], (const T*) m->diag[l], m->cap[l], m->agg[l], m->coord[l + 1],
+                               m->scene[l + 1], m->d_sdims[l + 1], m->col[l + 1], (T*) m->val[l + 1],
+                               (T*) m->diag[l + 1], (T*) m->dinv[l + 1], m->err);
         else
-            hipLaunchKernelGGL((k_mg_assemble<T, 27, 8>), mg_grid(np), dim3(kBlock), 0, st, np, m->child[l + 1],
-                               m->cap[l + 1], m->col[l], (const T*) m->val[l], (const T*) m->diag[l], m->cap[l],
-                               m->agg[l], m->col[l + 1], (T*) m->val[l + 1], (T*) m->diag[l + 1],
-                               (T*) m->dinv[l + 1], m->err);
+            hipLaunchKernelGGL((k_mg_assemble<T, 27, 8>), pg, ab, 0, st, np, m->child[l + 1], m->cap[l + 1], m->col[l],
+                               (const T*) m->val[l], (const T*) m->diag[l], m->cap[l], m->agg[l], m->coord[l + 1],
+                               m->scene[l + 1], m->d_sdims[l + 1], m->col[l + 1], (T*) m->val[l + 1],
+                               (T*) m->diag[l + 1], (T*) m->dinv[l + 1], m->err);
         m->n[l + 1] = np;
         l++;
     }

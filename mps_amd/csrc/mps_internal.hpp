@@ -136,6 +136,47 @@ __device__ __forceinline__ double wfun(double dist, double re) {
     return w;
 }
 
+// Candidates q in [b, e) of one cell, in ascending order.  Positions are
+// loaded kNbBatch at a time (indices clamped to e - 1, so every load stays in
+// the cell), so one loop trip has kNbBatch independent loads in flight
+// instead of one; the visit order and the arithmetic are those of a plain
+// q = b .. e-1 loop.  d2 > re2 skips the square root (see for_neighbors).
+constexpr int kNbBatch = 4;
+
+template <int DIM, typename F>
+__device__ __forceinline__ void scan_cell(const double* __restrict__ px, const double* __restrict__ py,
+                                          const double* __restrict__ pz, int b, int e, int s, double xi,
+                                          double yi, double zi, double re, double re2, F& f) {
+    for (int q0 = b; q0 < e; q0 += kNbBatch) {
+        double x[kNbBatch], y[kNbBatch], z[kNbBatch];
+#pragma unroll
+        for (int i = 0; i < kNbBatch; i++) {
+            const int q = q0 + i < e ? q0 + i : e - 1;
+            x[i] = px[q];
+            y[i] = py[q];
+            if constexpr (DIM == 3) z[i] = pz[q];
+        }
+#pragma unroll
+        for (int i = 0; i < kNbBatch; i++) {
+            const int q = q0 + i;
+            if (q >= e) break;
+            const double dx = x[i] - xi;
+            const double dy = y[i] - yi;
+            double dz, d2;
+            if constexpr (DIM == 3) {
+                dz = z[i] - zi;
+                d2 = (dx * dx + dy * dy) + dz * dz;
+            } else {
+                dz = 0.0;
+                d2 = dx * dx + dy * dy;
+            }
+            if (d2 > re2) continue;
+            const double dist = sqrt(d2);
+            if (q != s && dist < re) f(q, dist, dx, dy, dz);
+        }
+    }
+}
+
 // Visits the neighbours of slot s in the reference's order (src/mps.cpp:599-613):
 // jx outer, jy, jz inner, ascending id within a cell.  Calls f(q, dist, dx, dy, dz)
 // for every slot q != s with dist < re, dist = sqrt((dx*dx+dy*dy)+dz*dz) and
@@ -170,17 +211,7 @@ __device__ __forceinline__ void for_neighbors(const GridView& g, const double* _
             }
         }
 #pragma unroll
-        for (int c = 0; c < 9; c++) {
-            const int e = ce[c];
-            for (int q = cb[c]; q < e; q++) {
-                const double dx = px[q] - xi;
-                const double dy = py[q] - yi;
-                const double d2 = dx * dx + dy * dy;
-                if (d2 > re2) continue;
-                const double dist = sqrt(d2);
-                if (q != s && dist < re) f(q, dist, dx, dy, 0.0);
-            }
-        }
+        for (int c = 0; c < 9; c++) scan_cell<2>(px, py, pz, cb[c], ce[c], s, xi, yi, 0.0, re, re2, f);
     } else {
         const double re2 = re * re * (1.0 + 1e-12);  // see the 2D branch
         int ix, iy, iz;
@@ -204,19 +235,49 @@ __device__ __forceinline__ void for_neighbors(const GridView& g, const double* _
                         continue;
                     }
                     const int k = g.flat ? (jx + jy * g.nx) : (jx + jy * g.nx + jz * g.nx * g.ny);
-                    const int e = cs[k + 1];
-                    for (int q = cs[k]; q < e; q++) {
-                        const double dx = px[q] - xi;
-                        const double dy = py[q] - yi;
-                        const double dz = pz[q] - zi;
-                        const double d2 = (dx * dx + dy * dy) + dz * dz;
-                        if (d2 > re2) continue;
-                        const double dist = sqrt(d2);
-                        if (q != s && dist < re) f(q, dist, dx, dy, dz);
-                    }
+                    scan_cell<3>(px, py, pz, cs[k], cs[k + 1], s, xi, yi, zi, re, re2, f);
                 }
             }
         }
+    }
+}
+
+// Neighbour lists of the pressure stages.  The boundary-condition pass walks
+// the cells once at radius max(re_s, re_p) and records each slot's neighbours
+// in the walk's (the reference's) order with their distances; the later walks
+// over the same bucket build (diagonal and connectivity, matrix fill, minimum
+// pressure, gradient, higher-order source) read the list instead of scanning
+// the cells again.  Entry k of slot s sits at [k * stride + s], so the lanes
+// of a wavefront read entry k together.  cnt[s] counts every neighbour; a
+// slot with more than cap neighbours is walked again by its readers.
+struct NbList {
+    int* q;
+    double* d;
+    int* cnt;
+    int cap;
+    size_t stride;
+};
+
+// Calls f(q, dist, dx, dy, dz) for slot s's neighbours with dist < re (re at
+// most the list's radius), in for_neighbors' order and with its values.
+template <int DIM, typename F>
+__device__ __forceinline__ void for_listed(const NbList& L, const GridView& g, const double* __restrict__ px,
+                                           const double* __restrict__ py, const double* __restrict__ pz, int key,
+                                           int s, double xi, double yi, double zi, double re, F&& f) {
+    const int m = L.cnt[s];
+    if (m > L.cap) {
+        for_neighbors<DIM>(g, px, py, pz, key, s, xi, yi, zi, re, f);
+        return;
+    }
+    for (int k = 0; k < m; k++) {
+        const size_t e = (size_t) k * L.stride + s;
+        const int q = L.q[e];
+        const double dist = L.d[e];
+        if (!(dist < re)) continue;
+        const double dx = px[q] - xi;
+        const double dy = py[q] - yi;
+        const double dz = DIM == 3 ? pz[q] - zi : 0.0;
+        f(q, dist, dx, dy, dz);
     }
 }
 
@@ -276,6 +337,7 @@ struct StepScratch {
     int* coll_last;     // n, zero between steps
     int coll_cap;
     int* scan_tmp;
+    NbList nb;          // pressure-stage neighbour lists (written by the BC pass)
 };
 
 // Record of one step's collision pass for the reverse sweep (every pointer

@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneT
                                                   double* __restrict__ val64, float* __restrict__ val32,
                                                   double* __restrict__ sdiag, double* __restrict__ dgl,
                                                   const int* __restrict__ raw_off, int* __restrict__ raw_col,
-                                                  double* __restrict__ raw_val, int write_col) {
+                                                  double* __restrict__ raw_val, int write_col, NbList nb) {
     __shared__ double s_val[kFillStage][64];
     __shared__ int s_dl[kFillStage][64];
     const int lane = threadIdx.x;
@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneT
         int ro = raw_off ? raw_off[s] : 0;
         const double re = c.re_p;
         const double zi = DIM == 3 ? h.pz[s] : 0.0;
-        for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
+        for_listed<DIM>(nb, g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
                            [&](int q, double dist, double, double, double) {
                                if (bc_s[q] != MPS_BC_INNER) return;
                                // the off-diagonal value of src/mps.cpp:343-348
@@ -242,7 +242,7 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
 #define FILL_ARGS                                                                                                  \
     g, c, tab, h, rows_p, A.row_slot, w.row_of, w.bc_s, w.diag_s, w.isq_s, A.slice_off, A.col, A.dcol, A.maxdelta, A.val64, \
         want_val32 ? A.val32 : nullptr, A.sdiag, A.dgl, (want_raw && !pass) ? A.raw_off : nullptr,                \
-        (want_raw && !pass) ? A.raw_col : nullptr, (want_raw && !pass) ? A.raw_val : nullptr, write_col
+        (want_raw && !pass) ? A.raw_col : nullptr, (want_raw && !pass) ? A.raw_val : nullptr, write_col, w.nb
         if (dim == 2 && tab.g)
             hipLaunchKernelGGL((k_fill_sell<2, true>), fg, fb, 0, st, FILL_ARGS);
         else if (dim == 2)

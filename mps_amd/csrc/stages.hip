@@ -312,7 +312,7 @@ hipError_t launch_collision(int dim, const GridView& g, const SceneTab& tab, int
 template <int DIM, bool MULTI>
 __global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g0, Consts c0, SceneTab tab, int n, Hot h, Cold cold,
                                                       int8_t* __restrict__ bc_s, double* __restrict__ b_s,
-                                                      int* __restrict__ parent) {
+                                                      int* __restrict__ parent, NbList nb) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     GridView g = g0;
@@ -322,6 +322,7 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g0, Consts
     int8_t bc = MPS_BC_IGNORED;
     double b = 0.0;
     int m = s;
+    int cnt = 0;
     if (t != MPS_GHOST) {
         const double rs = c.re_s, rp = c.re_p;
         const double re = rs > rp ? rs : rp;
@@ -329,6 +330,12 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g0, Consts
         const double zi = DIM == 3 ? h.pz[s] : 0.0;
         for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
                            [&](int q, double dist, double, double, double) {
+                               if (cnt < nb.cap) {
+                                   const size_t e = (size_t) cnt * nb.stride + s;
+                                   nb.q[e] = q;
+                                   nb.d[e] = dist;
+                               }
+                               cnt++;
                                if (dist < rs) ns += wfun(dist, rs);
                                if (dist < rp) {
                                    np += wfun(dist, rp);
@@ -343,13 +350,15 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g0, Consts
     bc_s[s] = bc;
     b_s[s] = b;
     parent[s] = m;
+    nb.cnt[s] = cnt;
     cold.bc[h.id[s]] = bc;
 }
 
 // Higher-order source term (src/mps.cpp:288-308); needs every slot's BC.
 template <int DIM, bool MULTI>
 __global__ void __launch_bounds__(kMaxTuneBlock) k_source_hos(GridView g0, Consts c0, SceneTab tab, int n, Hot h, Cold cold,
-                                                       const int8_t* __restrict__ bc_s, double* __restrict__ b_s) {
+                                                       const int8_t* __restrict__ bc_s, double* __restrict__ b_s,
+                                                       NbList nb) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     GridView g = g0;
@@ -360,7 +369,8 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_source_hos(GridView g0, Const
     const double re = c.re_p;
     const double xi = h.px[s], yi = h.py[s], zi = DIM == 3 ? h.pz[s] : 0.0;
     const double vxi = h.vx[s], vyi = h.vy[s], vzi = DIM == 3 ? h.vz[s] : 0.0;
-    for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, xi, yi, zi, c.re_max,
+    // neighbours beyond re_p add nothing, so the list (radius >= re_p) covers the walk
+    for_listed<DIM>(nb, g, h.px, h.py, h.pz, h.key[s], s, xi, yi, zi, re,
                        [&](int q, double dist, double rx, double ry, double rz) {
                            if (bc_s[q] == MPS_BC_IGNORED) return;
                            const double ux = h.vx[q] - vxi, uy = h.vy[q] - vyi;
@@ -373,8 +383,8 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_source_hos(GridView g0, Const
 hipError_t launch_bc_source(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
                             Cold& cold, StepScratch& w, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
-    LAUNCH_DM(k_bc_source, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, cold, w.bc_s, w.b_s, w.parent);
-    if (c.hos_on) LAUNCH_DM(k_source_hos, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, cold, w.bc_s, w.b_s);
+    LAUNCH_DM(k_bc_source, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, cold, w.bc_s, w.b_s, w.parent, w.nb);
+    if (c.hos_on) LAUNCH_DM(k_source_hos, grid_for(n, bs), dim3(bs), st, g, c, tab, n, h, cold, w.bc_s, w.b_s, w.nb);
     return hipGetLastError();
 }
 
@@ -405,7 +415,7 @@ template <int DIM, bool MULTI>
 __global__ void __launch_bounds__(kMaxTuneBlock) k_diag_join(GridView g0, Consts c0, SceneTab tab, int n, Hot h,
                                                              const int8_t* __restrict__ bc_s,
                                                              double* __restrict__ diag, int* __restrict__ nnz,
-                                                             int* parent) {
+                                                             int* parent, NbList nb) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     GridView g = g0;
@@ -418,7 +428,7 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_diag_join(GridView g0, Consts
         const double re = c.re_p;
         const double zi = DIM == 3 ? h.pz[s] : 0.0;
         volatile int* vp = parent;
-        for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
+        for_listed<DIM>(nb, g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
                            [&](int q, double dist, double, double, double) {
                                if (inner) {
                                    const int8_t bq = bc_s[q];
@@ -483,7 +493,7 @@ hipError_t launch_diag_cc(int dim, const GridView& g, const SceneTab& tab, int n
                           StepScratch& w, Events& ev, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
     const dim3 gr = grid_for(n, bs), bl(bs);
-    LAUNCH_DM(k_diag_join, gr, bl, st, g, c, tab, n, h, w.bc_s, w.diag_s, w.nnz_s, w.parent);
+    LAUNCH_DM(k_diag_join, gr, bl, st, g, c, tab, n, h, w.bc_s, w.diag_s, w.nnz_s, w.parent, w.nb);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // surf[] is a byte array: clear it with the int fill over its rounded-up size
@@ -528,7 +538,7 @@ template <int DIM, bool MULTI>
 __global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g0, Consts c0, SceneTab tab, int n, Hot cur, Hot out, Cold cold,
                                                       const int8_t* __restrict__ bc_s,
                                                       const double* __restrict__ p_s,
-                                                      double* __restrict__ pmin_s, Events ev) {
+                                                      double* __restrict__ pmin_s, Events ev, NbList nb) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     GridView g = g0;
@@ -543,7 +553,7 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g0, Consts
         pmi = cold.pmin[id];
     } else {
         pmi = p_s[s];
-        for_neighbors<DIM>(g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, c.re_p,
+        for_listed<DIM>(nb, g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, c.re_p,
                            [&](int q, double, double, double, double) {
                                if (cur.type[q] == MPS_DUMMY_WALL) return;
                                if (pmi > p_s[q]) pmi = p_s[q];
@@ -554,7 +564,7 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g0, Consts
     if (t == MPS_FLUID) {
         const double re = c.re_p;
         double gx = 0.0, gy = 0.0, gz = 0.0;
-        for_neighbors<DIM>(g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, re,
+        for_listed<DIM>(nb, g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, re,
                            [&](int q, double dist, double dx, double dy, double dz) {
                                if (cur.type[q] == MPS_DUMMY_WALL) return;
                                const double dp = p_s[q] - pmi;
@@ -594,7 +604,8 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g0, Consts
 hipError_t launch_grad_move(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& cur,
                             Hot& out, Cold& cold, StepScratch& w, Events& ev, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
-    LAUNCH_DM(k_grad_move, grid_for(n, bs), dim3(bs), st, g, c, tab, n, cur, out, cold, w.bc_s, w.p_s, w.pmin_s, ev);
+    LAUNCH_DM(k_grad_move, grid_for(n, bs), dim3(bs), st, g, c, tab, n, cur, out, cold, w.bc_s, w.p_s, w.pmin_s, ev,
+              w.nb);
     return hipGetLastError();
 }
 

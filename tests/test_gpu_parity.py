@@ -229,6 +229,42 @@ def test_3d_one_step():
     assert np.linalg.norm(gp["x"] - op["x"]) <= P_RTOL * np.linalg.norm(op["x"])
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_neighbour_list_capacity_changes_nothing(dim, monkeypatch):
+    """The pressure stages read the neighbour lists the BC pass records;
+    slots beyond the list capacity are walked again.  Capacity 0 (every slot
+    walked), 5 (most slots walked) and the default (none walked) must give the
+    same bits, with the pressure stabilisation options on as well."""
+    if dim == 2:
+        s, sc, _ = configs.workload("dambreak2d_584")
+        s = dataclasses.replace(s, hos_on=True)
+        sc = _jitter(s, sc, 21, vamp=0.3)
+    else:
+        s = configs.dambreak_settings(0.02, 1e-3, 1.0, dim=3)
+        sc = configs.lattice_3d(0.02, (0.3, 0.3, 0.1), (0.0, 0.1), (0.0, 0.2), (0.0, 0.1), 1, 2)
+        s.domain_x, s.domain_y, s.domain_z = (-0.1, 0.4), (-0.1, 0.4), (-0.1, 0.2)
+        sc = _jitter(s, sc, 22, amp=0.1, vamp=0.3)
+    res = []
+    for cap in (None, "5", "0"):
+        if cap is None:
+            monkeypatch.delenv("MPS_AMD_NB_CAP", raising=False)
+        else:
+            monkeypatch.setenv("MPS_AMD_NB_CAP", cap)
+        g = capi.Simulation(s, sc)
+        g.set_solver(precision=capi.MPS_PREC_FP64, tolerance=1e-12, warm_start=True)
+        g.load_initial_state()
+        for _ in range(6):
+            g.step()
+        res.append((g.state(), g.debug_events()))
+        g.close()
+    monkeypatch.delenv("MPS_AMD_NB_CAP", raising=False)
+    for st, ev in res[1:]:
+        for k in ("type", "pos", "vel", "p", "ndr", "bc", "pmin"):
+            assert (st[k] == res[0][0][k]).all(), k
+        for a, b in zip(ev, res[0][1]):
+            assert (np.asarray(a) == np.asarray(b)).all()
+
+
 def test_graph_and_stream_paths_identical():
     s, sc, _ = configs.workload("dambreak2d_584")
     res = []

@@ -157,37 +157,67 @@ hipError_t fill_double(double* p, double v, size_t n, hipStream_t st) {
 
 // ---------------- build ----------------
 
+// Runs of equal keys along a wavefront's lanes (slots are sorted by the
+// previous build's key, so most neighbouring lanes share one): the run's
+// first lane and, for that lane, the run's length.  One atomic per run
+// replaces one per particle.  Every lane of the wavefront must call it.
+__device__ __forceinline__ void wave_key_run(int key, bool act, int& head_lane, int& len) {
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(key, 1, 64);
+    const int prev_act = __shfl_up(act ? 1 : 0, 1, 64);
+    const bool head = act && (lane == 0 || !prev_act || prev != key);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long ends = heads | ~__ballot(act);
+    const unsigned long long upto = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1);
+    const unsigned long long hb = heads & upto;
+    head_lane = hb ? 63 - __clzll((long long) hb) : lane;
+    const unsigned long long after = lane == 63 ? 0ull : ends >> (lane + 1);
+    len = after ? __ffsll((long long) after) : 64 - lane;
+}
+
 template <int DIM, bool MULTI>
 __global__ void __launch_bounds__(kMaxTuneBlock) k_key_count(GridView g0, SceneTab tab, int n, Hot cur,
                                                              int* __restrict__ cnt, int* __restrict__ err) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    GridView g = g0;
-    Consts c;
-    scene_of<MULTI>(tab, cur.id[s], g, c);
-    int key = g.ghost_key;
-    if (cur.type[s] != MPS_GHOST) {
-        int ix, iy, iz = 1;
-        bool ok = cell_coord(cur.px[s], g.xmin, g.len, g.nx, ix) &&
-                  cell_coord(cur.py[s], g.ymin, g.len, g.ny, iy);
-        if (DIM == 3) ok = ok && cell_coord(cur.pz[s], g.zmin, g.len, g.nz, iz);
-        if (ok && g.flat && iz != 1) ok = false;
-        if (ok) {
-            key = g.key0 + (g.flat ? ix + iy * g.nx : ix + iy * g.nx + iz * g.nx * g.ny);
-        } else {
-            atomicOr(err, kErrRange);
+    const bool act = s < n;
+    int key = -1;
+    if (act) {
+        GridView g = g0;
+        Consts c;
+        scene_of<MULTI>(tab, cur.id[s], g, c);
+        key = g.ghost_key;
+        if (cur.type[s] != MPS_GHOST) {
+            int ix, iy, iz = 1;
+            bool ok = cell_coord(cur.px[s], g.xmin, g.len, g.nx, ix) &&
+                      cell_coord(cur.py[s], g.ymin, g.len, g.ny, iy);
+            if (DIM == 3) ok = ok && cell_coord(cur.pz[s], g.zmin, g.len, g.nz, iz);
+            if (ok && g.flat && iz != 1) ok = false;
+            if (ok) {
+                key = g.key0 + (g.flat ? ix + iy * g.nx : ix + iy * g.nx + iz * g.nx * g.ny);
+            } else {
+                atomicOr(err, kErrRange);
+            }
         }
+        cur.key[s] = key;
     }
-    cur.key[s] = key;
-    atomicAdd(&cnt[key], 1);
+    int head, len;
+    wave_key_run(key, act, head, len);
+    if (act && head == (int) (threadIdx.x & 63)) atomicAdd(&cnt[key], len);
 }
 
 __global__ void __launch_bounds__(kMaxTuneBlock) k_bin(int n, Hot cur, int* __restrict__ cnt, const int* __restrict__ cs,
                                                 int* __restrict__ bin_id, int* __restrict__ dst) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const int key = cur.key[s];
-    const int pos = atomicSub(&cnt[key], 1) - 1;  // leaves cnt at zero for the next build
+    const bool act = s < n;
+    const int key = act ? cur.key[s] : -1;
+    int head, len;
+    wave_key_run(key, act, head, len);
+    const int lane = threadIdx.x & 63;
+    int top = 0;
+    if (act && head == lane) top = atomicSub(&cnt[key], len);  // leaves cnt at zero for the next build
+    top = __shfl(top, head, 64);
+    if (!act) return;
+    const int pos = top - 1 - (lane - head);
     const int d = cs[key] + pos;
     bin_id[d] = cur.id[s];
     dst[s] = d;

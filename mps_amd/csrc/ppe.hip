@@ -58,11 +58,34 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 __global__ void __launch_bounds__(kMaxTuneBlock) k_rows(int n, const int* __restrict__ inner_flag,
                                                  const int* __restrict__ row_of, const int* __restrict__ nnz_s,
                                                  int* __restrict__ row_slot, int* __restrict__ slice_w) {
+    // The rows of one wavefront's slots are consecutive, so they lie in at
+    // most two slices: the widths are reduced across the wavefront and set
+    // with at most two atomics.  Every lane reaches the shuffles.
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n || !inner_flag[s]) return;
-    const int r = row_of[s];
-    row_slot[r] = s;
-    atomicMax(&slice_w[r >> 6], nnz_s[s]);
+    const bool act = s < n && inner_flag[s];
+    int r = 0, v = 0, sl = 0x7fffffff;
+    if (act) {
+        r = row_of[s];
+        row_slot[r] = s;
+        sl = r >> 6;
+        v = nnz_s[s];
+    }
+    int smin = sl;
+    for (int o = 32; o > 0; o >>= 1) {
+        const int u = __shfl_xor(smin, o, 64);
+        smin = u < smin ? u : smin;
+    }
+    if (smin == 0x7fffffff) return;  // no Inner slot in this wavefront (uniform)
+    int v0 = act && sl == smin ? v : -1, v1 = act && sl != smin ? v : -1;
+    for (int o = 32; o > 0; o >>= 1) {
+        const int a = __shfl_xor(v0, o, 64), b = __shfl_xor(v1, o, 64);
+        v0 = a > v0 ? a : v0;
+        v1 = b > v1 ? b : v1;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&slice_w[smin], v0);
+        if (v1 >= 0) atomicMax(&slice_w[smin + 1], v1);
+    }
 }
 
 // SELL assembly: one wavefront per block and per slice.  A row's first

@@ -6,6 +6,8 @@
 // the stages up to the pressure solve reproduce the reference bit for bit.
 #pragma clang fp contract(off)
 
+#include <algorithm>
+
 #include "mps_internal.hpp"
 
 namespace mps {
@@ -644,17 +646,26 @@ template <bool MULTI>
 __global__ void __launch_bounds__(kBlock) k_vmax(int n, Hot h, int dim3d, const int16_t* __restrict__ sid,
                                                  unsigned long long* __restrict__ out) {
     __shared__ double red[kBlock / 64];
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    // grid-stride: a few hundred blocks, so a few hundred atomics on the result
     double v = 0.0;
-    int si = 0;
-    if (s < n && h.type[s] == MPS_FLUID) {
+    int si = -1;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        if (h.type[s] != MPS_FLUID) continue;
         const double a = h.vx[s], b = h.vy[s], cz = dim3d ? h.vz[s] : 0.0;
-        v = sqrt((a * a + b * b) + cz * cz);
+        const double u = sqrt((a * a + b * b) + cz * cz);
+        if constexpr (MULTI) {
+            // per-scene maxima: flush when the scene changes along the stride
+            const int ss = sid[h.id[s]];
+            if (ss != si) {
+                if (si >= 0 && v > 0.0) atomicMax(&out[si], (unsigned long long) __double_as_longlong(v));
+                si = ss;
+                v = 0.0;
+            }
+        }
+        v = u > v ? u : v;
     }
     if constexpr (MULTI) {
-        // per-scene maxima: one atomic per lane that carries a value
-        if (s < n) si = sid[h.id[s]];
-        if (v > 0.0) atomicMax(&out[si], (unsigned long long) __double_as_longlong(v));
+        if (si >= 0 && v > 0.0) atomicMax(&out[si], (unsigned long long) __double_as_longlong(v));
         return;
     }
     for (int o = 32; o > 0; o >>= 1) {
@@ -674,11 +685,12 @@ hipError_t launch_courant(int dim, int n, const Hot& h, const SceneTab& tab, int
                           hipStream_t st) {
     hipError_t e = fill_double(vmax, 0.0, (size_t) nscene, st);
     if (e != hipSuccess || n == 0) return e;
+    const dim3 grid(std::min(512u, grid_for(n, kBlock).x));
     if (tab.g)
-        hipLaunchKernelGGL(k_vmax<true>, grid_for(n, kBlock), dim3(kBlock), 0, st, n, h, dim == 3 ? 1 : 0, tab.sid,
+        hipLaunchKernelGGL(k_vmax<true>, grid, dim3(kBlock), 0, st, n, h, dim == 3 ? 1 : 0, tab.sid,
                            reinterpret_cast<unsigned long long*>(vmax));
     else
-        hipLaunchKernelGGL(k_vmax<false>, grid_for(n, kBlock), dim3(kBlock), 0, st, n, h, dim == 3 ? 1 : 0, tab.sid,
+        hipLaunchKernelGGL(k_vmax<false>, grid, dim3(kBlock), 0, st, n, h, dim == 3 ? 1 : 0, tab.sid,
                            reinterpret_cast<unsigned long long*>(vmax));
     return hipGetLastError();
 }

@@ -28,11 +28,11 @@ HBM_PEAK_GBS = 8000.0
 TOL = 1e-9
 
 
-def run_path(settings, scene, precision, steps, warmup, check_interval, reps, use_graph=True, precond=1):
+def run_path(settings, scene, precision, steps, warmup, check_interval, reps, use_graph=True, precond=1, warm=1):
     """settings, scene: one scene; settings = list of (settings, scene), scene =
     None: an ensemble in one handle."""
     sim = capi.Simulation(settings, scene)
-    sim.set_solver(precision=precision, tolerance=TOL, warm_start=True, check_interval=check_interval,
+    sim.set_solver(precision=precision, tolerance=TOL, warm_start=warm, check_interval=check_interval,
                    inner_tolerance=1e-4, use_graph=use_graph, preconditioner=precond)
     sim.load_initial_state()
     for _ in range(warmup):
@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--precond", choices=("multigrid", "jacobi"), default="multigrid",
                     help="CG preconditioner of both paths")
     ap.add_argument("--paths", default="fp64,mixed", help="paths to run (mixed sets value)")
+    ap.add_argument("--warm-start", type=int, default=2, choices=(0, 1, 2),
+                    help="CG start: 0 zero, 1 the last solution, 2 extrapolated from the last two")
     a = ap.parse_args()
     precond = capi.MPS_PRECOND_MULTIGRID if a.precond == "multigrid" else capi.MPS_PRECOND_JACOBI
     if a.gpus != 1:
@@ -121,7 +123,8 @@ def main():
     for name, prec in (("fp64", capi.MPS_PREC_FP64), ("mixed", capi.MPS_PREC_MIXED)):
         if name not in a.paths.split(","):
             continue
-        r = run_path(settings, scene, prec, a.steps, a.warmup, a.check_interval, a.reps, not a.no_graph, precond)
+        r = run_path(settings, scene, prec, a.steps, a.warmup, a.check_interval, a.reps, not a.no_graph, precond,
+                     a.warm_start)
         worst = max(r["residuals"]) if r["residuals"] else 0.0
         if not worst <= TOL:
             print(f"[bench] {name}: residual {worst:.3e} above {TOL:g}", file=sys.stderr)
@@ -175,7 +178,7 @@ def main():
             "n_fluid": n_fluid,
             "dt": settings.dt if scene is not None else "per scene: 0.2 l0 / 3.5",
             "tolerance": TOL,
-            "warm_start": True,
+            "warm_start": {0: "zero", 1: "last solution", 2: "extrapolated from the last two solutions"}[a.warm_start],
             "preconditioner": a.precond,
             "per_particle_value_basis": "n_fluid",
         },

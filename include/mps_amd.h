@@ -79,7 +79,8 @@ typedef struct mps_settings {
 #define MPS_PREC_MIXED 1       /* FP32 inner Jacobi-PCG + FP64 iterative refinement */
 typedef struct mps_solver_options {
     int32_t precision;         /* MPS_PREC_* */
-    int32_t warm_start;        /* 1: start from the previous step's unclamped solution */
+    int32_t warm_start;        /* 1: start from the previous step's unclamped solution;
+                                  2: from the line through the last two (where both are nonzero) */
     int32_t max_iterations;    /* cap on total CG iterations per solve; <=0: 2*rows */
     int32_t max_refinements;   /* mixed: cap on outer refinement steps; <=0: 20 */
     double tolerance;          /* stop when ||b-Ax||_2 <= tolerance*||b||_2 */
@@ -282,7 +283,7 @@ int mps_snapshot_end(mps_handle* h, int32_t slot, const int8_t** type, const dou
  * src/mps.cpp:38-94): the full particle state as a byte image, every field in
  * FP64 plus types, boundary conditions and the solver's warm start, so a run
  * resumed from it continues bit for bit as if uninterrupted.  The image is
- * mps_checkpoint_size() bytes (a 64-byte header, then 11 doubles and 2 bytes
+ * mps_checkpoint_size() bytes (a 64-byte header, then 12 doubles and 2 bytes
  * per particle); mps_checkpoint_load accepts only an image of a handle of the
  * same shape (particle count, dim, scenes) and validates it first.  Debug
  * events and solver statistics are not part of the state. */

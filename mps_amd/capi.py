@@ -232,7 +232,7 @@ class Simulation:
                    preconditioner=0):
         o = SolverOptions()
         o.precision = precision
-        o.warm_start = int(bool(warm_start))
+        o.warm_start = 2 if warm_start == 2 else int(bool(warm_start))
         o.max_iterations = max_iterations
         o.max_refinements = max_refinements
         o.tolerance = float(np.finfo(np.float64).eps) if tolerance is None else float(tolerance)

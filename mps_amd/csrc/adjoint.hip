@@ -36,8 +36,9 @@ __device__ __forceinline__ double wprime(double dist, double re) { return -re / 
 //
 // A saved state holds, per particle id: x y z vx vy vz (z and vz are the
 // frozen plane values in 2D), the warm-start solution, pressure, minimum
-// pressure, number density ratio and a ghost's stale number density (11 n
-// doubles), then type and boundary condition (2 n bytes).  Restoring writes
+// pressure, number density ratio, a ghost's stale number density and the
+// solution before the warm-start one (12 n doubles), then type and boundary
+// condition (2 n bytes).  Restoring writes
 // slot = id, the layout of a fresh handle; the next bucket build sorts it, and
 // the build's result does not depend on the slot order it starts from.
 
@@ -58,6 +59,7 @@ __global__ void __launch_bounds__(kBlock) k_save_state(int n, int dim, Hot h, Co
     f[8 * N + id] = cold.pmin[id];
     f[9 * N + id] = cold.ndr[id];
     f[10 * N + id] = cold.ghost_n[id];
+    f[11 * N + id] = cold.pprev2[id];
     t8[id] = h.type[s];
     t8[N + id] = cold.bc[id];
 }
@@ -84,6 +86,7 @@ __global__ void __launch_bounds__(kBlock) k_restore_state(int n, int dim, Hot h,
     cold.pmin[i] = f[8 * N + i];
     cold.ndr[i] = f[9 * N + i];
     cold.ghost_n[i] = f[10 * N + i];
+    cold.pprev2[i] = f[11 * N + i];
     h.id[i] = i;
     h.type[i] = t8[i];
     cold.bc[i] = t8[N + i];

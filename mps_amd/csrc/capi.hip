@@ -301,6 +301,7 @@ static int alloc_all(mps_handle* h) {
     HIPCHK(h, dalloc(&h->cold.ndr, n));
     HIPCHK(h, dalloc(&h->cold.bc, n));
     HIPCHK(h, dalloc(&h->cold.pprev, n));
+    HIPCHK(h, dalloc(&h->cold.pprev2, n));
     HIPCHK(h, dalloc(&h->cold.ghost_n, n));
     HIPCHK(h, dalloc(&h->cold.z, n));
     HIPCHK(h, dalloc(&h->cold.vz, n));
@@ -383,6 +384,7 @@ static int alloc_all(mps_handle* h) {
     HIPCHK(h, fill_double(h->cold.pmin, 0.0, n, h->st));
     HIPCHK(h, fill_double(h->cold.ndr, 0.0, n, h->st));
     HIPCHK(h, fill_double(h->cold.pprev, 0.0, n, h->st));
+    HIPCHK(h, fill_double(h->cold.pprev2, 0.0, n, h->st));
     HIPCHK(h, fill_double(h->cold.ghost_n, 0.0, n, h->st));
     HIPCHK(h, hipMemsetAsync(h->cold.bc, 0, n, h->st));  // Ignored
     h->cg = cg_create();
@@ -400,7 +402,7 @@ static void free_all(mps_handle* h) {
         hipFree(h->KEY[k]);
     }
     void* ptrs[] = {h->bb.cnt,      h->bb.cs,         h->bb.bin_id,  h->bb.dst,       h->bb.scan_tmp, h->bb.slot_of,
-                    h->cold.p,      h->cold.pmin,     h->cold.ndr,   h->cold.bc,      h->cold.pprev,  h->cold.ghost_n,
+                    h->cold.p,      h->cold.pmin,     h->cold.ndr,   h->cold.bc,      h->cold.pprev,  h->cold.pprev2, h->cold.ghost_n,
                     h->cold.z,      h->cold.vz,       h->w.bc_s,     h->w.b_s,        h->w.diag_s,    h->w.isq_s,    h->w.nnz_s,
                     h->w.parent,    h->w.surf,        h->w.inner_flag, h->w.row_of,   h->w.p_s,       h->w.pmin_s,
                     h->w.vmax,      h->w.coll_cnt,    h->w.coll_off, h->w.coll_pairs, h->w.coll_dist, h->w.coll_level,
@@ -541,7 +543,7 @@ static int step_core(mps_handle* h, int is_export, double* vmax_out, StepTap* ta
                             h->ev.err, &rows, &entries, h->st, h->bs[F_FILL]));
     tend(h);
     std::string msg;
-    HIPCHK(h, cg_solve(h->cg, h->A, c, n, h->w.b_s, h->cold.pprev, h->x_s, h->opt, h->sstats.data(), h->st, msg));
+    HIPCHK(h, cg_solve(h->cg, h->A, c, n, h->w.b_s, h->cold.pprev, h->cold.pprev2, h->x_s, h->opt, h->sstats.data(), h->st, msg));
     tbeg(h, F_POUT);
     HIPCHK(h, launch_pressure_out(h->dim, g, tab(h), n, h->c, c, h->x_s, h->cold, h->w, h->st, h->bs[F_POUT]));
     tend(h);
@@ -686,7 +688,7 @@ static int adj_reverse_step(mps_handle* h, const StepTap& tap, const double* f0,
     HIPCHK(h, adj_pressure_stage(dim, g, n, h->c, tap.c3, S.type_g, h->w, h->x_s, a, h->st));
     mps_solver_stats ss{};
     std::string msg;
-    HIPCHK(h, cg_solve(h->cg, h->A, tap.c3, n, a.rhs, nullptr, a.lam, oa, &ss, h->st, msg));
+    HIPCHK(h, cg_solve(h->cg, h->A, tap.c3, n, a.rhs, nullptr, nullptr, a.lam, oa, &ss, h->st, msg));
     agg.rows = std::max(agg.rows, ss.rows);
     agg.iterations += ss.iterations;
     agg.refinements += ss.refinements;
@@ -1042,6 +1044,7 @@ int mps_set_solver(mps_handle* h, const mps_solver_options* o) {
     if (!(o->tolerance > 0)) return fail(h, MPS_E_INVALID, "tolerance must be positive");
     if (o->preconditioner != MPS_PRECOND_JACOBI && o->preconditioner != MPS_PRECOND_MULTIGRID)
         return fail(h, MPS_E_INVALID, "unknown preconditioner");
+    if (o->warm_start < 0 || o->warm_start > 2) return fail(h, MPS_E_INVALID, "warm_start must be 0, 1 or 2");
     h->opt = *o;
     if (h->opt.check_interval < 1) h->opt.check_interval = 1;
     return MPS_OK;
@@ -1198,6 +1201,9 @@ int mps_upload(mps_handle* h, const int8_t* type, const double* pos, const doubl
     if (pressure) {
         HIPCHK(h, hipMemcpy(h->cold.p, pressure, N * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK(h, hipMemcpy(h->cold.pprev, pressure, N * sizeof(double), hipMemcpyHostToDevice));
+        // no earlier solution to extrapolate from
+        HIPCHK(h, fill_double(h->cold.pprev2, 0.0, N, h->st));
+        HIPCHK(h, hipStreamSynchronize(h->st));
     }
     if (ndr) HIPCHK(h, hipMemcpy(h->cold.ndr, ndr, N * sizeof(double), hipMemcpyHostToDevice));
     if (bc) {

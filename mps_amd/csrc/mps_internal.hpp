@@ -311,6 +311,7 @@ struct Cold {
     double* ndr;      // number density ratio (id order)
     int8_t* bc;       // boundary condition (id order)
     double* pprev;    // last unclamped solution (id order), warm start
+    double* pprev2;   // the one before it (extrapolated warm start)
     double* ghost_n;  // number density (re_p) of the last list a ghost had
     double* z;        // 2D: frozen z of every particle (id order)
     double* vz;       // 2D: frozen vz
@@ -435,6 +436,7 @@ hipError_t cg_reserve(CgWork* cw, int rows);
 // may be NULL) gives the warm start.  Fills stats.
 // stats: one entry per segment.
 hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double* b_s, const double* pprev_id,
+                    const double* pprev2_id,
                     double* x_s, const mps_solver_options& o, mps_solver_stats* stats, hipStream_t st,
                     std::string& msg);
 hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, hipStream_t st, double ms[4],
@@ -457,7 +459,7 @@ struct AdjBufs {
     double* part;    // block partials of the parameter sums
     double* acc;     // dL/dnu, dL/drestitution, dL/drelaxation
 };
-constexpr int kStateDoubles = 11;  // doubles per particle of a saved state
+constexpr int kStateDoubles = 12;  // doubles per particle of a saved state
 constexpr int kStateBytes = 2;     // bytes per particle of a saved state
 hipError_t save_state(int dim, int n, const Hot& h, const Cold& cold, double* f, int8_t* t8, hipStream_t st);
 hipError_t restore_state(int dim, int n, Hot& h, Cold& cold, int* slot_of, const double* f, const int8_t* t8,

@@ -554,7 +554,8 @@ __global__ void __launch_bounds__(kBlock) k_reduce_parts(Seg sg, const double* _
 // FP64 init: yhat = x0 sqrt(d), bhat = S b; partial ||b||^2.
 __global__ void __launch_bounds__(kBlock) k_cg_init(Seg sg, const int* __restrict__ row_slot,
                                                     const int32_t* __restrict__ id, const double* __restrict__ b_s,
-                                                    const double* __restrict__ pprev, const double* __restrict__ sdiag,
+                                                    const double* __restrict__ pprev,
+                                                    const double* __restrict__ pprev2, const double* __restrict__ sdiag,
                                                     double* __restrict__ y, double* __restrict__ bh,
                                                     double* __restrict__ part) {
     __shared__ double red[3 * kBlock / 64];
@@ -565,7 +566,18 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(Seg sg, const int* __restric
         const double b = b_s[s];
         const double si = sdiag[row];
         bh[row] = b * si;
-        y[row] = pprev ? pprev[id[s]] / si : 0.0;
+        double x0 = 0.0;
+        if (pprev) {
+            // previous solution, or the line through the last two where both
+            // steps solved for this particle
+            const double a = pprev[id[s]];
+            x0 = a;
+            if (pprev2) {
+                const double c = pprev2[id[s]];
+                if (a != 0.0 && c != 0.0) x0 = 2.0 * a - c;
+            }
+        }
+        y[row] = x0 / si;
         bb += b * b;
     }
     double v[1] = {bb};
@@ -874,6 +886,7 @@ static hipError_t run_cg(CgWork* cw, int which, const Sell& A, const T* val, T* 
 }
 
 hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double* b_s, const double* pprev_id,
+                    const double* pprev2_id,
                     double* x_s, const mps_solver_options& o, mps_solver_stats* stats, hipStream_t st,
                     std::string& msg) {
     (void) n;
@@ -903,7 +916,7 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
     }
 
     hipLaunchKernelGGL(k_cg_init, gr, bl, 0, st, sg, A.row_slot, h.id, b_s, o.warm_start ? pprev_id : nullptr,
-                       A.sdiag, cw->y, cw->bh, cw->part);
+                       o.warm_start == 2 ? pprev2_id : nullptr, A.sdiag, cw->y, cw->bh, cw->part);
     if ((e = reduce_and_read(cw, 1, st))) return e;
     std::vector<double> bn2(nseg), thr(nseg), rho2(nseg);
     std::vector<int> segrows(nseg), conv(nseg), total_it(nseg, 0), refinements(nseg, 0), max_it(nseg);

@@ -515,6 +515,7 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_pressure_out(int n, Hot h, co
     if (s >= n) return;
     const int id = h.id[s];
     const double x = bc_s[s] == MPS_BC_INNER ? x_s[s] : 0.0;  // empty rows solve to 0
+    cold.pprev2[id] = cold.pprev[id];
     cold.pprev[id] = x;
     const double p = x < 0.0 ? 0.0 : x;  // removeNegativePressure
     p_s[s] = p;

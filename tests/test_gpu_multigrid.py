@@ -74,3 +74,27 @@ def test_small_systems_keep_jacobi():
     a, _ = _run("dambreak2d_584", capi.MPS_PREC_FP64, capi.MPS_PRECOND_JACOBI, 3)
     b, _ = _run("dambreak2d_584", capi.MPS_PREC_FP64, capi.MPS_PRECOND_MULTIGRID, 3)
     assert (a["p"] == b["p"]).all()
+
+
+@pytest.mark.parametrize("precision", [capi.MPS_PREC_FP64, capi.MPS_PREC_MIXED])
+def test_extrapolated_warm_start_solves_the_same_system(precision):
+    """warm_start = 2 starts the CG from 2 p_n - p_(n-1): fewer iterations over
+    a run, the same solutions to what the tolerance leaves free."""
+    s, sc, _ = configs.workload("dambreak2d_35k")
+    out = {}
+    for warm in (1, 2):
+        sim = capi.Simulation(s, sc)
+        sim.set_solver(precision=precision, tolerance=1e-9, warm_start=warm,
+                       preconditioner=capi.MPS_PRECOND_MULTIGRID)
+        sim.load_initial_state()
+        its = 0
+        for _ in range(12):
+            sim.step()
+            st = sim.solver_stats()
+            assert st["converged"] == 1 and st["rel_residual"] <= 1e-9
+            its += st["iterations"]
+        out[warm] = (sim.state(), its)
+        sim.close()
+    assert out[2][1] <= out[1][1]
+    p1, p2 = out[1][0]["p"], out[2][0]["p"]
+    assert np.abs(p1 - p2).max() <= 1e-5 * np.abs(p1).max()

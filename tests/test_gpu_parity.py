@@ -333,14 +333,16 @@ def test_pressure_stabilisation_options(opts):
         g.upload(pos=st["pos"], vel=st["vel"], p=st["p"], ndr=st["ndr"])
 
 
-def test_checkpoint_resume_is_bit_identical():
+@pytest.mark.parametrize("warm", [1, 2])
+def test_checkpoint_resume_is_bit_identical(warm):
     """A run resumed from mps_checkpoint_save's image (in a fresh handle built
-    from the initial scene) continues bit for bit; a corrupted or foreign
-    image is refused."""
+    from the initial scene) continues bit for bit, with either warm start (the
+    extrapolated one needs the last two solutions from the image); a
+    corrupted or foreign image is refused."""
     s, sc, _ = configs.workload("dambreak2d_584")
     sc = _jitter(s, sc, 31, vamp=0.5)
     a = capi.Simulation(s, sc)
-    a.set_solver(precision=capi.MPS_PREC_FP64, tolerance=1e-12, warm_start=True)
+    a.set_solver(precision=capi.MPS_PREC_FP64, tolerance=1e-12, warm_start=warm)
     a.load_initial_state()
     for _ in range(7):
         a.step()
@@ -348,7 +350,7 @@ def test_checkpoint_resume_is_bit_identical():
     for _ in range(9):
         a.step()
     b = capi.Simulation(s, sc)
-    b.set_solver(precision=capi.MPS_PREC_FP64, tolerance=1e-12, warm_start=True)
+    b.set_solver(precision=capi.MPS_PREC_FP64, tolerance=1e-12, warm_start=warm)
     b.restore(img)
     for _ in range(9):
         b.step()

@@ -150,87 +150,108 @@ __device__ __forceinline__ int mg_stencil_of_cells(int k0, int k1, MgDims d) {
     return j;
 }
 
-// Coarse-row accumulator of one thread: the neighbours' columns and sums in
-// order of first appearance, found through their stencil positions (every
-// neighbour of an aggregate lies in the 3 x 3 (x 3) block of cells around
-// it).  Kept in LDS, indexed [slot][thread]; blocks of kMgAsmBlock threads.
-constexpr int kMgAsmBlock = 64;
-
-template <int W>
-struct MgAcc {
-    signed char (*slot)[kMgAsmBlock];
-    int (*col)[kMgAsmBlock];
-    double (*val)[kMgAsmBlock];
-    int t, m;
-    __device__ void init() {
-        for (int j = 0; j < W; j++) slot[j][t] = -1;
-        m = 0;
-    }
-    __device__ void add(int j, int c, double v) {
-        const int sl = slot[j][t];
-        if (sl >= 0) {
-            val[sl][t] += v;
-        } else {
-            slot[j][t] = (signed char) m;
-            col[m][t] = c;
-            val[m][t] = v;
-            m++;
-        }
-    }
-};
-
 // Galerkin level 1 from the fine SELL matrix (unit diagonal not stored;
 // padding entries point at their own row with value 0).  Aggregate K is the
-// bucket cell coord1[K] of scene scene1[K]; sums per neighbour in row, entry
-// order.
-template <typename T, typename IDX, int W>
-__global__ void __launch_bounds__(kMgAsmBlock) k_mg_assemble1(int n1, const int* __restrict__ cst1,
-                                                              const int* __restrict__ slice_pos,
-                                                              const IDX* __restrict__ col, const T* __restrict__ val,
-                                                              const int* __restrict__ agg0,
-                                                              const int* __restrict__ coord1,
-                                                              const int* __restrict__ scene1,
-                                                              const MgDims* __restrict__ dims, int cap1,
-                                                              int* __restrict__ col1, T* __restrict__ val1,
-                                                              T* __restrict__ diag1, T* __restrict__ dinv1,
-                                                              int* __restrict__ err) {
-    __shared__ signed char s_slot[W][kMgAsmBlock];
-    __shared__ int s_col[W][kMgAsmBlock];
-    __shared__ double s_val[W][kMgAsmBlock];
-    const int K = blockIdx.x * blockDim.x + threadIdx.x;
-    if (K >= n1) return;
-    MgAcc<W> acc{s_slot, s_col, s_val, (int) threadIdx.x, 0};
-    acc.init();
-    const MgDims d = dims[scene1[K]];
-    const int key = coord1[K];
-    double dg = 0.0;
-    for (int r = cst1[K]; r < cst1[K + 1]; r++) {
-        dg += 1.0;
-        const int sl = r >> 6, lane = r & 63;
-        const int s0 = slice_pos[sl], w = slice_pos[sl + 1] - s0;
-        const size_t base = (size_t) s0 * 64 + lane;
-        for (int k = 0; k < w; k++) {
-            const size_t idx = base + (size_t) k * 64;
-            const int K2 = agg0[mg_col(col, idx, r)];
-            const double v = (double) val[idx];
-            if (K2 == K) {
-                dg += v;
-                continue;
+// bucket cell coord1[K] of scene scene1[K].  A block takes A consecutive
+// aggregates, whose rows are one contiguous range, in chunks of kAsm1Rows
+// rows: one thread per row sums its entries into the row's stencil bins
+// (LDS, entry order), then one thread per (aggregate, bin) adds the rows'
+// bins in row order.  Row reads are coalesced; the sums are in a fixed order.
+constexpr int kAsm1Rows = 256;  // threads per block, rows per chunk
+
+template <typename T, typename IDX, int W, int A>
+__global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble1(int n1, const int* __restrict__ cst1,
+                                                            const int* __restrict__ slice_pos,
+                                                            const IDX* __restrict__ col, const T* __restrict__ val,
+                                                            const int* __restrict__ agg0,
+                                                            const int* __restrict__ coord1,
+                                                            const int* __restrict__ scene1,
+                                                            const MgDims* __restrict__ dims, int cap1,
+                                                            int* __restrict__ col1, T* __restrict__ val1,
+                                                            T* __restrict__ diag1, T* __restrict__ dinv1,
+                                                            int* __restrict__ err) {
+    __shared__ double s_bin[kAsm1Rows][W + 1];  // [W]: the row's entries inside its own aggregate
+    __shared__ double s_acc[A][W + 1];
+    __shared__ int s_nbr[A][W];                 // the aggregate at each stencil place (-1: none)
+    __shared__ int s_cst[A + 1];
+    __shared__ int s_key[A];
+    __shared__ MgDims s_dim[A];
+    const int tid = threadIdx.x;
+    const int K0 = blockIdx.x * A;
+    const int na = n1 - K0 < A ? n1 - K0 : A;
+    if (na <= 0) return;  // block-uniform
+    if (tid <= na) s_cst[tid] = cst1[K0 + tid];
+    if (tid < na) {
+        s_key[tid] = coord1[K0 + tid];
+        s_dim[tid] = dims[scene1[K0 + tid]];
+    }
+    for (int i = tid; i < A * W; i += kAsm1Rows) s_nbr[i / W][i % W] = -1;
+    for (int i = tid; i < A * (W + 1); i += kAsm1Rows) s_acc[i / (W + 1)][i % (W + 1)] = 0.0;
+    __syncthreads();
+    const int r0 = s_cst[0], r1 = s_cst[na];
+    for (int c0 = r0; c0 < r1; c0 += kAsm1Rows) {
+        const int r = c0 + tid;
+        for (int j = 0; j <= W; j++) s_bin[tid][j] = 0.0;
+        if (r < r1) {
+            int a = 0;
+            while (a + 1 < na && s_cst[a + 1] <= r) a++;
+            const int K = K0 + a, key = s_key[a];
+            const MgDims d = s_dim[a];
+            const int sl = r >> 6, lane = r & 63;
+            const int sp = slice_pos[sl], w = slice_pos[sl + 1] - sp;
+            const size_t base = (size_t) sp * 64 + lane;
+            for (int k = 0; k < w; k++) {
+                const size_t idx = base + (size_t) k * 64;
+                const int K2 = agg0[mg_col(col, idx, r)];
+                const double v = (double) val[idx];
+                if (K2 == K) {
+                    s_bin[tid][W] += v;
+                    continue;
+                }
+                const int j = mg_stencil_of_dkey<W>(coord1[K2] - key, d);
+                if (j < 0) {
+                    atomicOr(err, 1);
+                    continue;
+                }
+                s_bin[tid][j] += v;
+                s_nbr[a][j] = K2;  // every writer of this place writes the same aggregate
             }
-            const int j = mg_stencil_of_dkey<W>(coord1[K2] - key, d);
-            if (j < 0) {
-                atomicOr(err, 1);
-                continue;
-            }
-            acc.add(j, K2, v);
+        }
+        __syncthreads();
+        for (int t = tid; t < na * (W + 1); t += kAsm1Rows) {
+            const int a = t / (W + 1), j = t % (W + 1);
+            const int ra = s_cst[a] > c0 ? s_cst[a] : c0;
+            const int rb = s_cst[a + 1] < c0 + kAsm1Rows ? s_cst[a + 1] : c0 + kAsm1Rows;
+            double acc = s_acc[a][j];
+            for (int q = ra; q < rb; q++) acc += j == W ? 1.0 + s_bin[q - c0][W] : s_bin[q - c0][j];
+            s_acc[a][j] = acc;
+        }
+        __syncthreads();
+    }
+    // neighbours in stencil order, then padding
+    for (int t = tid; t < na * W; t += kAsm1Rows) {
+        const int a = t / W, j = t % W;
+        const int K = K0 + a;
+        int m = 0, slot = 0;
+        for (int q = 0; q < W; q++) {
+            const bool h = s_nbr[a][q] >= 0;
+            m += h ? 1 : 0;
+            slot += (h && q < j) ? 1 : 0;
+        }
+        if (s_nbr[a][j] >= 0) {
+            col1[(size_t) slot * cap1 + K] = s_nbr[a][j];
+            val1[(size_t) slot * cap1 + K] = (T) s_acc[a][j];
+        }
+        if (j >= m) {
+            col1[(size_t) j * cap1 + K] = -1;
+            val1[(size_t) j * cap1 + K] = (T) 0;
         }
     }
-    for (int j = 0; j < W; j++) {
-        col1[(size_t) j * cap1 + K] = j < acc.m ? s_col[j][threadIdx.x] : -1;
-        val1[(size_t) j * cap1 + K] = j < acc.m ? (T) s_val[j][threadIdx.x] : (T) 0;
+    if (tid < na) {
+        const double dg = s_acc[tid][W];
+        diag1[K0 + tid] = (T) dg;
+        dinv1[K0 + tid] = (T) (1.0 / dg);
     }
-    diag1[K] = (T) dg;
-    dinv1[K] = (T) (1.0 / dg);
 }
 
 // Parent (on the next level's dense index space) of level-l row K.
@@ -280,55 +301,89 @@ __global__ void __launch_bounds__(kBlock) k_mg_link(int n, const int* __restrict
     child[(size_t) slot * capp + pr] = K;
 }
 
-// Galerkin level l + 1 from level l: parent P sums its children's rows (child
-// slots in order, then their entries in order).
+// Galerkin level l + 1 from level l: parent P sums its children's rows.  A
+// block takes kAsm1Rows / C parents: one thread per (parent, child slot) sums
+// the child's entries into its stencil bins (LDS, entry order), then one
+// thread per (parent, bin) adds the children's bins in slot order.
 template <typename T, int W, int C>
-__global__ void __launch_bounds__(kMgAsmBlock) k_mg_assemble(int np, const int* __restrict__ child, int capp,
-                                                             const int* __restrict__ col, const T* __restrict__ val,
-                                                             const T* __restrict__ diag, int cap,
-                                                             const int* __restrict__ agg,
-                                                             const int* __restrict__ coordp,
-                                                             const int* __restrict__ scenep,
-                                                             const MgDims* __restrict__ dp, int* __restrict__ colp,
-                                                             T* __restrict__ valp, T* __restrict__ diagp,
-                                                             T* __restrict__ dinvp, int* __restrict__ err) {
-    __shared__ signed char s_slot[W][kMgAsmBlock];
-    __shared__ int s_col[W][kMgAsmBlock];
-    __shared__ double s_val[W][kMgAsmBlock];
-    const int P = blockIdx.x * blockDim.x + threadIdx.x;
-    if (P >= np) return;
-    MgAcc<W> acc{s_slot, s_col, s_val, (int) threadIdx.x, 0};
-    acc.init();
-    const MgDims d = dp[scenep[P]];
-    const int key = coordp[P];
-    double dg = 0.0;
-    for (int q = 0; q < C; q++) {
-        const int K = child[(size_t) q * capp + P];
-        if (K < 0) continue;
-        dg += (double) diag[K];
-        for (int j = 0; j < W; j++) {
-            const int c = col[(size_t) j * cap + K];
-            if (c < 0) break;
-            const double v = (double) val[(size_t) j * cap + K];
-            const int P2 = agg[c];
-            if (P2 == P) {
-                dg += v;
-                continue;
+__global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble(int np, const int* __restrict__ child, int capp,
+                                                           const int* __restrict__ col, const T* __restrict__ val,
+                                                           const T* __restrict__ diag, int cap,
+                                                           const int* __restrict__ agg,
+                                                           const int* __restrict__ coordp,
+                                                           const int* __restrict__ scenep,
+                                                           const MgDims* __restrict__ dp, int* __restrict__ colp,
+                                                           T* __restrict__ valp, T* __restrict__ diagp,
+                                                           T* __restrict__ dinvp, int* __restrict__ err) {
+    constexpr int A = kAsm1Rows / C;
+    __shared__ double s_bin[kAsm1Rows][W + 1];  // [W]: the child's diagonal and entries inside P
+    __shared__ int s_nbr[A][W];
+    const int tid = threadIdx.x;
+    const int P0 = blockIdx.x * A;
+    const int na = np - P0 < A ? np - P0 : A;
+    if (na <= 0) return;  // block-uniform
+    for (int i = tid; i < A * W; i += kAsm1Rows) s_nbr[i / W][i % W] = -1;
+    for (int j = 0; j <= W; j++) s_bin[tid][j] = 0.0;
+    __syncthreads();
+    {
+        const int a = tid / C, q = tid % C;
+        const int P = P0 + a;
+        const int K = a < na ? child[(size_t) q * capp + P] : -1;
+        if (K >= 0) {
+            const MgDims d = dp[scenep[P]];
+            const int key = coordp[P];
+            s_bin[tid][W] = (double) diag[K];
+            for (int j = 0; j < W; j++) {
+                const int c = col[(size_t) j * cap + K];
+                if (c < 0) break;
+                const double v = (double) val[(size_t) j * cap + K];
+                const int P2 = agg[c];
+                if (P2 == P) {
+                    s_bin[tid][W] += v;
+                    continue;
+                }
+                const int sj = mg_stencil_of_cells<W>(key, coordp[P2], d);
+                if (sj < 0) {
+                    atomicOr(err, 1);
+                    continue;
+                }
+                s_bin[tid][sj] += v;
+                s_nbr[a][sj] = P2;  // every writer of this place writes the same parent
             }
-            const int sj = mg_stencil_of_cells<W>(key, coordp[P2], d);
-            if (sj < 0) {
-                atomicOr(err, 1);
-                continue;
-            }
-            acc.add(sj, P2, v);
         }
     }
-    for (int j = 0; j < W; j++) {
-        colp[(size_t) j * capp + P] = j < acc.m ? s_col[j][threadIdx.x] : -1;
-        valp[(size_t) j * capp + P] = j < acc.m ? (T) s_val[j][threadIdx.x] : (T) 0;
+    __syncthreads();
+    // reduce the children's bins into the first child's row of LDS
+    for (int t = tid; t < na * (W + 1); t += kAsm1Rows) {
+        const int a = t / (W + 1), j = t % (W + 1);
+        double acc = 0.0;
+        for (int q = 0; q < C; q++) acc += s_bin[a * C + q][j];
+        s_bin[a * C][j] = acc;  // only this thread reads or writes [a * C][j] now
     }
-    diagp[P] = (T) dg;
-    dinvp[P] = (T) (1.0 / dg);
+    __syncthreads();
+    for (int t = tid; t < na * W; t += kAsm1Rows) {
+        const int a = t / W, j = t % W;
+        const int P = P0 + a;
+        int m = 0, slot = 0;
+        for (int q = 0; q < W; q++) {
+            const bool h = s_nbr[a][q] >= 0;
+            m += h ? 1 : 0;
+            slot += (h && q < j) ? 1 : 0;
+        }
+        if (s_nbr[a][j] >= 0) {
+            colp[(size_t) slot * capp + P] = s_nbr[a][j];
+            valp[(size_t) slot * capp + P] = (T) s_bin[a * C][j];
+        }
+        if (j >= m) {
+            colp[(size_t) j * capp + P] = -1;
+            valp[(size_t) j * capp + P] = (T) 0;
+        }
+    }
+    if (tid < na) {
+        const double dg = s_bin[tid * C][W];
+        diagp[P0 + tid] = (T) dg;
+        dinvp[P0 + tid] = (T) (1.0 / dg);
+    }
 }
 
 // ---------------- the V-cycle ----------------
@@ -897,15 +952,15 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
     if ((e = hipMemcpyAsync(m->d_sdims[1], m->sdims[1].data(), sizeof(MgDims) * (size_t) ns, hipMemcpyHostToDevice,
                             st)))
         return e;
-    const dim3 ab(kMgAsmBlock);
+    const dim3 ab(kAsm1Rows);
     if (dim == 2)
-        hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 9>), dim3((n1 + kMgAsmBlock - 1) / kMgAsmBlock), ab, 0, st, n1,
-                           m->cst1, A.slice_off, fcol, fval, m->agg[0], m->coord[1], m->scene[1], m->d_sdims[1],
-                           m->cap[1], m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1], m->err);
+        hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 9, 16>), dim3((n1 + 15) / 16), dim3(kAsm1Rows), 0, st, n1, m->cst1,
+                           A.slice_off, fcol, fval, m->agg[0], m->coord[1], m->scene[1], m->d_sdims[1], m->cap[1],
+                           m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1], m->err);
     else
-        hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 27>), dim3((n1 + kMgAsmBlock - 1) / kMgAsmBlock), ab, 0, st, n1,
-                           m->cst1, A.slice_off, fcol, fval, m->agg[0], m->coord[1], m->scene[1], m->d_sdims[1],
-                           m->cap[1], m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1], m->err);
+        hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 27, 8>), dim3((n1 + 7) / 8), dim3(kAsm1Rows), 0, st, n1, m->cst1,
+                           A.slice_off, fcol, fval, m->agg[0], m->coord[1], m->scene[1], m->d_sdims[1], m->cap[1],
+                           m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1], m->err);
     // stop at <= 64 rows, or once every scene's grid is a single cell
     const int coarse = std::max(kMgCoarse, ns);
     int l = 1;
@@ -951,7 +1006,7 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
         hipLaunchKernelGGL(k_mg_link, mg_grid(m->n[l]), dim3(kBlock), 0, st, m->n[l], m->coord[l], m->scene[l],
                            m->d_sdims[l], m->d_sdims[l + 1], m->d_goff[l + 1], m->pscan, m->agg[l],
                            m->child[l + 1], m->cap[l + 1]);
-        const dim3 pg((np + kMgAsmBlock - 1) / kMgAsmBlock);
+        const dim3 pg((np + kAsm1Rows / (dim == 2 ? 4 : 8) - 1) / (kAsm1Rows / (dim == 2 ? 4 : 8)));
         if (dim == 2)
             hipLaunchKernelGGL((k_mg_assemble<T, 9, 4>), pg, ab, 0, st, np, m->child[l + 1], m->cap[l + 1], m->col[l],
                                (const T*) m->val[l], (const T*) m->diag[l], m->cap[l], m->agg[l], m->coord[l + 1],

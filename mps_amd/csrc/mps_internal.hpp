@@ -139,22 +139,31 @@ __device__ __forceinline__ double wfun(double dist, double re) {
 // Candidates q in [b, e) of one cell, in ascending order.  Positions are
 // loaded kNbBatch at a time (indices clamped to e - 1, so every load stays in
 // the cell), so one loop trip has kNbBatch independent loads in flight
-// instead of one; the visit order and the arithmetic are those of a plain
-// q = b .. e-1 loop.  d2 > re2 skips the square root (see for_neighbors).
+// instead of one; the visit order and the arithmetic of what f receives are
+// those of a plain q = b .. e-1 loop.  d2 > re2 skips the square root (see
+// for_neighbors); re2 = re^2 (1 + 1e-12) leaves room for the fused
+// prefilter, whose d2 differs from the reference's rounding by a few ulp.
 constexpr int kNbBatch = 4;
 
 template <int DIM, typename F>
 __device__ __forceinline__ void scan_cell(const double* __restrict__ px, const double* __restrict__ py,
                                           const double* __restrict__ pz, int b, int e, int s, double xi,
                                           double yi, double zi, double re, double re2, F& f) {
+    // 32-bit byte offsets (slot counts stay below 2^28): one offset register
+    // serves the three arrays, with their bases in scalar registers
+    const char* __restrict__ bx = reinterpret_cast<const char*>(px);
+    const char* __restrict__ by = reinterpret_cast<const char*>(py);
+    const char* __restrict__ bz = reinterpret_cast<const char*>(pz);
+    const unsigned last = (unsigned) (e - 1);
     for (int q0 = b; q0 < e; q0 += kNbBatch) {
         double x[kNbBatch], y[kNbBatch], z[kNbBatch];
 #pragma unroll
         for (int i = 0; i < kNbBatch; i++) {
-            const int q = q0 + i < e ? q0 + i : e - 1;
-            x[i] = px[q];
-            y[i] = py[q];
-            if constexpr (DIM == 3) z[i] = pz[q];
+            const unsigned q = (unsigned) (q0 + i) < last ? (unsigned) (q0 + i) : last;
+            const unsigned off = q * 8u;
+            x[i] = *reinterpret_cast<const double*>(bx + off);
+            y[i] = *reinterpret_cast<const double*>(by + off);
+            if constexpr (DIM == 3) z[i] = *reinterpret_cast<const double*>(bz + off);
         }
 #pragma unroll
         for (int i = 0; i < kNbBatch; i++) {
@@ -162,14 +171,12 @@ __device__ __forceinline__ void scan_cell(const double* __restrict__ px, const d
             if (q >= e) break;
             const double dx = x[i] - xi;
             const double dy = y[i] - yi;
-            double dz, d2;
-            if constexpr (DIM == 3) {
-                dz = z[i] - zi;
-                d2 = (dx * dx + dy * dy) + dz * dz;
-            } else {
-                dz = 0.0;
-                d2 = dx * dx + dy * dy;
-            }
+            const double dz = DIM == 3 ? z[i] - zi : 0.0;
+            // cheap test first: d2 with fused multiply-adds is within a few
+            // ulp of the reference's expression, far inside re2's margin
+            const double d2f = DIM == 3 ? fma(dz, dz, fma(dy, dy, dx * dx)) : fma(dy, dy, dx * dx);
+            if (d2f > re2) continue;
+            const double d2 = DIM == 3 ? (dx * dx + dy * dy) + dz * dz : dx * dx + dy * dy;
             if (d2 > re2) continue;
             const double dist = sqrt(d2);
             if (q != s && dist < re) f(q, dist, dx, dy, dz);
@@ -236,6 +243,173 @@ __device__ __forceinline__ void for_neighbors(const GridView& g, const double* _
                     }
                     const int k = g.flat ? (jx + jy * g.nx) : (jx + jy * g.nx + jz * g.nx * g.ny);
                     scan_cell<3>(px, py, pz, cs[k], cs[k + 1], s, xi, yi, zi, re, re2, f);
+                }
+            }
+        }
+    }
+}
+
+// ---------------- walks from LDS ----------------
+//
+// A walk's candidate loads are gathers: the 64 lanes of a wavefront sit in
+// several cells, so one load instruction touches many cache lines, and the
+// L1 tag lookups (one line per cycle) bound the walk.  A block can instead
+// copy, coalesced, the slot ranges its walks can reach into LDS and read the
+// candidates there.  The block's slots are consecutive, so their cell keys
+// form one range [k_lo, k_hi]; every neighbour cell of a key k in it lies in
+// [k_lo - 1 + o, k_hi + 1 + o] for one stencil-row offset o = dy nx (+ dz nx
+// ny), and cells of consecutive keys are consecutive slot ranges: one slot
+// range per stencil row (3 in 2D, 9 in 3D).  Blocks whose ranges exceed the
+// LDS capacity, that hold ghosts or another scene's cells, or that are on a
+// flat 3D grid read global memory as before.  Either way the walk visits the
+// same candidates in the same order with the same arithmetic.
+template <int DIM>
+struct NbStage {
+    static constexpr int NR = DIM == 2 ? 3 : 9;  // stencil rows (dy, then dz)
+    int ok;           // this block's walks read LDS
+    const int* meta;  // LDS: [0] staged, [1 + r] LDS index minus slot for row r
+    const double* x;  // LDS copies of the staged positions
+    const double* y;
+    const double* z;
+};
+
+template <int DIM>
+constexpr int nb_stage_meta_ints() {
+    return 1 + 3 * NbStage<DIM>::NR + 1;
+}
+
+// Every thread of the block must call it (it holds two block barriers when
+// cap > 0).  meta: nb_stage_meta_ints<DIM>() ints of LDS; sx, sy, sz: cap
+// doubles each (sz unused in 2D).  cap == 0: no staging, no barrier.
+template <int DIM>
+__device__ __forceinline__ NbStage<DIM> nb_stage(const GridView& g, const Hot& h, int n, int cap, int* meta,
+                                                 double* sx, double* sy, double* sz) {
+    constexpr int NR = NbStage<DIM>::NR;
+    if (cap <= 0) return NbStage<DIM>{0, meta, sx, sy, sz};
+    int* first = meta + 1 + NR;
+    int* base = meta + 1 + 2 * NR;  // NR + 1 entries
+    if (threadIdx.x == 0) {
+        int ok = 0;
+        const int s0 = blockIdx.x * blockDim.x;
+        const int s1 = (s0 + (int) blockDim.x < n ? s0 + (int) blockDim.x : n) - 1;
+        if (s0 < n && !(DIM == 3 && g.flat)) {
+            const int k_lo = h.key[s0] - g.key0, k_hi = h.key[s1] - g.key0;
+            if (k_lo >= 0 && k_hi < g.ncell) {
+                const int* cs = g.cs + g.key0;
+                const int nx = g.nx, nxy = g.nx * g.ny;
+                int tot = 0;
+                for (int r = 0; r < NR; r++) {
+                    const int dy = r % 3 - 1, dz = DIM == 3 ? r / 3 - 1 : 0;
+                    int lo = k_lo - 1 + dy * nx + dz * nxy, hi = k_hi + 1 + dy * nx + dz * nxy;
+                    lo = lo < 0 ? 0 : lo;
+                    hi = hi > g.ncell - 1 ? g.ncell - 1 : hi;
+                    const int a = lo <= hi ? cs[lo] : 0, b = lo <= hi ? cs[hi + 1] : 0;
+                    first[r] = a;
+                    base[r] = tot;
+                    meta[1 + r] = tot - a;
+                    tot += b - a;
+                }
+                base[NR] = tot;
+                ok = tot <= cap ? 1 : 0;
+            }
+        }
+        meta[0] = ok;
+    }
+    __syncthreads();
+    if (meta[0]) {
+        const int tot = base[NR];
+        for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+            int r = 0;
+            while (r + 1 < NR && base[r + 1] <= i) r++;
+            const int q = first[r] + (i - base[r]);
+            sx[i] = h.px[q];
+            sy[i] = h.py[q];
+            if constexpr (DIM == 3) sz[i] = h.pz[q];
+        }
+    }
+    __syncthreads();
+    return NbStage<DIM>{meta[0], meta, sx, sy, sz};
+}
+
+// scan_cell on the LDS copy: candidate slot q sits at index q + off.
+template <int DIM, typename F>
+__device__ __forceinline__ void scan_cell_lds(const double* __restrict__ sx, const double* __restrict__ sy,
+                                              const double* __restrict__ sz, int off, int b, int e, int s,
+                                              double xi, double yi, double zi, double re, double re2, F& f) {
+    for (int q0 = b; q0 < e; q0 += kNbBatch) {
+        double x[kNbBatch], y[kNbBatch], z[kNbBatch];
+#pragma unroll
+        for (int i = 0; i < kNbBatch; i++) {
+            const int q = q0 + i < e ? q0 + i : e - 1;
+            x[i] = sx[q + off];
+            y[i] = sy[q + off];
+            if constexpr (DIM == 3) z[i] = sz[q + off];
+        }
+#pragma unroll
+        for (int i = 0; i < kNbBatch; i++) {
+            const int q = q0 + i;
+            if (q >= e) break;
+            const double dx = x[i] - xi;
+            const double dy = y[i] - yi;
+            const double dz = DIM == 3 ? z[i] - zi : 0.0;
+            const double d2f = DIM == 3 ? fma(dz, dz, fma(dy, dy, dx * dx)) : fma(dy, dy, dx * dx);
+            if (d2f > re2) continue;
+            const double d2 = DIM == 3 ? (dx * dx + dy * dy) + dz * dz : dx * dx + dy * dy;
+            if (d2 > re2) continue;
+            const double dist = sqrt(d2);
+            if (q != s && dist < re) f(q, dist, dx, dy, dz);
+        }
+    }
+}
+
+// for_neighbors reading the block's LDS copy when it has one.
+template <int DIM, typename F>
+__device__ __forceinline__ void for_neighbors(const NbStage<DIM>& st, const GridView& g,
+                                              const double* __restrict__ px, const double* __restrict__ py,
+                                              const double* __restrict__ pz, int key, int s, double xi, double yi,
+                                              double zi, double re, F&& f) {
+    if (!st.ok) {
+        for_neighbors<DIM>(g, px, py, pz, key, s, xi, yi, zi, re, f);
+        return;
+    }
+    key -= g.key0;
+    const int* __restrict__ cs = g.cs + g.key0;
+    const double re2 = re * re * (1.0 + 1e-12);
+    if constexpr (DIM == 2) {
+        const int ix = key % g.nx;
+        const int iy = key / g.nx;
+        int cb[9], ce[9];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                const int jx = ix - 1 + a, jy = iy - 1 + b;
+                const bool ok = jx >= 0 && jx < g.nx && jy >= 0 && jy < g.ny;
+                const int k = jx + jy * g.nx;
+                cb[3 * a + b] = ok ? cs[k] : 0;
+                ce[3 * a + b] = ok ? cs[k + 1] : 0;
+            }
+        }
+        const int o0 = st.meta[1], o1 = st.meta[2], o2 = st.meta[3];
+#pragma unroll
+        for (int c = 0; c < 9; c++) {
+            const int b = c % 3;
+            scan_cell_lds<2>(st.x, st.y, st.z, b == 0 ? o0 : (b == 1 ? o1 : o2), cb[c], ce[c], s, xi, yi, 0.0, re,
+                             re2, f);
+        }
+    } else {
+        const int ix = key % g.nx;
+        const int iy = (key / g.nx) % g.ny;
+        const int iz = key / (g.nx * g.ny);
+        for (int jx = ix - 1; jx <= ix + 1; jx++) {
+            if (jx < 0 || jx >= g.nx) continue;
+            for (int jy = iy - 1; jy <= iy + 1; jy++) {
+                if (jy < 0 || jy >= g.ny) continue;
+                for (int jz = iz - 1; jz <= iz + 1; jz++) {
+                    if (jz < 0 || jz >= g.nz) continue;
+                    const int k = jx + jy * g.nx + jz * g.nx * g.ny;
+                    const int r = (jy - iy + 1) + 3 * (jz - iz + 1);
+                    scan_cell_lds<3>(st.x, st.y, st.z, st.meta[1 + r], cs[k], cs[k + 1], s, xi, yi, zi, re, re2, f);
                 }
             }
         }

@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_rows(int n, const int* __rest
 // of their rows together, so every store is one coalesced 64-lane line.
 // Entries past kFillStage (dense 3D rows) are written directly.  The 32-bit
 // column array is written only when asked (some offset does not fit 16 bits).
-constexpr int kFillStage = 24;
+constexpr int kFillStage = 12;
 
 template <int DIM, bool MULTI>
 __global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneTab tab, Hot h,
@@ -180,7 +180,12 @@ __global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneT
         val64[idx] = 0.0;
         if (val32) val32[idx] = 0.0f;
     }
-    if (md > 0) atomicMax(maxdelta, md);
+    // one atomic per wavefront (the block is one wavefront; every lane is here)
+    for (int o = 32; o > 0; o >>= 1) {
+        const int u = __shfl_xor(md, o, 64);
+        md = u > md ? u : md;
+    }
+    if (lane == 0 && md > 0) atomicMax(maxdelta, md);
 }
 
 hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,

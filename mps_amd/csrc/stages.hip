@@ -53,11 +53,27 @@ __device__ __forceinline__ double ndens(const GridView& g, const Hot& h, int s, 
     return nd;
 }
 
+// LDS staging of a block's candidate positions (see nb_stage): capacity in
+// slots per block; every thread of the block declares and fills it first.
+// 2D: 3 ranges of ~280 slots for 256-thread blocks.  3D is not staged: its 9
+// ranges of ~320 slots (72 KB in FP64) leave two blocks per CU, and the
+// measured walks were slower than from global memory.
+template <int DIM>
+constexpr int kStageCap = DIM == 2 ? 1536 : 0;
+
+#define NB_STAGE(DIM, GRID, HOT, N)                                                                        \
+    __shared__ int s_nb_meta[nb_stage_meta_ints<DIM>()];                                                 \
+    __shared__ double s_nb_x[kStageCap<DIM> > 0 ? kStageCap<DIM> : 1];                                  \
+    __shared__ double s_nb_y[kStageCap<DIM> > 0 ? kStageCap<DIM> : 1];                                  \
+    __shared__ double s_nb_z[DIM == 3 && kStageCap<DIM> > 0 ? kStageCap<DIM> : 1];                      \
+    const NbStage<DIM> nbs = nb_stage<DIM>(GRID, HOT, N, kStageCap<DIM>, s_nb_meta, s_nb_x, s_nb_y, s_nb_z)
+
 // ---------------- gravity + viscosity + explicit move (src/mps.cpp:165-215) ----------------
 
 template <int DIM, bool MULTI>
 __global__ void __launch_bounds__(kMaxTuneBlock) k_visc_move(GridView g0, Consts c0, SceneTab tab, int n, Hot cur, Hot out, Cold cold,
                                                       Events ev) {
+    NB_STAGE(DIM, g0, cur, n);
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     GridView g = g0;
@@ -69,7 +85,7 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_visc_move(GridView g0, Consts
         double ax = c.g[0], ay = c.g[1], az = c.g[2];
         double tx = 0.0, ty = 0.0, tz = 0.0;
         const double re = c.re_v;
-        for_neighbors<DIM>(g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, re,
+        for_neighbors<DIM>(nbs, g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, re,
                            [&](int q, double dist, double, double, double) {
                                const double w = wfun(dist, re);
                                tx += (cur.vx[q] - vx) * w;
@@ -126,6 +142,7 @@ hipError_t launch_visc_move(int dim, const GridView& g, const SceneTab& tab, int
 
 template <int DIM, bool MULTI>
 __global__ void __launch_bounds__(kMaxTuneBlock) k_coll_count(GridView g0, Consts c0, SceneTab tab, int n, Hot h, int* __restrict__ cnt) {
+    NB_STAGE(DIM, g0, h, n);
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     GridView g = g0;
@@ -136,7 +153,7 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_coll_count(GridView g0, Const
     if (h.type[s] == MPS_FLUID) {
         const double re = c.dc < c.re_max ? c.dc : c.re_max;
         const double zi = DIM == 3 ? h.pz[s] : 0.0;
-        for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
+        for_neighbors<DIM>(nbs, g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
                            [&](int q, double, double, double, double) {
                                if (!(h.type[q] == MPS_FLUID && h.id[q] >= id)) k++;
                            });
@@ -315,6 +332,7 @@ template <int DIM, bool MULTI>
 __global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g0, Consts c0, SceneTab tab, int n, Hot h, Cold cold,
                                                       int8_t* __restrict__ bc_s, double* __restrict__ b_s,
                                                       int* __restrict__ parent, NbList nb) {
+    NB_STAGE(DIM, g0, h, n);
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     GridView g = g0;
@@ -330,7 +348,7 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g0, Consts
         const double re = rs > rp ? rs : rp;
         double ns = 0.0, np = 0.0;
         const double zi = DIM == 3 ? h.pz[s] : 0.0;
-        for_neighbors<DIM>(g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
+        for_neighbors<DIM>(nbs, g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
                            [&](int q, double dist, double, double, double) {
                                if (cnt < nb.cap) {
                                    const size_t e = (size_t) cnt * nb.stride + s;

@@ -443,15 +443,28 @@ __device__ __forceinline__ void for_listed(const NbList& L, const GridView& g, c
         for_neighbors<DIM>(g, px, py, pz, key, s, xi, yi, zi, re, f);
         return;
     }
-    for (int k = 0; k < m; k++) {
-        const size_t e = (size_t) k * L.stride + s;
-        const int q = L.q[e];
-        const double dist = L.d[e];
-        if (!(dist < re)) continue;
-        const double dx = px[q] - xi;
-        const double dy = py[q] - yi;
-        const double dz = DIM == 3 ? pz[q] - zi : 0.0;
-        f(q, dist, dx, dy, dz);
+    // entries four at a time: their loads are independent of each other
+    for (int k0 = 0; k0 < m; k0 += 4) {
+        int qq[4];
+        double dd[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int k = k0 + i < m ? k0 + i : m - 1;
+            const size_t e = (size_t) k * L.stride + s;
+            qq[i] = L.q[e];
+            dd[i] = L.d[e];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (k0 + i >= m) break;
+            const int q = qq[i];
+            const double dist = dd[i];
+            if (!(dist < re)) continue;
+            const double dx = px[q] - xi;
+            const double dy = py[q] - yi;
+            const double dz = DIM == 3 ? pz[q] - zi : 0.0;
+            f(q, dist, dx, dy, dz);
+        }
     }
 }
 

@@ -416,6 +416,125 @@ __device__ __forceinline__ void for_neighbors(const NbStage<DIM>& st, const Grid
     }
 }
 
+// 3D variant: the LDS copy holds FP32 offsets from the block's first slot
+// (12 bytes a slot instead of 24, so 9 ranges fit with room for occupancy).
+// They only skip candidates that are certainly beyond re: slack covers the
+// rounding of the offsets (half an FP32 ulp of the largest offset, bounded by
+// the grid's diagonal) and of the FP32 distance; every candidate not skipped
+// is tested in FP64 from global memory exactly as before.
+struct NbStageF {
+    int ok;
+    const int* meta;  // as NbStage<3>
+    const float* x;
+    const float* y;
+    const float* z;
+    double ox, oy, oz;  // the block's origin (its first slot)
+    float slack;        // absolute, added to re
+};
+
+__device__ __forceinline__ NbStageF nb_stage_f(const GridView& g, const Hot& h, int n, int cap, int* meta, float* sx,
+                                              float* sy, float* sz) {
+    constexpr int NR = 9;
+    int* first = meta + 1 + NR;
+    int* base = meta + 1 + 2 * NR;
+    const int s0 = blockIdx.x * blockDim.x;
+    if (threadIdx.x == 0) {
+        int ok = 0;
+        const int s1 = (s0 + (int) blockDim.x < n ? s0 + (int) blockDim.x : n) - 1;
+        if (s0 < n && !g.flat) {
+            const int k_lo = h.key[s0] - g.key0, k_hi = h.key[s1] - g.key0;
+            if (k_lo >= 0 && k_hi < g.ncell) {
+                const int* cs = g.cs + g.key0;
+                const int nx = g.nx, nxy = g.nx * g.ny;
+                int tot = 0;
+                for (int r = 0; r < NR; r++) {
+                    const int dy = r % 3 - 1, dz = r / 3 - 1;
+                    int lo = k_lo - 1 + dy * nx + dz * nxy, hi = k_hi + 1 + dy * nx + dz * nxy;
+                    lo = lo < 0 ? 0 : lo;
+                    hi = hi > g.ncell - 1 ? g.ncell - 1 : hi;
+                    const int a = lo <= hi ? cs[lo] : 0, b = lo <= hi ? cs[hi + 1] : 0;
+                    first[r] = a;
+                    base[r] = tot;
+                    meta[1 + r] = tot - a;
+                    tot += b - a;
+                }
+                base[NR] = tot;
+                ok = tot <= cap ? 1 : 0;
+            }
+        }
+        meta[0] = ok;
+    }
+    __syncthreads();
+    const int ok = meta[0];
+    const int o = s0 < n ? s0 : 0;
+    const double ox = h.px[o], oy = h.py[o], oz = h.pz[o];
+    if (ok) {
+        const int tot = base[NR];
+        for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+            int r = 0;
+            while (r + 1 < NR && base[r + 1] <= i) r++;
+            const int q = first[r] + (i - base[r]);
+            sx[i] = (float) (h.px[q] - ox);
+            sy[i] = (float) (h.py[q] - oy);
+            sz[i] = (float) (h.pz[q] - oz);
+        }
+    }
+    __syncthreads();
+    // every offset is at most the grid's diagonal; an FP32 offset is off by
+    // half an ulp of that, a difference of two by one ulp per axis
+    const double diag = g.len * sqrt((double) g.nx * g.nx + (double) g.ny * g.ny + (double) g.nz * g.nz);
+    const float slack = (float) (4.0 * diag * 1.2e-7);
+    return NbStageF{ok, meta, sx, sy, sz, ox, oy, oz, slack};
+}
+
+template <int DIM, typename F>
+__device__ __forceinline__ void for_neighbors(const NbStageF& st, const GridView& g, const double* __restrict__ px,
+                                              const double* __restrict__ py, const double* __restrict__ pz, int key,
+                                              int s, double xi, double yi, double zi, double re, F&& f) {
+    static_assert(DIM == 3, "the FP32 staging is the 3D walk's");
+    if (!st.ok) {
+        for_neighbors<3>(g, px, py, pz, key, s, xi, yi, zi, re, f);
+        return;
+    }
+    key -= g.key0;
+    const int* __restrict__ cs = g.cs + g.key0;
+    const double re2 = re * re * (1.0 + 1e-12);
+    const float xf = (float) (xi - st.ox), yf = (float) (yi - st.oy), zf = (float) (zi - st.oz);
+    // the FP32 distance itself is within a few ulp (relative 1e-6 covers it)
+    const float rf = ((float) re + st.slack) * 1.000001f;
+    const float rf2 = rf * rf;
+    const char* __restrict__ bx = reinterpret_cast<const char*>(px);
+    const char* __restrict__ by = reinterpret_cast<const char*>(py);
+    const char* __restrict__ bz = reinterpret_cast<const char*>(pz);
+    const int ix = key % g.nx;
+    const int iy = (key / g.nx) % g.ny;
+    const int iz = key / (g.nx * g.ny);
+    for (int jx = ix - 1; jx <= ix + 1; jx++) {
+        if (jx < 0 || jx >= g.nx) continue;
+        for (int jy = iy - 1; jy <= iy + 1; jy++) {
+            if (jy < 0 || jy >= g.ny) continue;
+            for (int jz = iz - 1; jz <= iz + 1; jz++) {
+                if (jz < 0 || jz >= g.nz) continue;
+                const int k = jx + jy * g.nx + jz * g.nx * g.ny;
+                const int off = st.meta[1 + (jy - iy + 1) + 3 * (jz - iz + 1)];
+                const int b = cs[k], e = cs[k + 1];
+                for (int q = b; q < e; q++) {
+                    const float dxf = st.x[q + off] - xf, dyf = st.y[q + off] - yf, dzf = st.z[q + off] - zf;
+                    if (dxf * dxf + dyf * dyf + dzf * dzf > rf2) continue;
+                    const unsigned boff = (unsigned) q * 8u;
+                    const double dx = *reinterpret_cast<const double*>(bx + boff) - xi;
+                    const double dy = *reinterpret_cast<const double*>(by + boff) - yi;
+                    const double dz = *reinterpret_cast<const double*>(bz + boff) - zi;
+                    const double d2 = (dx * dx + dy * dy) + dz * dz;
+                    if (d2 > re2) continue;
+                    const double dist = sqrt(d2);
+                    if (q != s && dist < re) f(q, dist, dx, dy, dz);
+                }
+            }
+        }
+    }
+}
+
 // Neighbour lists of the pressure stages.  The boundary-condition pass walks
 // the cells once at radius max(re_s, re_p) and records each slot's neighbours
 // in the walk's (the reference's) order with their distances; the later walks

@@ -7,6 +7,7 @@
 #pragma clang fp contract(off)
 
 #include <algorithm>
+#include <type_traits>
 
 #include "mps_internal.hpp"
 
@@ -53,20 +54,40 @@ __device__ __forceinline__ double ndens(const GridView& g, const Hot& h, int s, 
     return nd;
 }
 
-// LDS staging of a block's candidate positions (see nb_stage): capacity in
-// slots per block; every thread of the block declares and fills it first.
-// 2D: 3 ranges of ~280 slots for 256-thread blocks.  3D is not staged: its 9
-// ranges of ~320 slots (72 KB in FP64) leave two blocks per CU, and the
-// measured walks were slower than from global memory.
+// LDS staging of a block's candidate positions (see nb_stage, nb_stage_f):
+// capacity in slots per block; every thread of the block declares and fills
+// it first.  2D: FP64, 3 ranges of ~280 slots for 256-thread blocks (24 KB).
+// 3D: FP32 offsets, 9 ranges of ~320 slots (36 KB); in FP64 (72 KB) two
+// blocks fit a CU and the walks were slower than from global memory.
 template <int DIM>
-constexpr int kStageCap = DIM == 2 ? 1536 : 0;
+struct NbStageOf {
+    using type = NbStage<2>;
+};
+template <>
+struct NbStageOf<3> {
+    using type = NbStageF;
+};
+template <int DIM>
+constexpr int kStageCap = DIM == 2 ? 1536 : 3072;
+template <int DIM>
+using StageT = typename std::conditional<DIM == 2, double, float>::type;
+
+template <int DIM>
+__device__ __forceinline__ typename NbStageOf<DIM>::type nb_stage_any(const GridView& g, const Hot& h, int n,
+                                                                      int* meta, StageT<DIM>* sx, StageT<DIM>* sy,
+                                                                      StageT<DIM>* sz) {
+    if constexpr (DIM == 2)
+        return nb_stage<2>(g, h, n, kStageCap<2>, meta, sx, sy, sz);
+    else
+        return nb_stage_f(g, h, n, kStageCap<3>, meta, sx, sy, sz);
+}
 
 #define NB_STAGE(DIM, GRID, HOT, N)                                                                        \
     __shared__ int s_nb_meta[nb_stage_meta_ints<DIM>()];                                                 \
-    __shared__ double s_nb_x[kStageCap<DIM> > 0 ? kStageCap<DIM> : 1];                                  \
-    __shared__ double s_nb_y[kStageCap<DIM> > 0 ? kStageCap<DIM> : 1];                                  \
-    __shared__ double s_nb_z[DIM == 3 && kStageCap<DIM> > 0 ? kStageCap<DIM> : 1];                      \
-    const NbStage<DIM> nbs = nb_stage<DIM>(GRID, HOT, N, kStageCap<DIM>, s_nb_meta, s_nb_x, s_nb_y, s_nb_z)
+    __shared__ StageT<DIM> s_nb_x[kStageCap<DIM>];                                                       \
+    __shared__ StageT<DIM> s_nb_y[kStageCap<DIM>];                                                       \
+    __shared__ StageT<DIM> s_nb_z[DIM == 3 ? kStageCap<DIM> : 1];                                        \
+    const auto nbs = nb_stage_any<DIM>(GRID, HOT, N, s_nb_meta, s_nb_x, s_nb_y, s_nb_z)
 
 // ---------------- gravity + viscosity + explicit move (src/mps.cpp:165-215) ----------------
 

@@ -61,7 +61,8 @@ int main(int argc, char** argv) {
         custom = true;
     }
     if (const char* w = std::getenv("MPS_AMD_WARM_START")) {
-        o.warm_start = std::atoi(w) ? 1 : 0;
+        const int v = std::atoi(w);
+        o.warm_start = v == 2 ? 2 : (v ? 1 : 0);  // 2: extrapolated from the last two solutions
         custom = true;
     }
     if (const char* m = std::getenv("MPS_AMD_PRECONDITIONER")) {

@@ -686,27 +686,43 @@ template <bool MULTI>
 __global__ void __launch_bounds__(kBlock) k_vmax(int n, Hot h, int dim3d, const int16_t* __restrict__ sid,
                                                  unsigned long long* __restrict__ out) {
     __shared__ double red[kBlock / 64];
+    if constexpr (MULTI) {
+        // One slot per thread.  A scene's slots are contiguous (slots are
+        // sorted by key and every scene owns one key range), so a wavefront
+        // nearly always holds one scene: then one atomic for the wavefront,
+        // else one per lane that carries a value.
+        const int s = blockIdx.x * blockDim.x + threadIdx.x;
+        double v = 0.0;
+        int si = -1;
+        if (s < n && h.type[s] == MPS_FLUID) {
+            const double a = h.vx[s], b = h.vy[s], cz = dim3d ? h.vz[s] : 0.0;
+            v = sqrt((a * a + b * b) + cz * cz);
+            si = sid[h.id[s]];
+        }
+        int lo = si < 0 ? 0x7fffffff : si, hi = si;
+        double m = v;
+        for (int o = 32; o > 0; o >>= 1) {
+            const int a = __shfl_xor(lo, o, 64), b = __shfl_xor(hi, o, 64);
+            const double u = __shfl_xor(m, o, 64);
+            lo = a < lo ? a : lo;
+            hi = b > hi ? b : hi;
+            m = u > m ? u : m;
+        }
+        if (hi < 0) return;  // no fluid slot in this wavefront (uniform)
+        if (lo == hi) {
+            if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(&out[hi], (unsigned long long) __double_as_longlong(m));
+        } else if (v > 0.0) {
+            atomicMax(&out[si], (unsigned long long) __double_as_longlong(v));
+        }
+        return;
+    }
     // grid-stride: a few hundred blocks, so a few hundred atomics on the result
     double v = 0.0;
-    int si = -1;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
         if (h.type[s] != MPS_FLUID) continue;
         const double a = h.vx[s], b = h.vy[s], cz = dim3d ? h.vz[s] : 0.0;
         const double u = sqrt((a * a + b * b) + cz * cz);
-        if constexpr (MULTI) {
-            // per-scene maxima: flush when the scene changes along the stride
-            const int ss = sid[h.id[s]];
-            if (ss != si) {
-                if (si >= 0 && v > 0.0) atomicMax(&out[si], (unsigned long long) __double_as_longlong(v));
-                si = ss;
-                v = 0.0;
-            }
-        }
         v = u > v ? u : v;
-    }
-    if constexpr (MULTI) {
-        if (si >= 0 && v > 0.0) atomicMax(&out[si], (unsigned long long) __double_as_longlong(v));
-        return;
     }
     for (int o = 32; o > 0; o >>= 1) {
         const double u = __shfl_down(v, o, 64);
@@ -727,7 +743,7 @@ hipError_t launch_courant(int dim, int n, const Hot& h, const SceneTab& tab, int
     if (e != hipSuccess || n == 0) return e;
     const dim3 grid(std::min(512u, grid_for(n, kBlock).x));
     if (tab.g)
-        hipLaunchKernelGGL(k_vmax<true>, grid, dim3(kBlock), 0, st, n, h, dim == 3 ? 1 : 0, tab.sid,
+        hipLaunchKernelGGL(k_vmax<true>, grid_for(n, kBlock), dim3(kBlock), 0, st, n, h, dim == 3 ? 1 : 0, tab.sid,
                            reinterpret_cast<unsigned long long*>(vmax));
     else
         hipLaunchKernelGGL(k_vmax<false>, grid, dim3(kBlock), 0, st, n, h, dim == 3 ? 1 : 0, tab.sid,

@@ -275,15 +275,27 @@ struct NbStage {
 
 template <int DIM>
 constexpr int nb_stage_meta_ints() {
-    return 1 + 3 * NbStage<DIM>::NR + 1;
+    return 1 + 3 * NbStage<DIM>::NR + 2;
+}
+
+// The grid a block's staging uses: the handle's, or for several scenes the
+// scene of the block's first and last slot (no staging if they differ).
+__device__ __forceinline__ bool nb_block_grid(const GridView& g0, const SceneTab* tab, const Hot& h, int s0, int s1,
+                                              GridView& g) {
+    g = g0;
+    if (!tab) return true;
+    const int a = tab->sid[h.id[s0]], b = tab->sid[h.id[s1]];
+    if (a != b) return false;
+    g = tab->g[a];
+    return true;
 }
 
 // Every thread of the block must call it (it holds two block barriers when
 // cap > 0).  meta: nb_stage_meta_ints<DIM>() ints of LDS; sx, sy, sz: cap
 // doubles each (sz unused in 2D).  cap == 0: no staging, no barrier.
 template <int DIM>
-__device__ __forceinline__ NbStage<DIM> nb_stage(const GridView& g, const Hot& h, int n, int cap, int* meta,
-                                                 double* sx, double* sy, double* sz) {
+__device__ __forceinline__ NbStage<DIM> nb_stage(const GridView& g0, const SceneTab* tab, const Hot& h, int n,
+                                                 int cap, int* meta, double* sx, double* sy, double* sz) {
     constexpr int NR = NbStage<DIM>::NR;
     if (cap <= 0) return NbStage<DIM>{0, meta, sx, sy, sz};
     int* first = meta + 1 + NR;
@@ -292,7 +304,8 @@ __device__ __forceinline__ NbStage<DIM> nb_stage(const GridView& g, const Hot& h
         int ok = 0;
         const int s0 = blockIdx.x * blockDim.x;
         const int s1 = (s0 + (int) blockDim.x < n ? s0 + (int) blockDim.x : n) - 1;
-        if (s0 < n && !(DIM == 3 && g.flat)) {
+        GridView g;
+        if (s0 < n && nb_block_grid(g0, tab, h, s0, s1, g) && !(DIM == 3 && g.flat)) {
             const int k_lo = h.key[s0] - g.key0, k_hi = h.key[s1] - g.key0;
             if (k_lo >= 0 && k_hi < g.ncell) {
                 const int* cs = g.cs + g.key0;
@@ -432,16 +445,18 @@ struct NbStageF {
     float slack;        // absolute, added to re
 };
 
-__device__ __forceinline__ NbStageF nb_stage_f(const GridView& g, const Hot& h, int n, int cap, int* meta, float* sx,
-                                              float* sy, float* sz) {
+__device__ __forceinline__ NbStageF nb_stage_f(const GridView& g0, const SceneTab* tab, const Hot& h, int n, int cap,
+                                              int* meta, float* sx, float* sy, float* sz) {
     constexpr int NR = 9;
     int* first = meta + 1 + NR;
     int* base = meta + 1 + 2 * NR;
     const int s0 = blockIdx.x * blockDim.x;
     if (threadIdx.x == 0) {
         int ok = 0;
+        float slack = 0.0f;
         const int s1 = (s0 + (int) blockDim.x < n ? s0 + (int) blockDim.x : n) - 1;
-        if (s0 < n && !g.flat) {
+        GridView g;
+        if (s0 < n && nb_block_grid(g0, tab, h, s0, s1, g) && !g.flat) {
             const int k_lo = h.key[s0] - g.key0, k_hi = h.key[s1] - g.key0;
             if (k_lo >= 0 && k_hi < g.ncell) {
                 const int* cs = g.cs + g.key0;
@@ -460,9 +475,16 @@ __device__ __forceinline__ NbStageF nb_stage_f(const GridView& g, const Hot& h, 
                 }
                 base[NR] = tot;
                 ok = tot <= cap ? 1 : 0;
+                // every offset is at most the grid's diagonal; an FP32 offset
+                // is off by half an ulp of that, a difference of two by one
+                // ulp per axis
+                const double diag =
+                    g.len * sqrt((double) g.nx * g.nx + (double) g.ny * g.ny + (double) g.nz * g.nz);
+                slack = (float) (4.0 * diag * 1.2e-7);
             }
         }
         meta[0] = ok;
+        meta[1 + 3 * NR + 1] = __float_as_int(slack);
     }
     __syncthreads();
     const int ok = meta[0];
@@ -480,11 +502,7 @@ __device__ __forceinline__ NbStageF nb_stage_f(const GridView& g, const Hot& h, 
         }
     }
     __syncthreads();
-    // every offset is at most the grid's diagonal; an FP32 offset is off by
-    // half an ulp of that, a difference of two by one ulp per axis
-    const double diag = g.len * sqrt((double) g.nx * g.nx + (double) g.ny * g.ny + (double) g.nz * g.nz);
-    const float slack = (float) (4.0 * diag * 1.2e-7);
-    return NbStageF{ok, meta, sx, sy, sz, ox, oy, oz, slack};
+    return NbStageF{ok, meta, sx, sy, sz, ox, oy, oz, __int_as_float(meta[1 + 3 * NR + 1])};
 }
 
 template <int DIM, typename F>

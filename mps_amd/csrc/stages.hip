@@ -73,13 +73,14 @@ template <int DIM>
 using StageT = typename std::conditional<DIM == 2, double, float>::type;
 
 template <int DIM>
-__device__ __forceinline__ typename NbStageOf<DIM>::type nb_stage_any(const GridView& g, const Hot& h, int n,
-                                                                      int* meta, StageT<DIM>* sx, StageT<DIM>* sy,
+__device__ __forceinline__ typename NbStageOf<DIM>::type nb_stage_any(const GridView& g, const SceneTab* tab,
+                                                                      const Hot& h, int n, int* meta,
+                                                                      StageT<DIM>* sx, StageT<DIM>* sy,
                                                                       StageT<DIM>* sz) {
     if constexpr (DIM == 2)
-        return nb_stage<2>(g, h, n, kStageCap<2>, meta, sx, sy, sz);
+        return nb_stage<2>(g, tab, h, n, kStageCap<2>, meta, sx, sy, sz);
     else
-        return nb_stage_f(g, h, n, kStageCap<3>, meta, sx, sy, sz);
+        return nb_stage_f(g, tab, h, n, kStageCap<3>, meta, sx, sy, sz);
 }
 
 #define NB_STAGE(DIM, GRID, HOT, N)                                                                        \
@@ -87,7 +88,7 @@ __device__ __forceinline__ typename NbStageOf<DIM>::type nb_stage_any(const Grid
     __shared__ StageT<DIM> s_nb_x[kStageCap<DIM>];                                                       \
     __shared__ StageT<DIM> s_nb_y[kStageCap<DIM>];                                                       \
     __shared__ StageT<DIM> s_nb_z[DIM == 3 ? kStageCap<DIM> : 1];                                        \
-    const auto nbs = nb_stage_any<DIM>(GRID, HOT, N, s_nb_meta, s_nb_x, s_nb_y, s_nb_z)
+    const auto nbs = nb_stage_any<DIM>(GRID, MULTI ? &tab : nullptr, HOT, N, s_nb_meta, s_nb_x, s_nb_y, s_nb_z)
 
 // ---------------- gravity + viscosity + explicit move (src/mps.cpp:165-215) ----------------
 
@@ -708,11 +709,33 @@ __global__ void __launch_bounds__(kBlock) k_vmax(int n, Hot h, int dim3d, const 
             hi = b > hi ? b : hi;
             m = u > m ? u : m;
         }
-        if (hi < 0) return;  // no fluid slot in this wavefront (uniform)
-        if (lo == hi) {
-            if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(&out[hi], (unsigned long long) __double_as_longlong(m));
-        } else if (v > 0.0) {
-            atomicMax(&out[si], (unsigned long long) __double_as_longlong(v));
+        // the block's wavefronts: one atomic if they all hold the same scene
+        __shared__ int s_sc[kBlock / 64];
+        __shared__ double s_m[kBlock / 64];
+        const int wv = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            s_sc[wv] = hi < 0 ? -1 : (lo == hi ? hi : -2);  // -1 none, -2 mixed
+            s_m[wv] = m;
+        }
+        __syncthreads();
+        int bs = -1;
+        double bm = 0.0;
+        for (int k = 0; k < (int) (blockDim.x >> 6); k++) {
+            const int c = s_sc[k];
+            if (c == -1) continue;
+            bs = (bs == -1 || bs == c) ? c : -2;
+            bm = s_m[k] > bm ? s_m[k] : bm;
+        }
+        if (bs >= 0) {
+            if (threadIdx.x == 0 && bm > 0.0) atomicMax(&out[bs], (unsigned long long) __double_as_longlong(bm));
+        } else if (bs == -2) {
+            if (hi < 0) return;
+            if (lo == hi) {
+                if ((threadIdx.x & 63) == 0 && m > 0.0)
+                    atomicMax(&out[hi], (unsigned long long) __double_as_longlong(m));
+            } else if (v > 0.0) {
+                atomicMax(&out[si], (unsigned long long) __double_as_longlong(v));
+            }
         }
         return;
     }

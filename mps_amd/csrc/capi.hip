@@ -810,7 +810,11 @@ static int create_impl(int32_t nscene, const mps_settings* sset, const int32_t* 
         }
         ntot += counts[k];
     }
-    if (ntot > 0x7fffffff / 4 || (ntot > 0 && (!id || !type || !pos || !vel))) {
+    if (ntot > 0x7fffffff / 4) {  // byte offsets of 8-byte fields must fit 32 bits
+        g_create_error = "too many particles for one handle (limit 536870911)";
+        return MPS_E_INVALID;
+    }
+    if (ntot > 0 && (!id || !type || !pos || !vel)) {
         g_create_error = "invalid arguments";
         return MPS_E_INVALID;
     }

@@ -149,7 +149,8 @@ template <int DIM, typename F>
 __device__ __forceinline__ void scan_cell(const double* __restrict__ px, const double* __restrict__ py,
                                           const double* __restrict__ pz, int b, int e, int s, double xi,
                                           double yi, double zi, double re, double re2, F& f) {
-    // 32-bit byte offsets (slot counts stay below 2^28): one offset register
+    // 32-bit byte offsets (handles hold fewer than 2^29 particles, so 8 q
+    // fits): one offset register
     // serves the three arrays, with their bases in scalar registers
     const char* __restrict__ bx = reinterpret_cast<const char*>(px);
     const char* __restrict__ by = reinterpret_cast<const char*>(py);

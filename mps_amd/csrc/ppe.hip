@@ -31,6 +31,7 @@
 #pragma clang fp contract(off)
 
 #include <math.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <vector>
@@ -675,16 +676,27 @@ struct CgWork {
     float* u32 = nullptr;
     MgWork* mg = nullptr;     // multigrid preconditioner (single segment)
     int mg_on = 0;            // the current solve uses it
-    hipGraphExec_t graph[2] = {nullptr, nullptr};  // chunk graphs: FP64 CG, FP32 inner CG
-    int graph_chunk[2] = {0, 0};
+    // chunk graphs of the FP64 CG [0] and the FP32 inner CG [1], one per
+    // chunk length (the first chunk of a solve is sized by prediction)
+    static constexpr int kGraphSlots = 8;
+    hipGraphExec_t graph[2][kGraphSlots] = {};
+    int graph_chunk[2][kGraphSlots] = {};
+    int graph_next[2] = {0, 0};  // slot to replace next
     const void* graph_key[2] = {nullptr, nullptr};
+    static constexpr int kPredSolves = 4;
+    bool predict = true;  // MPS_AMD_CG_PREDICT=0: fixed chunks of the check interval
+    int pred[2][kPredSolves] = {};  // last step's iterations per solve of a step
     int graph_idx16[2] = {-1, -1};
     int graph_blocks[2] = {0, 0};
     int graph_mg[2] = {-2, -2};  // multigrid generation captured (-1: none)
     Seg seg() const { return Seg{nseg, blk_off, row_off}; }
 };
 
-CgWork* cg_create() { return new CgWork(); }
+CgWork* cg_create() {
+    CgWork* cw = new CgWork();
+    if (const char* v = getenv("MPS_AMD_CG_PREDICT")) cw->predict = atoi(v) != 0;
+    return cw;
+}
 
 static void cg_free_vectors(CgWork* cw) {
     void* ptrs[] = {cw->y,   cw->bh,  cw->r64, cw->p64, cw->s64, cw->w64, cw->r32,
@@ -703,7 +715,8 @@ void cg_destroy(CgWork* cw) {
     hipHostFree(cw->h_blk);
     hipHostFree(cw->h_row);
     for (int k = 0; k < 2; k++)
-        if (cw->graph[k]) hipGraphExecDestroy(cw->graph[k]);
+        for (int j = 0; j < CgWork::kGraphSlots; j++)
+            if (cw->graph[k][j]) hipGraphExecDestroy(cw->graph[k][j]);
     mg_destroy(cw->mg);
     delete cw;
 }
@@ -740,10 +753,12 @@ hipError_t cg_reserve(CgWork* cw, int rows) {
         (e = hipMalloc(&cw->u32, f)))
         return e;
     cw->cap = cap;
-    for (int k = 0; k < 2; k++) {
-        if (cw->graph[k]) hipGraphExecDestroy(cw->graph[k]);
-        cw->graph[k] = nullptr;
-    }
+    for (int k = 0; k < 2; k++)
+        for (int j = 0; j < CgWork::kGraphSlots; j++) {
+            if (cw->graph[k][j]) hipGraphExecDestroy(cw->graph[k][j]);
+            cw->graph[k][j] = nullptr;
+            cw->graph_chunk[k][j] = 0;
+        }
     return hipSuccess;
 }
 
@@ -817,35 +832,53 @@ static bool all_done(const CgWork* cw) {
     return true;
 }
 
+// Runs chunks of CG iterations (each a captured graph, or enqueued directly)
+// until every segment is done.  `first` > 0: length of the first chunk (a
+// prediction of the solve's iterations), then chunks of 2; else chunks of
+// the check interval.  Iterations past convergence return at once.
 template <typename T, typename IDX, bool WEIGHTED>
 static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* col, const T* val, T* y, T* r, T* p,
-                              T* s, T* w, const mps_solver_options& o, hipStream_t st) {
-    const int chunk = std::max(2, (o.check_interval + 1) & ~1);
+                              T* s, T* w, const mps_solver_options& o, int first, hipStream_t st) {
+    const int interval = std::max(2, (o.check_interval + 1) & ~1);
     const int idx16 = sizeof(IDX) == 2 ? 1 : 0;
     const size_t sbytes = sizeof(CgScalars) * (size_t) cw->nseg;
     hipError_t e;
-    for (;;) {
+    if (o.use_graph) {
+        const int mgk = cw->mg_on ? mg_generation(cw->mg) : -1;
+        if (cw->graph_key[which] != (const void*) val || cw->graph_idx16[which] != idx16 ||
+            cw->graph_blocks[which] != cw->total_blocks || cw->graph_mg[which] != mgk) {
+            for (int j = 0; j < CgWork::kGraphSlots; j++) {
+                if (cw->graph[which][j]) hipGraphExecDestroy(cw->graph[which][j]);
+                cw->graph[which][j] = nullptr;
+                cw->graph_chunk[which][j] = 0;
+            }
+            cw->graph_key[which] = val;
+            cw->graph_idx16[which] = idx16;
+            cw->graph_blocks[which] = cw->total_blocks;
+            cw->graph_mg[which] = mgk;
+        }
+    }
+    for (int round = 0;; round++) {
+        const int chunk = round == 0 ? (first > 0 ? first : interval) : (first > 0 ? 2 : interval);
         if (o.use_graph) {
-            const int mgk = cw->mg_on ? mg_generation(cw->mg) : -1;
-            if (!cw->graph[which] || cw->graph_chunk[which] != chunk || cw->graph_key[which] != (const void*) val ||
-                cw->graph_idx16[which] != idx16 || cw->graph_blocks[which] != cw->total_blocks ||
-                cw->graph_mg[which] != mgk) {
-                if (cw->graph[which]) hipGraphExecDestroy(cw->graph[which]);
-                cw->graph[which] = nullptr;
+            int slot = -1;
+            for (int j = 0; j < CgWork::kGraphSlots; j++)
+                if (cw->graph[which][j] && cw->graph_chunk[which][j] == chunk) slot = j;
+            if (slot < 0) {
+                slot = cw->graph_next[which];
+                cw->graph_next[which] = (slot + 1) % CgWork::kGraphSlots;
+                if (cw->graph[which][slot]) hipGraphExecDestroy(cw->graph[which][slot]);
+                cw->graph[which][slot] = nullptr;
                 hipGraph_t gph;
                 if ((e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal))) return e;
                 enqueue_chunk<T, IDX, WEIGHTED>(cw, A, col, val, y, r, p, s, w, chunk, st);
                 if ((e = hipStreamEndCapture(st, &gph))) return e;
-                e = hipGraphInstantiate(&cw->graph[which], gph, nullptr, nullptr, 0);
+                e = hipGraphInstantiate(&cw->graph[which][slot], gph, nullptr, nullptr, 0);
                 hipGraphDestroy(gph);
                 if (e) return e;
-                cw->graph_chunk[which] = chunk;
-                cw->graph_key[which] = val;
-                cw->graph_idx16[which] = idx16;
-                cw->graph_blocks[which] = cw->total_blocks;
-                cw->graph_mg[which] = mgk;
+                cw->graph_chunk[which][slot] = chunk;
             }
-            if ((e = hipGraphLaunch(cw->graph[which], st))) return e;
+            if ((e = hipGraphLaunch(cw->graph[which][slot], st))) return e;
         } else {
             enqueue_chunk<T, IDX, WEIGHTED>(cw, A, col, val, y, r, p, s, w, chunk, st);
         }
@@ -883,11 +916,25 @@ static void launch_residual(CgWork* cw, const Sell& A, hipStream_t st) {
                            cw->r64, cw->part);
 }
 
+// `solve`: index of this solve within the step (0 for the FP64 CG, the
+// refinement for the inner CG); the last step's iteration count of the same
+// solve sizes the first chunk, one less, rounded down to even.
 template <typename T, bool WEIGHTED>
-static hipError_t run_cg(CgWork* cw, int which, const Sell& A, const T* val, T* y, T* r, T* p, T* s, T* w,
+static hipError_t run_cg(CgWork* cw, int which, int solve, const Sell& A, const T* val, T* y, T* r, T* p, T* s, T* w,
                          const mps_solver_options& o, hipStream_t st) {
-    if (A.use16) return run_cg_loop<T, short, WEIGHTED>(cw, which, A, A.dcol, val, y, r, p, s, w, o, st);
-    return run_cg_loop<T, int, WEIGHTED>(cw, which, A, A.col, val, y, r, p, s, w, o, st);
+    const int slot = solve < CgWork::kPredSolves ? solve : CgWork::kPredSolves - 1;
+    const int pred = cw->predict ? cw->pred[which][slot] : 0;
+    const int first = pred >= 4 ? std::min(((pred - 1) / 2) * 2, 64) : 0;
+    hipError_t e;
+    if (A.use16)
+        e = run_cg_loop<T, short, WEIGHTED>(cw, which, A, A.dcol, val, y, r, p, s, w, o, first, st);
+    else
+        e = run_cg_loop<T, int, WEIGHTED>(cw, which, A, A.col, val, y, r, p, s, w, o, first, st);
+    if (e) return e;
+    int it = 0;  // the segment that took longest sets the chunk
+    for (int k = 0; k < cw->nseg; k++) it = std::max(it, cw->host[k].iter);
+    cw->pred[which][slot] = it;
+    return hipSuccess;
 }
 
 hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double* b_s, const double* pprev_id,
@@ -967,7 +1014,7 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
                 cw->host[s] = z;
             }
             if ((e = upload_scalars(cw, st))) return e;
-            if ((e = run_cg<double, true>(cw, 0, A, A.val64, cw->y, cw->r64, cw->p64, cw->s64, cw->w64, o, st)))
+            if ((e = run_cg<double, true>(cw, 0, 0, A, A.val64, cw->y, cw->r64, cw->p64, cw->s64, cw->w64, o, st)))
                 return e;
             for (int s = 0; s < nseg; s++) {
                 if (conv[s]) continue;
@@ -978,7 +1025,7 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
     } else {
         const int max_ref = o.max_refinements > 0 ? o.max_refinements : 20;
         const double eta = o.inner_tolerance > 0 ? o.inner_tolerance : 1e-4;
-        for (;;) {
+        for (int round = 0;; round++) {
             bool active = false;
             for (int s = 0; s < nseg; s++)
                 active = active || (!conv[s] && refinements[s] < max_ref && total_it[s] < max_it[s]);
@@ -1003,7 +1050,8 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
                 cw->host[s] = z;
             }
             if ((e = upload_scalars(cw, st))) return e;
-            if ((e = run_cg<float, false>(cw, 1, A, A.val32, cw->e32, cw->r32, cw->p32, cw->s32, cw->w32, o, st)))
+            if ((e = run_cg<float, false>(cw, 1, round, A, A.val32, cw->e32, cw->r32, cw->p32, cw->s32, cw->w32, o,
+                                          st)))
                 return e;
             for (int s = 0; s < nseg; s++)
                 if (run[s]) total_it[s] += cw->host[s].iter;

@@ -27,6 +27,7 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "mps_internal.hpp"
@@ -765,6 +766,14 @@ struct MgWork {
     size_t scan_cap = 0;
     MgLevelDesc* desc = nullptr;     // device table of the levels
     MgLevelDesc hdesc[kMgMaxLevels + 1] = {};
+    // what the device copies of the per-level tables last received: the grid
+    // tables depend only on the scenes, so most setups upload nothing (an
+    // upload from pageable memory waits for the stream)
+    std::vector<MgDims> up_sdims[kMgMaxLevels + 1];
+    std::vector<int> up_goff[kMgMaxLevels + 1];
+    MgLevelDesc up_desc[kMgMaxLevels + 1] = {};
+    bool up_desc_valid = false;
+    int* hpin = nullptr;  // pinned host words for the level-size reads
     int tail = 1;                    // first level handled by the tail kernel
     // smoother weight, coarse-correction factor and coarsest sweeps, chosen by
     // measurement: 2D (2M dam break, 64-scene ensemble) 1.0, 1.4, 1; 3D
@@ -820,6 +829,7 @@ void mg_destroy(MgWork* m) {
     void* ps[] = {m->nd,  m->err,   m->cst1,  m->head,     m->hscan, m->rkey,
                   m->res0, m->xx0, m->flag, m->pscan, m->scan_tmp, m->desc};
     for (void* p : ps) hipFree(p);
+    if (m->hpin) hipHostFree(m->hpin);
     delete m;
 }
 
@@ -867,10 +877,24 @@ static hipError_t mg_scan(MgWork* m, const int* in, int* out, int len, hipStream
     return scan_exclusive(in, out, len, m->scan_tmp, st);
 }
 
-static hipError_t mg_read_int(const int* dev, int* host, hipStream_t st) {
-    hipError_t e = hipMemcpyAsync(host, dev, sizeof(int), hipMemcpyDeviceToHost, st);
-    if (e) return e;
-    return hipStreamSynchronize(st);
+static hipError_t mg_read_int(MgWork* m, const int* dev, int* host, hipStream_t st) {
+    hipError_t e;
+    if (!m->hpin && (e = hipHostMalloc((void**) &m->hpin, 16 * sizeof(int), hipHostMallocDefault))) return e;
+    if ((e = hipMemcpyAsync(m->hpin, dev, sizeof(int), hipMemcpyDeviceToHost, st))) return e;
+    if ((e = hipStreamSynchronize(st))) return e;
+    *host = m->hpin[0];
+    return hipSuccess;
+}
+
+// Host-to-device copy of a small table unless the device already holds it.
+template <typename V>
+static hipError_t mg_upload(V* dev, const std::vector<V>& host, std::vector<V>& last, hipStream_t st) {
+    if (last.size() == host.size() && std::equal(host.begin(), host.end(), last.begin(), [](const V& a, const V& b) {
+            return std::memcmp(&a, &b, sizeof(V)) == 0;
+        }))
+        return hipSuccess;
+    last = host;
+    return hipMemcpyAsync(dev, host.data(), sizeof(V) * host.size(), hipMemcpyHostToDevice, st);
 }
 
 template <typename T, int W, int C>
@@ -924,10 +948,13 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
             m->d_sdims[k] = nullptr;
             m->d_goff[k] = nullptr;
         }
-        for (int k = 0; k <= kMgMaxLevels; k++)
+        for (int k = 0; k <= kMgMaxLevels; k++) {
+            m->up_sdims[k].clear();  // the new buffers hold nothing yet
+            m->up_goff[k].clear();
             if ((e = mg_alloc(&m->d_sdims[k], sizeof(MgDims) * (size_t) ns)) ||
                 (e = mg_alloc(&m->d_goff[k], sizeof(int) * (size_t) (ns + 1))))
                 return e;
+        }
         m->scap = ns;
         m->gen++;
     }
@@ -937,7 +964,7 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
     hipLaunchKernelGGL(k_mg_heads, mg_grid(R), dim3(kBlock), 0, st, R, A.row_slot, h.key, m->head, m->rkey);
     if ((e = mg_scan(m, m->head, m->hscan, R, st))) return e;
     int n1 = 0;
-    if ((e = mg_read_int(m->hscan + R, &n1, st))) return e;
+    if ((e = mg_read_int(m, m->hscan + R, &n1, st))) return e;
     if ((e = mg_reserve_level(m, 1, n1))) return e;
     hipLaunchKernelGGL(k_mg_level1, mg_grid(R), dim3(kBlock), 0, st, R, m->head, m->hscan, m->rkey, A.scene_key0, ns,
                        m->agg[0], m->cst1, m->coord[1], m->scene[1]);
@@ -949,9 +976,7 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
         m->sdims[1][k] = MgDims{gk.nx, gk.ny, gk.flat ? 1 : gk.nz};
         if (gk.nx < 3 || gk.ny < 3 || (!gk.flat && dim == 3 && gk.nz < 3)) return hipErrorInvalidValue;
     }
-    if ((e = hipMemcpyAsync(m->d_sdims[1], m->sdims[1].data(), sizeof(MgDims) * (size_t) ns, hipMemcpyHostToDevice,
-                            st)))
-        return e;
+    if ((e = mg_upload(m->d_sdims[1], m->sdims[1], m->up_sdims[1], st))) return e;
     const dim3 ab(kAsm1Rows);
     if (dim == 2)
         hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 9, 16>), dim3((n1 + 15) / 16), dim3(kAsm1Rows), 0, st, n1, m->cst1,
@@ -988,17 +1013,15 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
                 return e;
             m->gcap = cap;
         }
-        if ((e = hipMemcpyAsync(m->d_sdims[l + 1], dp.data(), sizeof(MgDims) * (size_t) ns, hipMemcpyHostToDevice,
-                                st)) ||
-            (e = hipMemcpyAsync(m->d_goff[l + 1], go.data(), sizeof(int) * (size_t) (ns + 1), hipMemcpyHostToDevice,
-                                st)))
+        if ((e = mg_upload(m->d_sdims[l + 1], dp, m->up_sdims[l + 1], st)) ||
+            (e = mg_upload(m->d_goff[l + 1], go, m->up_goff[l + 1], st)))
             return e;
         if ((e = fill_int(m->flag, 0, (size_t) G, st))) return e;
         hipLaunchKernelGGL(k_mg_mark, mg_grid(m->n[l]), dim3(kBlock), 0, st, m->n[l], m->coord[l], m->scene[l],
                            m->d_sdims[l], m->d_sdims[l + 1], m->d_goff[l + 1], m->flag);
         if ((e = mg_scan(m, m->flag, m->pscan, G, st))) return e;
         int np = 0;
-        if ((e = mg_read_int(m->pscan + G, &np, st))) return e;
+        if ((e = mg_read_int(m, m->pscan + G, &np, st))) return e;
         if ((e = mg_reserve_level(m, l + 1, np))) return e;
         hipLaunchKernelGGL(k_mg_parents, mg_grid(G), dim3(kBlock), 0, st, G, m->flag, m->pscan, m->d_goff[l + 1], ns,
                            m->coord[l + 1], m->scene[l + 1]);
@@ -1039,11 +1062,16 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
     for (int k = 1; k <= l; k++)
         m->hdesc[k] = MgLevelDesc{m->cap[k], 0,         m->col[k], m->val[k],   m->diag[k],
                                   m->dinv[k], m->agg[k], m->child[k], m->b[k], m->x[k]};
-    if ((e = hipMemcpyAsync(m->desc, m->hdesc, sizeof(MgLevelDesc) * (kMgMaxLevels + 1), hipMemcpyHostToDevice, st)))
-        return e;
+    if (!m->up_desc_valid || std::memcmp(m->up_desc, m->hdesc, sizeof(m->hdesc)) != 0) {
+        std::memcpy(m->up_desc, m->hdesc, sizeof(m->hdesc));
+        m->up_desc_valid = true;
+        if ((e = hipMemcpyAsync(m->desc, m->up_desc, sizeof(MgLevelDesc) * (kMgMaxLevels + 1), hipMemcpyHostToDevice,
+                                st)))
+            return e;
+    }
     if ((e = hipMemcpyAsync(m->nd, m->n, sizeof(int) * (kMgMaxLevels + 1), hipMemcpyHostToDevice, st))) return e;
     int bad = 0;
-    if ((e = mg_read_int(m->err, &bad, st))) return e;
+    if ((e = mg_read_int(m, m->err, &bad, st))) return e;
     if (bad) return hipErrorInvalidValue;  // a coarse row with more neighbours than the stencil allows
     return hipGetLastError();
 }

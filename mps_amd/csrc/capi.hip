@@ -376,6 +376,7 @@ static int alloc_all(mps_handle* h) {
     h->ev.err = h->dev_counters + 2;
     h->ev.cap = h->ev_cap;
     HIPCHK(h, hipHostMalloc((void**) &h->pinned_i, 16 * sizeof(int), hipHostMallocDefault));
+    HIPCHK(h, hipHostMalloc((void**) &h->A.pin, 4 * sizeof(int), hipHostMallocDefault));
     HIPCHK(h, hipHostMalloc((void**) &h->pinned_d, (16 + ns) * sizeof(double), hipHostMallocDefault));
     HIPCHK(h, fill_int(h->bb.cnt, 0, nc + 1, h->st));
     HIPCHK(h, fill_int(h->w.coll_last, 0, n, h->st));
@@ -424,6 +425,7 @@ static void free_all(mps_handle* h) {
         hipFree(x.dtype);
     }
     if (h->pinned_i) hipHostFree(h->pinned_i);
+    if (h->A.pin) hipHostFree(h->A.pin);
     if (h->pinned_d) hipHostFree(h->pinned_d);
     if (h->cg) cg_destroy(h->cg);
     if (h->st) hipStreamDestroy(h->st);

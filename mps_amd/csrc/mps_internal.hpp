@@ -714,6 +714,7 @@ struct Sell {
     int use16;           // every |column - row| fits 16 bits
     int* maxdelta;       // device: max |column - row| of the last assembly
     int wide_hint;       // host: the last assembly needed 32-bit columns (write them in the first pass)
+    int* pin;            // 4 pinned host words for the size reads of the assembly
     double* val64;       // scaled off-diagonal values (FP64)
     float* val32;        // scaled off-diagonal values (FP32), mixed path
     int* row_slot;       // rows -> slot

@@ -206,7 +206,7 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
                            A.slice_w);
     e = scan_exclusive(A.slice_w, A.slice_off, nsl_max, w.scan_tmp, st);
     if (e != hipSuccess) return e;
-    int hv[2];
+    int* hv = A.pin;  // pinned: the copies need no staging
     e = hipMemcpyAsync(&hv[0], w.row_of + n, sizeof(int), hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
     e = hipMemcpyAsync(&hv[1], A.slice_off + nsl_max, sizeof(int), hipMemcpyDeviceToHost, st);
@@ -282,8 +282,10 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
             hipLaunchKernelGGL((k_fill_sell<3, false>), fg, fb, 0, st, FILL_ARGS);
 #undef FILL_ARGS
         // 16-bit column offsets when every row's neighbours lie within +-32767 rows
-        if ((e = hipMemcpyAsync(&md, A.maxdelta, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(&A.pin[2], A.maxdelta, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess)
+            return e;
         if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+        md = A.pin[2];
         if (md <= 32767 || write_col) break;
     }
     A.use16 = md <= 32767 ? 1 : 0;

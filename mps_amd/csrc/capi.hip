@@ -690,7 +690,10 @@ static int adj_reverse_step(mps_handle* h, const StepTap& tap, const double* f0,
     HIPCHK(h, adj_pressure_stage(dim, g, n, h->c, tap.c3, S.type_g, h->w, h->x_s, a, h->st));
     mps_solver_stats ss{};
     std::string msg;
-    HIPCHK(h, cg_solve(h->cg, h->A, tap.c3, n, a.rhs, nullptr, nullptr, a.lam, oa, &ss, h->st, msg));
+    cg_prediction_bank(h->cg, 1);
+    const hipError_t ce = cg_solve(h->cg, h->A, tap.c3, n, a.rhs, nullptr, nullptr, a.lam, oa, &ss, h->st, msg);
+    cg_prediction_bank(h->cg, 0);
+    if (ce != hipSuccess) return hip_fail(h, ce, "cg_solve (adjoint)");
     agg.rows = std::max(agg.rows, ss.rows);
     agg.iterations += ss.iterations;
     agg.refinements += ss.refinements;

@@ -757,6 +757,9 @@ void mg_cycle(MgWork* m, const Sell& A, const IDX* fcol, const T* fval, const T*
 struct CgWork;  // defined in ppe.hip
 CgWork* cg_create();
 void cg_destroy(CgWork* cw);
+// Selects which record of last iteration counts sizes the CG chunks: 0 for
+// the step's own solves, 1 for the adjoint's.
+void cg_prediction_bank(CgWork* cw, int bank);
 hipError_t cg_reserve(CgWork* cw, int rows);
 // Solves A x = b for the Inner rows.  b_s / x_s in slot order; x0_id (id order,
 // may be NULL) gives the warm start.  Fills stats.

@@ -687,12 +687,17 @@ struct CgWork {
     const void* graph_key[2] = {nullptr, nullptr};
     static constexpr int kPredSolves = 4;
     bool predict = true;  // MPS_AMD_CG_PREDICT=0: fixed chunks of the check interval
-    int pred[2][kPredSolves] = {};  // last step's iterations per solve of a step
+    // last step's iterations per solve of a step; bank 1 holds the adjoint
+    // solves', whose counts (zero start) differ from the forward ones'
+    int pred[2][2][kPredSolves] = {};
+    int pred_bank = 0;
     int graph_idx16[2] = {-1, -1};
     int graph_blocks[2] = {0, 0};
     int graph_mg[2] = {-2, -2};  // multigrid generation captured (-1: none)
     Seg seg() const { return Seg{nseg, blk_off, row_off}; }
 };
+
+void cg_prediction_bank(CgWork* cw, int bank) { cw->pred_bank = bank ? 1 : 0; }
 
 CgWork* cg_create() {
     CgWork* cw = new CgWork();
@@ -835,12 +840,13 @@ static bool all_done(const CgWork* cw) {
 }
 
 // Runs chunks of CG iterations (each a captured graph, or enqueued directly)
-// until every segment is done.  `first` > 0: length of the first chunk (a
-// prediction of the solve's iterations), then chunks of 2; else chunks of
-// the check interval.  Iterations past convergence return at once.
+// until every segment is done; iterations past convergence return at once.
+// pred > 0 (the iterations this solve took in the previous step): chunks
+// cover the predicted remainder (even lengths, at most 64), then, once the
+// prediction is used up, 2, 4, 8, ... 64; else chunks of the check interval.
 template <typename T, typename IDX, bool WEIGHTED>
 static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* col, const T* val, T* y, T* r, T* p,
-                              T* s, T* w, const mps_solver_options& o, int first, hipStream_t st) {
+                              T* s, T* w, const mps_solver_options& o, int pred, hipStream_t st) {
     const int interval = std::max(2, (o.check_interval + 1) & ~1);
     const int idx16 = sizeof(IDX) == 2 ? 1 : 0;
     const size_t sbytes = sizeof(CgScalars) * (size_t) cw->nseg;
@@ -860,8 +866,19 @@ static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* c
             cw->graph_mg[which] = mgk;
         }
     }
-    for (int round = 0;; round++) {
-        const int chunk = round == 0 ? (first > 0 ? first : interval) : (first > 0 ? 2 : interval);
+    int launched = 0, grow = 2;
+    for (;;) {
+        int chunk = interval;
+        if (pred > 0) {
+            const int rem = (pred - 1 - launched) & ~1;  // one less than predicted, even
+            if (rem >= 2) {
+                chunk = rem < 64 ? rem : 64;
+            } else {
+                chunk = grow;
+                grow = grow < 64 ? 2 * grow : 64;
+            }
+        }
+        launched += chunk;
         if (o.use_graph) {
             int slot = -1;
             for (int j = 0; j < CgWork::kGraphSlots; j++)
@@ -920,22 +937,22 @@ static void launch_residual(CgWork* cw, const Sell& A, hipStream_t st) {
 
 // `solve`: index of this solve within the step (0 for the FP64 CG, the
 // refinement for the inner CG); the last step's iteration count of the same
-// solve sizes the first chunk, one less, rounded down to even.
+// solve sizes the chunks.
 template <typename T, bool WEIGHTED>
 static hipError_t run_cg(CgWork* cw, int which, int solve, const Sell& A, const T* val, T* y, T* r, T* p, T* s, T* w,
                          const mps_solver_options& o, hipStream_t st) {
     const int slot = solve < CgWork::kPredSolves ? solve : CgWork::kPredSolves - 1;
-    const int pred = cw->predict ? cw->pred[which][slot] : 0;
-    const int first = pred >= 4 ? std::min(((pred - 1) / 2) * 2, 64) : 0;
+    int* const last = &cw->pred[cw->pred_bank][which][slot];
+    const int pred = cw->predict ? *last : 0;
     hipError_t e;
     if (A.use16)
-        e = run_cg_loop<T, short, WEIGHTED>(cw, which, A, A.dcol, val, y, r, p, s, w, o, first, st);
+        e = run_cg_loop<T, short, WEIGHTED>(cw, which, A, A.dcol, val, y, r, p, s, w, o, pred, st);
     else
-        e = run_cg_loop<T, int, WEIGHTED>(cw, which, A, A.col, val, y, r, p, s, w, o, first, st);
+        e = run_cg_loop<T, int, WEIGHTED>(cw, which, A, A.col, val, y, r, p, s, w, o, pred, st);
     if (e) return e;
     int it = 0;  // the segment that took longest sets the chunk
     for (int k = 0; k < cw->nseg; k++) it = std::max(it, cw->host[k].iter);
-    cw->pred[which][slot] = it;
+    *last = it;
     return hipSuccess;
 }
 

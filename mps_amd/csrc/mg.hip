@@ -52,6 +52,21 @@ __device__ __forceinline__ bool mg_done(MgDone d) {
     return s_done != 0;
 }
 
+// The same test made by each wave on its own (lane i reads segment i's
+// flag), for kernels without a workgroup barrier: every wave reads the same
+// flags, written by earlier launches, so all waves agree.
+__device__ __forceinline__ bool mg_done_wave(MgDone d) {
+    if (!d.flag) return false;
+    const int lane = threadIdx.x & 63;
+    bool all = true;
+    for (int i0 = 0; i0 < d.count; i0 += 64) {
+        const int i = i0 + lane;
+        const int f = i < d.count ? *(volatile const int*) (d.flag + (size_t) i * d.stride) : 1;
+        all = all && __all(f != 0);
+    }
+    return all;
+}
+
 // Scene of a key / a dense index: the last i with first[i] <= v (first has
 // count + 1 ascending entries).
 __device__ __forceinline__ int mg_bucket(const int* __restrict__ first, int count, int v) {
@@ -424,80 +439,102 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict1(const int* __restrict__
     b1[K] = s;
 }
 
+// Coarse levels are small: their kernels are bound by the round trips of
+// dependent loads, not by bytes.  A row's W ELL entries are read without a
+// branch, every load of a round issued before any is used: padding (column
+// -1, value 0, after the row's columns) is read at the row's own index and
+// left out of the sum, which keeps the entries' order of summation.
+
 // Residual of level-l row K at x = omega D^-1 b (x recomputed where read).
 template <typename T, int W>
 __device__ __forceinline__ T mg_child_residual(int K, const int* __restrict__ col, const T* __restrict__ val,
                                                const T* __restrict__ diag, const T* __restrict__ dinv, int cap,
                                                const T* __restrict__ b, T omega) {
 #pragma clang fp contract(fast)
-    const T bK = b[K];
-    T acc = diag[K] * (omega * bK * dinv[K]);
+    int c[W];
+    T v[W], bb[W], dd[W];
+#pragma unroll
     for (int j = 0; j < W; j++) {
-        const int c = col[(size_t) j * cap + K];
-        if (c < 0) break;
-        acc += val[(size_t) j * cap + K] * (omega * b[c] * dinv[c]);
+        c[j] = col[(size_t) j * cap + K];
+        v[j] = val[(size_t) j * cap + K];
     }
+    const T bK = b[K], gK = diag[K], dK = dinv[K];
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        const int cc = c[j] >= 0 ? c[j] : K;
+        bb[j] = b[cc];
+        dd[j] = dinv[cc];
+    }
+    T acc = gK * (omega * bK * dK);
+#pragma unroll
+    for (int j = 0; j < W; j++)
+        if (c[j] >= 0) acc += v[j] * (omega * bb[j] * dd[j]);
     return bK - acc;
 }
 
-// Restriction of level-l residuals to parent P: one lane per child slot, the
-// C lanes of a parent summed in a fixed tree.  Every lane must call it.
-template <typename T, int W, int C>
-__device__ __forceinline__ void mg_down_item(int t, int np, const int* __restrict__ child, int capp,
-                                             const int* __restrict__ col, const T* __restrict__ val,
-                                             const T* __restrict__ diag, const T* __restrict__ dinv, int cap,
-                                             const T* __restrict__ b, T* __restrict__ bp, T omega) {
-    const int P = t / C, q = t % C;
-    T v = (T) 0;
-    if (P < np) {
-        const int K = child[(size_t) q * capp + P];
-        if (K >= 0) v = mg_child_residual<T, W>(K, col, val, diag, dinv, cap, b, omega);
-    }
-    for (int o = C / 2; o > 0; o >>= 1) v += __shfl_down(v, o, C);
-    if (P < np && q == 0) bp[P] = v;
-}
-
 // Coarse level l, down: b_{l+1}[P] = sum over P's children K of (b - A x)_K.
+// One lane per child slot, the C lanes of a parent summed in a fixed tree.
+// The level size, the done flags and the child are read in one round (the
+// child is in range for every P < capp and used only for P < np).
 template <typename T, int W, int C>
 __global__ void __launch_bounds__(kBlock) k_mg_down(const int* __restrict__ nparent, const int* __restrict__ child,
                                                     int capp, const int* __restrict__ col, const T* __restrict__ val,
                                                     const T* __restrict__ diag, const T* __restrict__ dinv, int cap,
                                                     const T* __restrict__ b, T* __restrict__ bp, T omega,
                                                     MgDone done) {
-    if (mg_done(done)) return;
-    mg_down_item<T, W, C>((int) (blockIdx.x * blockDim.x + threadIdx.x), *nparent, child, capp, col, val, diag, dinv, cap, b,
-                          bp, omega);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int P = t / C, q = t % C;
+    const int np = *nparent;
+    const int Kc = P < capp ? child[(size_t) q * capp + P] : -1;
+    if (mg_done_wave(done)) return;  // wave-uniform
+    const bool live = P < np;
+    T v = (T) 0;
+    if (live && Kc >= 0) v = mg_child_residual<T, W>(Kc, col, val, diag, dinv, cap, b, omega);
+    for (int o = C / 2; o > 0; o >>= 1) v += __shfl_down(v, o, C);
+    if (live && q == 0) bp[P] = v;
 }
 
 // Coarse level l, up at row K: xx = omega D^-1 b + cf x_{l+1}[agg] (recomputed
-// where read), then x = xx + omega D^-1 (b - A xx).
-template <typename T, int W>
-__device__ __forceinline__ void mg_up_row(int K, const int* __restrict__ col, const T* __restrict__ val,
-                                          const T* __restrict__ diag, const T* __restrict__ dinv, int cap,
-                                          const int* __restrict__ agg, const T* __restrict__ b,
-                                          const T* __restrict__ xc, T* __restrict__ x, T omega, T cf) {
-#pragma clang fp contract(fast)
-    const T bK = b[K], dK = dinv[K];
-    const T xx = omega * bK * dK + cf * xc[agg[K]];
-    T acc = diag[K] * xx;
-    for (int j = 0; j < W; j++) {
-        const int c = col[(size_t) j * cap + K];
-        if (c < 0) break;
-        acc += val[(size_t) j * cap + K] * (omega * b[c] * dinv[c] + cf * xc[agg[c]]);
-    }
-    x[K] = xx + omega * dK * (bK - acc);
-}
-
+// where read), then x = xx + omega D^-1 (b - A xx).  The row's own values
+// and entries are read in the round that reads the level size and the done
+// flags (in range for every K < cap; rows past cap read row 0 and store
+// nothing).
 template <typename T, int W>
 __global__ void __launch_bounds__(kBlock) k_mg_up(const int* __restrict__ nrow, const int* __restrict__ col,
                                                   const T* __restrict__ val, const T* __restrict__ diag,
                                                   const T* __restrict__ dinv, int cap, const int* __restrict__ agg,
                                                   const T* __restrict__ b, const T* __restrict__ xc,
                                                   T* __restrict__ x, T omega, T cf, MgDone done) {
-    if (mg_done(done)) return;
+#pragma clang fp contract(fast)
     const int K = blockIdx.x * blockDim.x + threadIdx.x;
-    if (K >= *nrow) return;
-    mg_up_row<T, W>(K, col, val, diag, dinv, cap, agg, b, xc, x, omega, cf);
+    const int Kr = K < cap ? K : 0;
+    const int nr = *nrow;
+    int c[W], ag[W];
+    T v[W], bb[W], dd[W], xv[W];
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        c[j] = col[(size_t) j * cap + Kr];
+        v[j] = val[(size_t) j * cap + Kr];
+    }
+    const T bK = b[Kr], dK = dinv[Kr], gK = diag[Kr];
+    const int aK = agg[Kr];
+    if (mg_done_wave(done) || K >= nr) return;
+#pragma unroll
+    for (int j = 0; j < W; j++) {
+        const int cc = c[j] >= 0 ? c[j] : K;
+        bb[j] = b[cc];
+        dd[j] = dinv[cc];
+        ag[j] = agg[cc];
+    }
+    const T xcK = xc[aK];
+#pragma unroll
+    for (int j = 0; j < W; j++) xv[j] = xc[ag[j]];
+    const T xx = omega * bK * dK + cf * xcK;
+    T acc = gK * xx;
+#pragma unroll
+    for (int j = 0; j < W; j++)
+        if (c[j] >= 0) acc += v[j] * (omega * bb[j] * dd[j] + cf * xv[j]);
+    x[K] = xx + omega * dK * (bK - acc);
 }
 
 // Device table of one level's arrays, for the tail kernel.
@@ -565,27 +602,39 @@ struct MgLdsLevel {
 };
 
 // Levels l0..L in one workgroup, out of LDS: every level's operator, transfer
-// maps and b[l0] are loaded once (one round of independent loads), then the
-// restrictions down to the coarsest, `sweeps` damped Jacobi sweeps there and
-// the prolongation + smoothing phases back to level l0 run on LDS, separated
-// by workgroup barriers; x[l0] goes back to global memory for the next kernel.
+// maps and b[l0] are loaded once, then the restrictions down to the
+// coarsest, `sweeps` damped Jacobi sweeps there and the prolongation +
+// smoothing phases back to level l0 run on LDS, separated by workgroup
+// barriers; x[l0] goes back to global memory for the next kernel.  The load
+// takes the rows of all levels as one list, each thread reading all of a
+// row's values into registers before it writes any to LDS, so the whole
+// load costs about one round trip per 1024 rows.
 template <typename T, int W, int C>
 __global__ void __launch_bounds__(kMgTailThreads) k_mg_tail(const MgLevelDesc* __restrict__ lv,
                                                             const int* __restrict__ nd, int l0, int L, T omega, T cf,
                                                             int sweeps, MgDone done) {
 #pragma clang fp contract(fast)
+    static_assert(W >= C, "child reads of the first level fall inside its column array");
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int s_off[kMgMaxLevels + 2];
+    __shared__ int s_row[kMgMaxLevels + 2];  // first row of each level in the joint list
     __shared__ int s_n[kMgMaxLevels + 1];
-    if (mg_done(done)) return;
+    __shared__ MgLevelDesc s_lv[kMgMaxLevels + 1];
     const int tid = threadIdx.x;
+    if (tid <= L - l0) {
+        s_n[l0 + tid] = nd[l0 + tid];
+        s_lv[l0 + tid] = lv[l0 + tid];
+    }
+    if (mg_done(done)) return;  // its barrier also publishes the writes above
     if (tid == 0) {
-        int o = 0;
+        int o = 0, r = 0;
         for (int l = l0; l <= L; l++) {
-            s_n[l] = nd[l];
             s_off[l] = o;
+            s_row[l] = r;
             o += MgLdsLevel<T, W, C>::bytes(s_n[l]);
+            r += s_n[l];
         }
+        s_row[L + 1] = r;
     }
     __syncthreads();
     auto level = [&](int l) {
@@ -594,22 +643,41 @@ __global__ void __launch_bounds__(kMgTailThreads) k_mg_tail(const MgLevelDesc* _
         return q;
     };
     // load
-    for (int l = l0; l <= L; l++) {
-        const MgLevelDesc d = lv[l];
+    for (int t = tid; t < s_row[L + 1]; t += kMgTailThreads) {
+        int l = l0;
+        while (t >= s_row[l + 1]) l++;
+        const int K = t - s_row[l];
+        const MgLevelDesc& d = s_lv[l];
+        // arrays a level does not have are read at in-range places of its
+        // column array and the values dropped
+        const int* agp = l < L ? d.agg : d.col;
+        const int* chp = l > l0 ? d.child : d.col;
+        const T* bp = l == l0 ? (const T*) d.b : (const T*) d.val;
+        int cc[W], ch[C];
+        T vv[W];
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            cc[j] = d.col[(size_t) j * d.cap + K];
+            vv[j] = ((const T*) d.val)[(size_t) j * d.cap + K];
+        }
+#pragma unroll
+        for (int c = 0; c < C; c++) ch[c] = chp[(size_t) c * d.cap + K];
+        const T dg = ((const T*) d.diag)[K], di = ((const T*) d.dinv)[K], bb = bp[K];
+        const int ag = agp[K];
         const int n = s_n[l];
         const MgLdsLevel<T, W, C> q = level(l);
-        for (int K = tid; K < n; K += kMgTailThreads) {
-            for (int j = 0; j < W; j++) {
-                q.col[j * n + K] = (short) d.col[(size_t) j * d.cap + K];
-                q.val[j * n + K] = ((const T*) d.val)[(size_t) j * d.cap + K];
-            }
-            q.diag[K] = ((const T*) d.diag)[K];
-            q.dinv[K] = ((const T*) d.dinv)[K];
-            if (l == l0) q.b[K] = ((const T*) d.b)[K];
-            if (l < L) q.agg[K] = (short) d.agg[K];
-            if (l > l0)
-                for (int c = 0; c < C; c++) q.child[c * n + K] = (short) d.child[(size_t) c * d.cap + K];
+#pragma unroll
+        for (int j = 0; j < W; j++) {
+            q.col[j * n + K] = (short) cc[j];
+            q.val[j * n + K] = vv[j];
         }
+        q.diag[K] = dg;
+        q.dinv[K] = di;
+        if (l == l0) q.b[K] = bb;
+        if (l < L) q.agg[K] = (short) ag;
+        if (l > l0)
+#pragma unroll
+            for (int c = 0; c < C; c++) q.child[c * n + K] = (short) ch[c];
     }
     __syncthreads();
     // down

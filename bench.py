@@ -28,12 +28,13 @@ HBM_PEAK_GBS = 8000.0
 TOL = 1e-9
 
 
-def run_path(settings, scene, precision, steps, warmup, check_interval, reps, use_graph=True, precond=1, warm=1):
+def run_path(settings, scene, precision, steps, warmup, check_interval, reps, use_graph=True, precond=1, warm=1,
+             inner_tol=1e-4):
     """settings, scene: one scene; settings = list of (settings, scene), scene =
     None: an ensemble in one handle."""
     sim = capi.Simulation(settings, scene)
     sim.set_solver(precision=precision, tolerance=TOL, warm_start=warm, check_interval=check_interval,
-                   inner_tolerance=1e-4, use_graph=use_graph, preconditioner=precond)
+                   inner_tolerance=inner_tol, use_graph=use_graph, preconditioner=precond)
     sim.load_initial_state()
     for _ in range(warmup):
         sim.step()
@@ -96,6 +97,8 @@ def main():
     ap.add_argument("--paths", default="fp64,mixed", help="paths to run (mixed sets value)")
     ap.add_argument("--warm-start", type=int, default=2, choices=(0, 1, 2),
                     help="CG start: 0 zero, 1 the last solution, 2 extrapolated from the last two")
+    ap.add_argument("--inner-tol", type=float, default=1e-4,
+                    help="mixed path: residual reduction asked of each FP32 inner solve")
     a = ap.parse_args()
     precond = capi.MPS_PRECOND_MULTIGRID if a.precond == "multigrid" else capi.MPS_PRECOND_JACOBI
     if a.gpus != 1:
@@ -124,14 +127,15 @@ def main():
         if name not in a.paths.split(","):
             continue
         r = run_path(settings, scene, prec, a.steps, a.warmup, a.check_interval, a.reps, not a.no_graph, precond,
-                     a.warm_start)
+                     a.warm_start, a.inner_tol)
         worst = max(r["residuals"]) if r["residuals"] else 0.0
         if not worst <= TOL:
             print(f"[bench] {name}: residual {worst:.3e} above {TOL:g}", file=sys.stderr)
             sys.exit(1)
         out[name] = r
         print(f"[bench] {name}: {1e3 * r['seconds'] / a.steps:.2f} ms/step, mean CG iterations "
-              f"{np.mean(r['iterations']):.1f}, worst residual {worst:.2e}", file=sys.stderr, flush=True)
+              f"{np.mean(r['iterations']):.1f}, refinements {np.mean(r['refinements']):.2f}, worst residual {worst:.2e}",
+              file=sys.stderr, flush=True)
 
     if "mixed" not in out or "fp64" not in out:
         print(json.dumps({k: dict(ms_per_step=1e3 * v["seconds"] / a.steps,
@@ -180,6 +184,7 @@ def main():
             "tolerance": TOL,
             "warm_start": {0: "zero", 1: "last solution", 2: "extrapolated from the last two solutions"}[a.warm_start],
             "preconditioner": a.precond,
+            "inner_tolerance": a.inner_tol,
             "per_particle_value_basis": "n_fluid",
         },
         "fp64": {

@@ -404,6 +404,35 @@ __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble(int np, const int* __
 
 // ---------------- the V-cycle ----------------
 
+// acc + sum over fine row r's entries of Ahat_rc x_c, in entry order (SELL-64
+// slices); entries are read four at a time, the loads of a group issued
+// before any is used.
+template <typename T, typename IDX>
+__device__ __forceinline__ T mg_fine_row(const int* __restrict__ slice_pos, const IDX* __restrict__ col,
+                                         const T* __restrict__ val, const T* __restrict__ x, int r, T acc) {
+#pragma clang fp contract(fast)
+    const int sl = r >> 6, lane = r & 63;
+    const int s0 = slice_pos[sl], w = slice_pos[sl + 1] - s0;
+    const size_t base = (size_t) s0 * 64 + lane;
+    int k = 0;
+    for (; k + 4 <= w; k += 4) {
+        const size_t i0 = base + (size_t) k * 64;
+        const int c0 = mg_col(col, i0, r), c1 = mg_col(col, i0 + 64, r);
+        const int c2 = mg_col(col, i0 + 128, r), c3 = mg_col(col, i0 + 192, r);
+        const T v0 = val[i0], v1 = val[i0 + 64], v2 = val[i0 + 128], v3 = val[i0 + 192];
+        const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        acc += v0 * x0;
+        acc += v1 * x1;
+        acc += v2 * x2;
+        acc += v3 * x3;
+    }
+    for (; k < w; k++) {
+        const size_t idx = base + (size_t) k * 64;
+        acc += val[idx] * x[mg_col(col, idx, r)];
+    }
+    return acc;
+}
+
 // Fine level, down: x0 = omega b (unit diagonal); res = b - Ahat x0.
 template <typename T, typename IDX>
 __global__ void __launch_bounds__(kBlock) k_mg_down0(const int* __restrict__ nrow, const int* __restrict__ slice_pos,
@@ -411,18 +440,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_down0(const int* __restrict__ nro
                                                      const T* __restrict__ b, T* __restrict__ res, T omega,
                                                      MgDone done) {
 #pragma clang fp contract(fast)
-    if (mg_done(done)) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= *nrow) return;
-    const int sl = r >> 6, lane = r & 63;
-    const int s0 = slice_pos[sl], w = slice_pos[sl + 1] - s0;
-    const size_t base = (size_t) s0 * 64 + lane;
+    const int nr = *nrow;
+    if (mg_done_wave(done) || r >= nr) return;
     const T br = b[r];
-    T acc = br;
-    for (int k = 0; k < w; k++) {
-        const size_t idx = base + (size_t) k * 64;
-        acc += val[idx] * b[mg_col(col, idx, r)];
-    }
+    const T acc = mg_fine_row<T, IDX>(slice_pos, col, val, b, r, br);
     res[r] = br - omega * acc;
 }
 
@@ -431,11 +453,19 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_mg_restrict1(const int* __restrict__ nrow, const int* __restrict__ cst1,
                                                          const T* __restrict__ res, T* __restrict__ b1,
                                                          MgDone done) {
-    if (mg_done(done)) return;
     const int K = blockIdx.x * blockDim.x + threadIdx.x;
-    if (K >= *nrow) return;
+    const int nr = *nrow;
+    if (mg_done_wave(done) || K >= nr) return;
+    const int r0 = cst1[K], r1 = cst1[K + 1];
     T s = (T) 0;
-    for (int r = cst1[K]; r < cst1[K + 1]; r++) s += res[r];
+    for (int r = r0; r < r1; r += 4) {  // four loads in flight, summed in row order
+        T v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = res[r + j < r1 ? r + j : r1 - 1];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (r + j < r1) s += v[j];
+    }
     b1[K] = s;
 }
 
@@ -764,9 +794,9 @@ __global__ void __launch_bounds__(kBlock) k_mg_prolong0(const int* __restrict__ 
                                                         const int* __restrict__ agg0, const T* __restrict__ x1,
                                                         T* __restrict__ xx, T omega, T cf,
                                                         MgDone done) {
-    if (mg_done(done)) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= *nrow) return;
+    const int nr = *nrow;
+    if (mg_done_wave(done) || r >= nr) return;
     xx[r] = omega * b[r] + cf * x1[agg0[r]];
 }
 
@@ -777,19 +807,12 @@ __global__ void __launch_bounds__(kBlock) k_mg_up0(const int* __restrict__ nrow,
                                                    const T* __restrict__ b, const T* __restrict__ xx,
                                                    T* __restrict__ u, T omega, MgDone done) {
 #pragma clang fp contract(fast)
-    if (mg_done(done)) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= *nrow) return;
-    const int sl = r >> 6, lane = r & 63;
-    const int s0 = slice_pos[sl], w = slice_pos[sl + 1] - s0;
-    const size_t base = (size_t) s0 * 64 + lane;
-    const T xr = xx[r];
-    T acc = xr;
-    for (int k = 0; k < w; k++) {
-        const size_t idx = base + (size_t) k * 64;
-        acc += val[idx] * xx[mg_col(col, idx, r)];
-    }
-    u[r] = xr + omega * (b[r] - acc);
+    const int nr = *nrow;
+    if (mg_done_wave(done) || r >= nr) return;
+    const T xr = xx[r], br = b[r];
+    const T acc = mg_fine_row<T, IDX>(slice_pos, col, val, xx, r, xr);
+    u[r] = xr + omega * (br - acc);
 }
 
 // ---------------- host side ----------------

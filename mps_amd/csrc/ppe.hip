@@ -315,6 +315,9 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
 // apply gathers only r, once per stored entry; step is elementwise.
 
 constexpr int kMaxBlocks = 1024;  // per segment
+// Threads per block of the per-iteration kernels (apply, step): 1024 blocks
+// of 512 fill every SIMD with 8 waves, twice the loads in flight of 256.
+constexpr int kCgBlock = 512;
 // Below this many rows the V-cycle's setup and launches cost more than the
 // iterations it saves (measured: 584-particle dam break), so small systems
 // keep the Jacobi scaling alone.
@@ -466,20 +469,20 @@ __device__ __forceinline__ T sell_row(const int* __restrict__ slice_pos, const I
 // apply: w = Ahat u with u = M r (u = r without a preconditioner; unit
 // diagonal); partials r.u, w.u and the stopping norm of r.
 template <typename T, typename IDX, bool WEIGHTED, bool PRE>
-__global__ void __launch_bounds__(kBlock) k_cg_apply(Seg sg, const int* __restrict__ slice_pos,
+__global__ void __launch_bounds__(kCgBlock) k_cg_apply(Seg sg, const int* __restrict__ slice_pos,
                                                      const IDX* __restrict__ col, const T* __restrict__ val,
                                                      const T* __restrict__ r, const T* __restrict__ u,
                                                      T* __restrict__ w, const double* __restrict__ dgl,
                                                      double* __restrict__ part, CgScalars* scs) {
 #pragma clang fp contract(fast)
-    __shared__ double red[3 * kBlock / 64];
+    __shared__ double red[3 * kCgBlock / 64];
     const BlockRows br = block_rows(sg);
     CgScalars* sc = scs + br.seg;
     if (block_uniform_done(sc)) return;
     if (br.v == 0 && threadIdx.x == 0 && !sc->frozen) sc->iter += 1;
     const T* x = PRE ? u : r;
     double g = 0.0, d = 0.0, rw = 0.0;
-    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) {
+    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kCgBlock) {
         const T rr = r[row];
         const T uu = PRE ? x[row] : rr;
         const T acc = sell_row<T, IDX>(slice_pos, col, val, x, row, uu);
@@ -495,12 +498,12 @@ __global__ void __launch_bounds__(kBlock) k_cg_apply(Seg sg, const int* __restri
 
 // step: convergence test on r, then the vector recurrences.
 template <typename T, bool WEIGHTED, bool PRE>
-__global__ void __launch_bounds__(kBlock) k_cg_step(Seg sg, T* __restrict__ y, T* __restrict__ r, T* __restrict__ p,
+__global__ void __launch_bounds__(kCgBlock) k_cg_step(Seg sg, T* __restrict__ y, T* __restrict__ r, T* __restrict__ p,
                                                     T* __restrict__ s, const T* __restrict__ w,
                                                     const T* __restrict__ u, const double* __restrict__ part,
                                                     CgScalars* scs, int par) {
 #pragma clang fp contract(fast)
-    __shared__ double red[3 * kBlock / 64];
+    __shared__ double red[3 * kCgBlock / 64];
     const BlockRows br = block_rows(sg);
     CgScalars* sc = scs + br.seg;
     if (block_uniform_done(sc)) return;
@@ -535,7 +538,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_step(Seg sg, T* __restrict__ y, T
         alpha = (T) a;
         beta = (T) b;
     }
-    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) {
+    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kCgBlock) {
         const T rr = r[row], ww = w[row];
         const T uu = PRE ? u[row] : rr;
         const T pp = first ? uu : uu + beta * p[row];
@@ -813,7 +816,7 @@ template <> float* u_vec<float>(CgWork* cw) { return cw->u32; }
 template <typename T, typename IDX, bool WEIGHTED>
 static void enqueue_chunk(CgWork* cw, const Sell& A, const IDX* col, const T* val, T* y, T* r, T* p, T* s, T* w,
                           int chunk, hipStream_t st) {
-    const dim3 gr(cw->total_blocks), bl(kBlock);
+    const dim3 gr(cw->total_blocks), bl(kCgBlock);
     const Seg sg = cw->seg();
     T* u = u_vec<T>(cw);
     for (int k = 0; k < chunk; k++) {
@@ -1104,7 +1107,7 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
 
 template <typename IDX>
 static void time_launch(CgWork* cw, const Sell& A, const IDX* col, int kind, hipStream_t st) {
-    const dim3 gr(cw->total_blocks), bl(kBlock);
+    const dim3 gr(cw->total_blocks), bl(kCgBlock);
     const Seg sg = cw->seg();
     switch (kind) {
         case 0:

@@ -209,28 +209,50 @@ __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble1(int n1, const int* _
         const int r = c0 + tid;
         for (int j = 0; j <= W; j++) s_bin[tid][j] = 0.0;
         if (r < r1) {
-            int a = 0;
-            while (a + 1 < na && s_cst[a + 1] <= r) a++;
+            int a = 0, hi = na - 1;  // the last aggregate whose first row is <= r
+            while (a < hi) {
+                const int mid = (a + hi + 1) >> 1;
+                if (s_cst[mid] <= r)
+                    a = mid;
+                else
+                    hi = mid - 1;
+            }
             const int K = K0 + a, key = s_key[a];
             const MgDims d = s_dim[a];
             const int sl = r >> 6, lane = r & 63;
             const int sp = slice_pos[sl], w = slice_pos[sl + 1] - sp;
             const size_t base = (size_t) sp * 64 + lane;
-            for (int k = 0; k < w; k++) {
-                const size_t idx = base + (size_t) k * 64;
-                const int K2 = agg0[mg_col(col, idx, r)];
-                const double v = (double) val[idx];
-                if (K2 == K) {
-                    s_bin[tid][W] += v;
-                    continue;
+            // four entries per round: columns and values, then their
+            // aggregates, then the aggregates' cells, each round's loads
+            // issued together (entries past w re-read the last and are dropped)
+            for (int k0 = 0; k0 < w; k0 += 4) {
+                int K2[4], cd[4];
+                double v[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const size_t idx = base + (size_t) (k0 + q < w ? k0 + q : w - 1) * 64;
+                    K2[q] = mg_col(col, idx, r);
+                    v[q] = (double) val[idx];
                 }
-                const int j = mg_stencil_of_dkey<W>(coord1[K2] - key, d);
-                if (j < 0) {
-                    atomicOr(err, 1);
-                    continue;
+#pragma unroll
+                for (int q = 0; q < 4; q++) K2[q] = agg0[K2[q]];
+#pragma unroll
+                for (int q = 0; q < 4; q++) cd[q] = coord1[K2[q]];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (k0 + q >= w) break;
+                    if (K2[q] == K) {
+                        s_bin[tid][W] += v[q];
+                        continue;
+                    }
+                    const int j = mg_stencil_of_dkey<W>(cd[q] - key, d);
+                    if (j < 0) {
+                        atomicOr(err, 1);
+                        continue;
+                    }
+                    s_bin[tid][j] += v[q];
+                    s_nbr[a][j] = K2[q];  // every writer of this place writes the same aggregate
                 }
-                s_bin[tid][j] += v;
-                s_nbr[a][j] = K2;  // every writer of this place writes the same aggregate
             }
         }
         __syncthreads();
@@ -1070,12 +1092,12 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
     if ((e = mg_upload(m->d_sdims[1], m->sdims[1], m->up_sdims[1], st))) return e;
     const dim3 ab(kAsm1Rows);
     if (dim == 2)
-        hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 9, 16>), dim3((n1 + 15) / 16), dim3(kAsm1Rows), 0, st, n1, m->cst1,
+        hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 9, 48>), dim3((n1 + 47) / 48), dim3(kAsm1Rows), 0, st, n1, m->cst1,
                            A.slice_off, fcol, fval, m->agg[0], m->coord[1], m->scene[1], m->d_sdims[1], m->cap[1],
                            m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1], m->err);
     else
-        hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 27, 8>), dim3((n1 + 7) / 8), dim3(kAsm1Rows), 0, st, n1, m->cst1,
-                           A.slice_off, fcol, fval, m->agg[0], m->coord[1], m->scene[1], m->d_sdims[1], m->cap[1],
+        hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 27, 16>), dim3((n1 + 15) / 16), dim3(kAsm1Rows), 0, st, n1,
+                           m->cst1, A.slice_off, fcol, fval, m->agg[0], m->coord[1], m->scene[1], m->d_sdims[1], m->cap[1],
                            m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1], m->err);
     // stop at <= 64 rows, or once every scene's grid is a single cell
     const int coarse = std::max(kMgCoarse, ns);

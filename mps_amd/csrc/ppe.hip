@@ -563,16 +563,16 @@ __global__ void __launch_bounds__(kBlock) k_reduce_parts(Seg sg, const double* _
 }
 
 // FP64 init: yhat = x0 sqrt(d), bhat = S b; partial ||b||^2.
-__global__ void __launch_bounds__(kBlock) k_cg_init(Seg sg, const int* __restrict__ row_slot,
+__global__ void __launch_bounds__(kCgBlock) k_cg_init(Seg sg, const int* __restrict__ row_slot,
                                                     const int32_t* __restrict__ id, const double* __restrict__ b_s,
                                                     const double* __restrict__ pprev,
                                                     const double* __restrict__ pprev2, const double* __restrict__ sdiag,
                                                     double* __restrict__ y, double* __restrict__ bh,
                                                     double* __restrict__ part) {
-    __shared__ double red[3 * kBlock / 64];
+    __shared__ double red[3 * kCgBlock / 64];
     const BlockRows br = block_rows(sg);
     double bb = 0.0;
-    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) {
+    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kCgBlock) {
         const int s = row_slot[row];
         const double b = b_s[s];
         const double si = sdiag[row];
@@ -597,16 +597,16 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(Seg sg, const int* __restric
 
 // FP64 residual r = bhat - Ahat y (unit diagonal); partials r.r, sum d r^2.
 template <typename IDX>
-__global__ void __launch_bounds__(kBlock) k_residual64(Seg sg, const int* __restrict__ slice_pos,
+__global__ void __launch_bounds__(kCgBlock) k_residual64(Seg sg, const int* __restrict__ slice_pos,
                                                        const IDX* __restrict__ col, const double* __restrict__ val,
                                                        const double* __restrict__ y, const double* __restrict__ bh,
                                                        const double* __restrict__ dgl, double* __restrict__ r,
                                                        double* __restrict__ part) {
 #pragma clang fp contract(fast)
-    __shared__ double red[3 * kBlock / 64];
+    __shared__ double red[3 * kCgBlock / 64];
     const BlockRows br = block_rows(sg);
     double rr = 0.0, rw = 0.0;
-    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) {
+    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kCgBlock) {
         const double acc = sell_row<double, IDX>(slice_pos, col, val, y, row, y[row]);
         const double rv = bh[row] - acc;
         r[row] = rv;
@@ -618,13 +618,13 @@ __global__ void __launch_bounds__(kBlock) k_residual64(Seg sg, const int* __rest
 }
 
 // Inner solve setup: r32 = r64 (scaled residual), e = 0; partial r.r.
-__global__ void __launch_bounds__(kBlock) k_inner_init(Seg sg, const double* __restrict__ r64,
+__global__ void __launch_bounds__(kCgBlock) k_inner_init(Seg sg, const double* __restrict__ r64,
                                                        float* __restrict__ r32, float* __restrict__ e32,
                                                        double* __restrict__ part) {
-    __shared__ double red[3 * kBlock / 64];
+    __shared__ double red[3 * kCgBlock / 64];
     const BlockRows br = block_rows(sg);
     double rr = 0.0;
-    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) {
+    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kCgBlock) {
         const float rv = (float) r64[row];
         r32[row] = rv;
         e32[row] = 0.0f;
@@ -634,16 +634,16 @@ __global__ void __launch_bounds__(kBlock) k_inner_init(Seg sg, const double* __r
     write_partials<1>(v, part, red);
 }
 
-__global__ void __launch_bounds__(kBlock) k_add_correction(Seg sg, double* __restrict__ y, const float* __restrict__ e) {
+__global__ void __launch_bounds__(kCgBlock) k_add_correction(Seg sg, double* __restrict__ y, const float* __restrict__ e) {
     const BlockRows br = block_rows(sg);
-    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) y[row] += (double) e[row];
+    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kCgBlock) y[row] += (double) e[row];
 }
 
-__global__ void __launch_bounds__(kBlock) k_cg_finish(Seg sg, const int* __restrict__ row_slot,
+__global__ void __launch_bounds__(kCgBlock) k_cg_finish(Seg sg, const int* __restrict__ row_slot,
                                                       const double* __restrict__ y, const double* __restrict__ sdiag,
                                                       double* __restrict__ x_s) {
     const BlockRows br = block_rows(sg);
-    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kBlock) x_s[row_slot[row]] = y[row] * sdiag[row];
+    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kCgBlock) x_s[row_slot[row]] = y[row] * sdiag[row];
 }
 
 // Per-segment first rows: row_off[s] = compact row of the first slot of scene s.
@@ -928,7 +928,7 @@ static hipError_t reduce_and_read(CgWork* cw, int nv, hipStream_t st) {
 }
 
 static void launch_residual(CgWork* cw, const Sell& A, hipStream_t st) {
-    const dim3 gr(cw->total_blocks), bl(kBlock);
+    const dim3 gr(cw->total_blocks), bl(kCgBlock);
     const Seg sg = cw->seg();
     if (A.use16)
         hipLaunchKernelGGL(k_residual64<short>, gr, bl, 0, st, sg, A.slice_off, A.dcol, A.val64, cw->y, cw->bh, A.dgl,
@@ -974,7 +974,7 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
     hipError_t e = cg_reserve(cw, rows);
     if (e) return e;
     if ((e = cg_layout(cw, A, st))) return e;
-    const dim3 gr(cw->total_blocks), bl(kBlock);
+    const dim3 gr(cw->total_blocks), bl(kCgBlock);
     const Seg sg = cw->seg();
     // multigrid preconditioner (each segment its own hierarchy), built in the
     // precision of the CG it serves; if it cannot be built the solve runs
@@ -1140,7 +1140,7 @@ hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, h
     hipError_t e = cg_reserve(cw, rows);
     if (e) return e;
     if ((e = cg_layout(cw, A, st))) return e;
-    const dim3 gr(cw->total_blocks), bl(kBlock);
+    const dim3 gr(cw->total_blocks), bl(kCgBlock);
     const Seg sg = cw->seg();
     // finite vector contents
     const size_t R = (size_t) rows;

@@ -570,40 +570,74 @@ struct NbList {
     size_t stride;
 };
 
-// Calls f(q, dist, dx, dy, dz) for slot s's neighbours with dist < re (re at
-// most the list's radius), in for_neighbors' order and with its values.
-template <int DIM, typename F>
-__device__ __forceinline__ void for_listed(const NbList& L, const GridView& g, const double* __restrict__ px,
-                                           const double* __restrict__ py, const double* __restrict__ pz, int key,
-                                           int s, double xi, double yi, double zi, double re, F&& f) {
+// Calls f(q, dist, dx, dy, dz, gather(q)) for slot s's neighbours with
+// dist < re (re at most the list's radius), in for_neighbors' order and with
+// its values.  Entries go four at a time: the next four entries' indices are
+// loaded while this four's positions and gather(q) values are read, all of
+// them before f runs, so a group costs one round trip of dependent loads.
+// gather(q) must only read (it runs for entries f then skips).
+template <int DIM, typename G, typename F>
+__device__ __forceinline__ void for_listed_g(const NbList& L, const GridView& g, const double* __restrict__ px,
+                                             const double* __restrict__ py, const double* __restrict__ pz, int key,
+                                             int s, double xi, double yi, double zi, double re, G&& gather, F&& f) {
     const int m = L.cnt[s];
     if (m > L.cap) {
-        for_neighbors<DIM>(g, px, py, pz, key, s, xi, yi, zi, re, f);
+        for_neighbors<DIM>(g, px, py, pz, key, s, xi, yi, zi, re,
+                           [&](int q, double dist, double dx, double dy, double dz) {
+                               f(q, dist, dx, dy, dz, gather(q));
+                           });
         return;
     }
-    // entries four at a time: their loads are independent of each other
+    if (m <= 0) return;
+    int qn[4];
+    double dn[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {  // entries past m re-read the last one
+        const size_t e = (size_t) (i < m ? i : m - 1) * L.stride + s;
+        qn[i] = L.q[e];
+        dn[i] = L.d[e];
+    }
     for (int k0 = 0; k0 < m; k0 += 4) {
         int qq[4];
         double dd[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int k = k0 + i < m ? k0 + i : m - 1;
+            qq[i] = qn[i];
+            dd[i] = dn[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int k = k0 + 4 + i < m ? k0 + 4 + i : m - 1;
             const size_t e = (size_t) k * L.stride + s;
-            qq[i] = L.q[e];
-            dd[i] = L.d[e];
+            qn[i] = L.q[e];
+            dn[i] = L.d[e];
+        }
+        decltype(gather(0)) gv[4];
+        double xq[4], yq[4], zq[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            gv[i] = gather(qq[i]);
+            xq[i] = px[qq[i]];
+            yq[i] = py[qq[i]];
+            zq[i] = DIM == 3 ? pz[qq[i]] : 0.0;
         }
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             if (k0 + i >= m) break;
-            const int q = qq[i];
             const double dist = dd[i];
             if (!(dist < re)) continue;
-            const double dx = px[q] - xi;
-            const double dy = py[q] - yi;
-            const double dz = DIM == 3 ? pz[q] - zi : 0.0;
-            f(q, dist, dx, dy, dz);
+            f(qq[i], dist, xq[i] - xi, yq[i] - yi, DIM == 3 ? zq[i] - zi : 0.0, gv[i]);
         }
     }
+}
+
+// for_listed_g without per-neighbour gathers: f(q, dist, dx, dy, dz).
+template <int DIM, typename F>
+__device__ __forceinline__ void for_listed(const NbList& L, const GridView& g, const double* __restrict__ px,
+                                           const double* __restrict__ py, const double* __restrict__ pz, int key,
+                                           int s, double xi, double yi, double zi, double re, F&& f) {
+    for_listed_g<DIM>(L, g, px, py, pz, key, s, xi, yi, zi, re, [](int) { return 0; },
+                      [&](int q, double dist, double dx, double dy, double dz, int) { f(q, dist, dx, dy, dz); });
 }
 
 // ---------------- launchers (defined in the .hip translation units) ----------------

@@ -132,15 +132,21 @@ __global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneT
         int ro = raw_off ? raw_off[s] : 0;
         const double re = c.re_p;
         const double zi = DIM == 3 ? h.pz[s] : 0.0;
-        for_listed<DIM>(nb, g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
-                           [&](int q, double dist, double, double, double) {
-                               if (bc_s[q] != MPS_BC_INNER) return;
+        struct Nbr {
+            int bc;      // boundary class
+            double isq;  // 1 / sqrt(diag_s[q])
+            int row;     // compacted row
+        };
+        for_listed_g<DIM>(nb, g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
+                          [&](int q) { return Nbr{(int) bc_s[q], isq_s[q], row_of[q]}; },
+                          [&](int q, double dist, double, double, double, const Nbr& nq) {
+                               if (nq.bc != MPS_BC_INNER) return;
                                // the off-diagonal value of src/mps.cpp:343-348
                                const double a = c.coefA * wfun(dist, re) / c.rho;
                                const double v = -a;
-                               const double sj = isq_s[q];  // 1 / sqrt(diag_s[q])
+                               const double sj = nq.isq;
                                const double sv = v * (si * sj);
-                               const int cq = row_of[q];
+                               const int cq = nq.row;
                                const int dl = cq - r;
                                const int ad = dl < 0 ? -dl : dl;
                                md = ad > md ? ad : md;

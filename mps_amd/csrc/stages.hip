@@ -412,11 +412,16 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_source_hos(GridView g0, Const
     const double xi = h.px[s], yi = h.py[s], zi = DIM == 3 ? h.pz[s] : 0.0;
     const double vxi = h.vx[s], vyi = h.vy[s], vzi = DIM == 3 ? h.vz[s] : 0.0;
     // neighbours beyond re_p add nothing, so the list (radius >= re_p) covers the walk
-    for_listed<DIM>(nb, g, h.px, h.py, h.pz, h.key[s], s, xi, yi, zi, re,
-                       [&](int q, double dist, double rx, double ry, double rz) {
-                           if (bc_s[q] == MPS_BC_IGNORED) return;
-                           const double ux = h.vx[q] - vxi, uy = h.vy[q] - vyi;
-                           const double uz = DIM == 3 ? h.vz[q] - vzi : 0.0;
+    struct Nbr {
+        int bc;
+        double vx, vy, vz;
+    };
+    for_listed_g<DIM>(nb, g, h.px, h.py, h.pz, h.key[s], s, xi, yi, zi, re,
+                      [&](int q) { return Nbr{(int) bc_s[q], h.vx[q], h.vy[q], DIM == 3 ? h.vz[q] : 0.0}; },
+                      [&](int q, double dist, double rx, double ry, double rz, const Nbr& nq) {
+                           if (nq.bc == MPS_BC_IGNORED) return;
+                           const double ux = nq.vx - vxi, uy = nq.vy - vyi;
+                           const double uz = DIM == 3 ? nq.vz - vzi : 0.0;
                            if (dist < re) b -= c.hos_coef * re * ((rx * ux + ry * uy) + rz * uz) / (dist * dist * dist);
                        });
     b_s[s] = b;
@@ -470,10 +475,10 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_diag_join(GridView g0, Consts
         const double re = c.re_p;
         const double zi = DIM == 3 ? h.pz[s] : 0.0;
         volatile int* vp = parent;
-        for_listed<DIM>(nb, g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
-                           [&](int q, double dist, double, double, double) {
+        for_listed_g<DIM>(nb, g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
+                          [&](int q) { return (int) bc_s[q]; },
+                          [&](int q, double dist, double, double, double, int bq) {
                                if (inner) {
-                                   const int8_t bq = bc_s[q];
                                    if (bq != MPS_BC_IGNORED) {
                                        const double a = c.coefA * wfun(dist, re) / c.rho;
                                        d += a;
@@ -591,15 +596,20 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g0, Consts
     double vx = cur.vx[s], vy = cur.vy[s], vz = DIM == 3 ? cur.vz[s] : 0.0;
     const int t = cur.type[s];
     const int id = cur.id[s];
+    struct NbrTP {
+        int type;
+        double p;
+    };
+    auto nbr_tp = [&](int q) { return NbrTP{(int) cur.type[q], p_s[q]}; };
     double pmi;
     if (bc_s[s] == MPS_BC_IGNORED) {
         pmi = cold.pmin[id];
     } else {
         pmi = p_s[s];
-        for_listed<DIM>(nb, g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, c.re_p,
-                           [&](int q, double, double, double, double) {
-                               if (cur.type[q] == MPS_DUMMY_WALL) return;
-                               if (pmi > p_s[q]) pmi = p_s[q];
+        for_listed_g<DIM>(nb, g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, c.re_p, nbr_tp,
+                          [&](int, double, double, double, double, const NbrTP& nq) {
+                               if (nq.type == MPS_DUMMY_WALL) return;
+                               if (pmi > nq.p) pmi = nq.p;
                            });
         cold.pmin[id] = pmi;
     }
@@ -607,10 +617,10 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g0, Consts
     if (t == MPS_FLUID) {
         const double re = c.re_p;
         double gx = 0.0, gy = 0.0, gz = 0.0;
-        for_listed<DIM>(nb, g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, re,
-                           [&](int q, double dist, double dx, double dy, double dz) {
-                               if (cur.type[q] == MPS_DUMMY_WALL) return;
-                               const double dp = p_s[q] - pmi;
+        for_listed_g<DIM>(nb, g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, re, nbr_tp,
+                          [&](int, double dist, double dx, double dy, double dz, const NbrTP& nq) {
+                               if (nq.type == MPS_DUMMY_WALL) return;
+                               const double dp = nq.p - pmi;
                                const double w = wfun(dist, re);
                                const double d2 = dist * dist;
                                gx += dx * dp * w / d2;

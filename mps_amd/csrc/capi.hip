@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <string>
+#include <cstring>
 #include <vector>
 
 #include "mps_internal.hpp"
@@ -107,6 +108,8 @@ struct mps_handle {
     std::vector<int32_t> ghost_ids, unconn_ids;
     int64_t counts[3] = {0, 0, 0};
     int* pinned_i = nullptr;  // host pinned small buffer
+    Mailbox mb;               // small device-to-host reads of the step
+    double* vmax_dst = nullptr;  // where the next drain puts the scenes' largest speeds
     double* pinned_d = nullptr;
     double* vmax = nullptr;
     int want_raw = 0;
@@ -376,7 +379,6 @@ static int alloc_all(mps_handle* h) {
     h->ev.err = h->dev_counters + 2;
     h->ev.cap = h->ev_cap;
     HIPCHK(h, hipHostMalloc((void**) &h->pinned_i, 16 * sizeof(int), hipHostMallocDefault));
-    HIPCHK(h, hipHostMalloc((void**) &h->A.pin, 4 * sizeof(int), hipHostMallocDefault));
     HIPCHK(h, hipHostMalloc((void**) &h->pinned_d, (16 + ns) * sizeof(double), hipHostMallocDefault));
     HIPCHK(h, fill_int(h->bb.cnt, 0, nc + 1, h->st));
     HIPCHK(h, fill_int(h->w.coll_last, 0, n, h->st));
@@ -425,7 +427,7 @@ static void free_all(mps_handle* h) {
         hipFree(x.dtype);
     }
     if (h->pinned_i) hipHostFree(h->pinned_i);
-    if (h->A.pin) hipHostFree(h->A.pin);
+    mailbox_free(h->mb);
     if (h->pinned_d) hipHostFree(h->pinned_d);
     if (h->cg) cg_destroy(h->cg);
     if (h->st) hipStreamDestroy(h->st);
@@ -451,10 +453,18 @@ static GridView gview(mps_handle* h) {
     return g;
 }
 
-// Reads back the device error mask and event counters (synchronises).
+// Reads back the device error mask and event counters, and the scenes'
+// largest speeds when the step asked for them (waits for the stream's work
+// before the read).
 static int drain(mps_handle* h) {
-    HIPCHK(h, hipMemcpyAsync(h->pinned_i, h->dev_counters, 4 * sizeof(int), hipMemcpyDeviceToHost, h->st));
-    HIPCHK(h, hipStreamSynchronize(h->st));
+    {
+        MailSrc src[2] = {{h->dev_counters, 4 * (int) sizeof(int)}, {h->w.vmax, (int) sizeof(double) * h->nscene}};
+        std::vector<double> buf(2 + (size_t) h->nscene);
+        HIPCHK(h, mailbox_read(h->mb, src, h->vmax_dst ? 2 : 1, buf.data(), h->st));
+        std::memcpy(h->pinned_i, buf.data(), 4 * sizeof(int));
+        if (h->vmax_dst) std::memcpy(h->vmax_dst, buf.data() + 2, sizeof(double) * (size_t) h->nscene);
+        h->vmax_dst = nullptr;
+    }
     const int ng = h->pinned_i[0], nu = h->pinned_i[1], er = h->pinned_i[2];
     if (ng > 0 || nu > 0) {
         std::vector<int32_t> tmp((size_t) std::max(std::min(ng, h->ev_cap), std::min(nu, h->ev_cap)));
@@ -542,7 +552,7 @@ static int step_core(mps_handle* h, int is_export, double* vmax_out, StepTap* ta
     size_t entries = 0;
     tbeg(h, F_FILL);
     HIPCHK(h, assemble_sell(h->dim, g, tab(h), n, h->c, c, h->w, h->A, h->opt.precision == MPS_PREC_MIXED, h->want_raw,
-                            h->ev.err, &rows, &entries, h->st, h->bs[F_FILL]));
+                            h->ev.err, &rows, &entries, h->mb, h->st, h->bs[F_FILL]));
     tend(h);
     std::string msg;
     HIPCHK(h, cg_solve(h->cg, h->A, c, n, h->w.b_s, h->cold.pprev, h->cold.pprev2, h->x_s, h->opt, h->sstats.data(), h->st, msg));
@@ -563,10 +573,7 @@ static int step_core(mps_handle* h, int is_export, double* vmax_out, StepTap* ta
         if ((rc = number_density(h))) return rc;
     }
     HIPCHK(h, launch_courant(h->dim, n, cur(h), tab(h), h->nscene, h->w.vmax, h->st));
-    if (vmax_out) {
-        HIPCHK(h, hipMemcpyAsync(vmax_out, h->w.vmax, sizeof(double) * (size_t) h->nscene, hipMemcpyDeviceToHost,
-                                 h->st));
-    }
+    h->vmax_dst = vmax_out;  // read by the drain that follows
     return MPS_OK;
 }
 

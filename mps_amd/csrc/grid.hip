@@ -14,6 +14,9 @@
 // scalar FP64 code does (no fused multiply-add).
 #pragma clang fp contract(off)
 
+#include <chrono>
+#include <cstring>
+
 #include "mps_internal.hpp"
 
 namespace mps {
@@ -153,6 +156,70 @@ hipError_t fill_int(int* p, int v, size_t n, hipStream_t st) {
 hipError_t fill_double(double* p, double v, size_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_fill<double>, fill_grid(n), dim3(kBlock), 0, st, p, v, n);
     return hipGetLastError();
+}
+
+// ---------------- host reads ----------------
+
+constexpr int kMailSrcMax = 4;
+struct MailArgs {
+    const uint32_t* p[kMailSrcMax];
+    int words[kMailSrcMax];
+    int n;
+};
+
+__global__ void __launch_bounds__(kBlock) k_mail(MailArgs a, uint32_t* __restrict__ box, uint32_t seq) {
+    int o = 1;
+    for (int k = 0; k < a.n; k++) {
+        for (int i = threadIdx.x; i < a.words[k]; i += kBlock) box[o + i] = a.p[k][i];
+        o += a.words[k];
+    }
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(box, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+void mailbox_free(Mailbox& mb) {
+    if (mb.host) hipHostFree(mb.host);
+    mb = Mailbox{};
+}
+
+hipError_t mailbox_read(Mailbox& mb, const MailSrc* src, int nsrc, void* out, hipStream_t st) {
+    if (nsrc > kMailSrcMax) return hipErrorInvalidValue;
+    MailArgs a{};
+    int words = 0;
+    for (int k = 0; k < nsrc; k++) {
+        a.p[k] = (const uint32_t*) src[k].p;
+        a.words[k] = src[k].bytes / 4;
+        words += a.words[k];
+    }
+    a.n = nsrc;
+    hipError_t e;
+    if (words > mb.cap) {
+        // grow only while the stream has no read in flight (the caller waits
+        // for each read before the next)
+        if (mb.host) hipHostFree(mb.host);
+        mb.host = nullptr;
+        const int cap = words < 256 ? 256 : words;
+        if ((e = hipHostMalloc((void**) &mb.host, sizeof(uint32_t) * (size_t) (cap + 1),
+                               hipHostMallocMapped | hipHostMallocCoherent)))
+            return e;
+        if ((e = hipHostGetDevicePointer((void**) &mb.dev, mb.host, 0))) return e;
+        mb.cap = cap;
+        __atomic_store_n(&mb.host[0], mb.seq, __ATOMIC_RELEASE);
+    }
+    const uint32_t want = ++mb.seq;
+    hipLaunchKernelGGL(k_mail, dim3(1), dim3(kBlock), 0, st, a, mb.dev, want);
+    if ((e = hipGetLastError())) return e;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spin = 0; __atomic_load_n(&mb.host[0], __ATOMIC_ACQUIRE) != want; spin++) {
+        if ((spin & 1023) == 1023 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) {
+            if ((e = hipStreamSynchronize(st))) return e;
+            if (__atomic_load_n(&mb.host[0], __ATOMIC_ACQUIRE) != want) return hipErrorUnknown;
+            break;
+        }
+    }
+    std::memcpy(out, mb.host + 1, sizeof(uint32_t) * (size_t) words);
+    return hipSuccess;
 }
 
 // ---------------- build ----------------

@@ -886,7 +886,7 @@ struct MgWork {
     std::vector<int> up_goff[kMgMaxLevels + 1];
     MgLevelDesc up_desc[kMgMaxLevels + 1] = {};
     bool up_desc_valid = false;
-    int* hpin = nullptr;  // pinned host words for the level-size reads
+    Mailbox mb;  // level-size reads
     int tail = 1;                    // first level handled by the tail kernel
     // smoother weight, coarse-correction factor and coarsest sweeps, chosen by
     // measurement: 2D (2M dam break, 64-scene ensemble) 1.0, 1.4, 1; 3D
@@ -942,7 +942,7 @@ void mg_destroy(MgWork* m) {
     void* ps[] = {m->nd,  m->err,   m->cst1,  m->head,     m->hscan, m->rkey,
                   m->res0, m->xx0, m->flag, m->pscan, m->scan_tmp, m->desc};
     for (void* p : ps) hipFree(p);
-    if (m->hpin) hipHostFree(m->hpin);
+    mailbox_free(m->mb);
     delete m;
 }
 
@@ -991,12 +991,8 @@ static hipError_t mg_scan(MgWork* m, const int* in, int* out, int len, hipStream
 }
 
 static hipError_t mg_read_int(MgWork* m, const int* dev, int* host, hipStream_t st) {
-    hipError_t e;
-    if (!m->hpin && (e = hipHostMalloc((void**) &m->hpin, 16 * sizeof(int), hipHostMallocDefault))) return e;
-    if ((e = hipMemcpyAsync(m->hpin, dev, sizeof(int), hipMemcpyDeviceToHost, st))) return e;
-    if ((e = hipStreamSynchronize(st))) return e;
-    *host = m->hpin[0];
-    return hipSuccess;
+    const MailSrc src{dev, (int) sizeof(int)};
+    return mailbox_read(m->mb, &src, 1, host, st);
 }
 
 // Host-to-device copy of a small table unless the device already holds it.

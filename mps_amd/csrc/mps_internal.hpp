@@ -648,6 +648,25 @@ size_t scan_tmp_ints(int m);
 hipError_t fill_int(int* p, int v, size_t n, hipStream_t st);
 hipError_t fill_double(double* p, double v, size_t n, hipStream_t st);
 
+// Small device-to-host reads that skip the copy engine and the stream wait:
+// a one-block kernel stores the words into pinned host memory, then a
+// sequence number with a system-scope release; the host spins on the
+// number (after 2 ms it waits on the stream instead, which also reports a
+// failed launch).  Reads are ordered with the stream's other work.
+struct Mailbox {
+    uint32_t* host = nullptr;  // [0] sequence, then the words (pinned, mapped)
+    uint32_t* dev = nullptr;   // device address of host
+    int cap = 0;               // words after the sequence
+    uint32_t seq = 0;
+};
+struct MailSrc {
+    const void* p;  // device address, 4-byte aligned
+    int bytes;      // multiple of 4
+};
+// Reads the sources, back to back, into out (host memory).
+hipError_t mailbox_read(Mailbox& mb, const MailSrc* src, int nsrc, void* out, hipStream_t st);
+void mailbox_free(Mailbox& mb);
+
 struct BuildBufs {
     int* cnt;        // ncell + 1 counters (kept at zero between builds)
     int* cs;         // ncell + 2 cell starts
@@ -748,7 +767,6 @@ struct Sell {
     int use16;           // every |column - row| fits 16 bits
     int* maxdelta;       // device: max |column - row| of the last assembly
     int wide_hint;       // host: the last assembly needed 32-bit columns (write them in the first pass)
-    int* pin;            // 4 pinned host words for the size reads of the assembly
     double* val64;       // scaled off-diagonal values (FP64)
     float* val32;        // scaled off-diagonal values (FP32), mixed path
     int* row_slot;       // rows -> slot
@@ -764,7 +782,7 @@ struct Sell {
 
 hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
                          const StepScratch& w, Sell& A, int want_val32, int want_raw, int* err,
-                         int* host_rows, size_t* host_entries, hipStream_t st, int bs);
+                         int* host_rows, size_t* host_entries, Mailbox& mb, hipStream_t st, int bs);
 
 // mg.hip — aggregation multigrid preconditioner (single segment)
 struct MgWork;

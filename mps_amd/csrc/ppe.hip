@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneT
 hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
                          const StepScratch& w,
                          Sell& A, int want_val32, int want_raw, int* err, int* host_rows, size_t* host_entries,
-                         hipStream_t st, int bs) {
+                         Mailbox& mb, hipStream_t st, int bs) {
     (void) err;
     hipError_t e;
     const int nsl_max = (n + 63) / 64;
@@ -212,13 +212,11 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
                            A.slice_w);
     e = scan_exclusive(A.slice_w, A.slice_off, nsl_max, w.scan_tmp, st);
     if (e != hipSuccess) return e;
-    int* hv = A.pin;  // pinned: the copies need no staging
-    e = hipMemcpyAsync(&hv[0], w.row_of + n, sizeof(int), hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess) return e;
-    e = hipMemcpyAsync(&hv[1], A.slice_off + nsl_max, sizeof(int), hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return e;
+    int hv[2];
+    {
+        const MailSrc src[2] = {{w.row_of + n, (int) sizeof(int)}, {A.slice_off + nsl_max, (int) sizeof(int)}};
+        if ((e = mailbox_read(mb, src, 2, hv, st)) != hipSuccess) return e;
+    }
     const int rows = hv[0];
     const size_t entries = (size_t) hv[1] * 64;
     *host_rows = rows;
@@ -288,10 +286,8 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
             hipLaunchKernelGGL((k_fill_sell<3, false>), fg, fb, 0, st, FILL_ARGS);
 #undef FILL_ARGS
         // 16-bit column offsets when every row's neighbours lie within +-32767 rows
-        if ((e = hipMemcpyAsync(&A.pin[2], A.maxdelta, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess)
-            return e;
-        if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-        md = A.pin[2];
+        const MailSrc src{A.maxdelta, (int) sizeof(int)};
+        if ((e = mailbox_read(mb, &src, 1, &md, st)) != hipSuccess) return e;
         if (md <= 32767 || write_col) break;
     }
     A.use16 = md <= 32767 ? 1 : 0;
@@ -679,6 +675,7 @@ struct CgWork {
     size_t part_cap = 0;
     CgScalars* sc = nullptr;  // device scalars, one per segment
     CgScalars* host = nullptr;
+    Mailbox mb;  // scalar reads
     int* blk_off = nullptr;   // device, nseg + 1
     int* row_off = nullptr;   // device, nseg + 1
     int* h_blk = nullptr;     // pinned host mirrors
@@ -730,6 +727,7 @@ void cg_destroy(CgWork* cw) {
     hipHostFree(cw->host);
     hipHostFree(cw->h_blk);
     hipHostFree(cw->h_row);
+    mailbox_free(cw->mb);
     for (int k = 0; k < 2; k++)
         for (int j = 0; j < CgWork::kGraphSlots; j++)
             if (cw->graph[k][j]) hipGraphExecDestroy(cw->graph[k][j]);
@@ -790,10 +788,8 @@ static hipError_t cg_layout(CgWork* cw, const Sell& A, hipStream_t st) {
     } else {
         hipLaunchKernelGGL(k_seg_rows, dim3((A.nseg + 1 + 255) / 256), dim3(256), 0, st, A.nseg, A.scene_key0, A.cs,
                            A.row_of, cw->row_off);
-        if ((e = hipMemcpyAsync(cw->h_row, cw->row_off, sizeof(int) * (size_t) (A.nseg + 1), hipMemcpyDeviceToHost,
-                                st)))
-            return e;
-        if ((e = hipStreamSynchronize(st))) return e;
+        const MailSrc src{cw->row_off, (int) sizeof(int) * (A.nseg + 1)};
+        if ((e = mailbox_read(cw->mb, &src, 1, cw->h_row, st))) return e;
     }
     cw->h_blk[0] = 0;
     for (int s = 0; s < A.nseg; s++) cw->h_blk[s + 1] = cw->h_blk[s] + cg_blocks(cw->h_row[s + 1] - cw->h_row[s]);
@@ -910,8 +906,8 @@ static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* c
         } else {
             enqueue_chunk<T, IDX, WEIGHTED>(cw, A, col, val, y, r, p, s, w, chunk, st);
         }
-        if ((e = hipMemcpyAsync(cw->host, cw->sc, sbytes, hipMemcpyDeviceToHost, st))) return e;
-        if ((e = hipStreamSynchronize(st))) return e;
+        const MailSrc src{cw->sc, (int) sbytes};
+        if ((e = mailbox_read(cw->mb, &src, 1, cw->host, st))) return e;
         if (all_done(cw)) return hipSuccess;
     }
 }
@@ -928,9 +924,8 @@ static hipError_t reduce_and_read(CgWork* cw, int nv, hipStream_t st) {
         hipLaunchKernelGGL(k_reduce_parts<1>, dim3(cw->nseg), dim3(kBlock), 0, st, sg, cw->part, cw->sc);
     else
         hipLaunchKernelGGL(k_reduce_parts<2>, dim3(cw->nseg), dim3(kBlock), 0, st, sg, cw->part, cw->sc);
-    hipError_t e = hipMemcpyAsync(cw->host, cw->sc, sizeof(CgScalars) * (size_t) cw->nseg, hipMemcpyDeviceToHost, st);
-    if (e) return e;
-    return hipStreamSynchronize(st);
+    const MailSrc src{cw->sc, (int) (sizeof(CgScalars) * (size_t) cw->nseg)};
+    return mailbox_read(cw->mb, &src, 1, cw->host, st);
 }
 
 static void launch_residual(CgWork* cw, const Sell& A, hipStream_t st) {

@@ -1090,14 +1090,19 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
             }
         }
     }
-    // true FP64 residual of the final iterate
-    launch_residual(cw, A, st);
-    if ((e = reduce_and_read(cw, 2, st))) return e;
+    // true FP64 residual of the final iterate: the refinement loop (and a
+    // solve that needed no iteration) already computed it from this iterate
+    const bool fresh = o.precision != MPS_PREC_FP64 || all_conv;
+    if (!fresh) {
+        launch_residual(cw, A, st);
+        if ((e = reduce_and_read(cw, 2, st))) return e;
+        for (int s = 0; s < nseg; s++) rho2[s] = cw->host[s].red[1];
+    }
     bool all_ok = true;
     for (int s = 0; s < nseg; s++) {
         stats[s].iterations = total_it[s];
         stats[s].refinements = refinements[s];
-        stats[s].rel_residual = bn2[s] > 0 ? sqrt(cw->host[s].red[1] / bn2[s]) : 0.0;
+        stats[s].rel_residual = bn2[s] > 0 ? sqrt(rho2[s] / bn2[s]) : 0.0;
         stats[s].converged = conv[s] ? 1 : 0;
         all_ok = all_ok && conv[s];
     }

@@ -174,6 +174,7 @@ __device__ __forceinline__ int mg_stencil_of_cells(int k0, int k1, MgDims d) {
 // (LDS, entry order), then one thread per (aggregate, bin) adds the rows'
 // bins in row order.  Row reads are coalesced; the sums are in a fixed order.
 constexpr int kAsm1Rows = 256;  // threads per block, rows per chunk
+constexpr int kMgBatch = 9;     // ELL entries per round of loads in the coarse assembly (W is a multiple)
 
 template <typename T, typename IDX, int W, int A>
 __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble1(int n1, const int* __restrict__ cst1,
@@ -353,6 +354,7 @@ __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble(int np, const int* __
                                                            const MgDims* __restrict__ dp, int* __restrict__ colp,
                                                            T* __restrict__ valp, T* __restrict__ diagp,
                                                            T* __restrict__ dinvp, int* __restrict__ err) {
+    static_assert(W % kMgBatch == 0, "stencil width is a multiple of the batch");
     constexpr int A = kAsm1Rows / C;
     __shared__ double s_bin[kAsm1Rows][W + 1];  // [W]: the child's diagonal and entries inside P
     __shared__ int s_nbr[A][W];
@@ -371,22 +373,43 @@ __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble(int np, const int* __
             const MgDims d = dp[scenep[P]];
             const int key = coordp[P];
             s_bin[tid][W] = (double) diag[K];
-            for (int j = 0; j < W; j++) {
-                const int c = col[(size_t) j * cap + K];
-                if (c < 0) break;
-                const double v = (double) val[(size_t) j * cap + K];
-                const int P2 = agg[c];
-                if (P2 == P) {
-                    s_bin[tid][W] += v;
-                    continue;
+            // the row's entries in rounds of kMgBatch: columns and values,
+            // then their parents, then the parents' cells, each round's
+            // loads issued together (padding, column -1, is read at K and
+            // ends the row)
+            for (int j0 = 0; j0 < W; j0 += kMgBatch) {
+                int cj[kMgBatch], pj[kMgBatch], kj[kMgBatch];
+                double vj[kMgBatch];
+#pragma unroll
+                for (int q = 0; q < kMgBatch; q++) {
+                    cj[q] = col[(size_t) (j0 + q) * cap + K];
+                    vj[q] = (double) val[(size_t) (j0 + q) * cap + K];
                 }
-                const int sj = mg_stencil_of_cells<W>(key, coordp[P2], d);
-                if (sj < 0) {
-                    atomicOr(err, 1);
-                    continue;
+#pragma unroll
+                for (int q = 0; q < kMgBatch; q++) pj[q] = agg[cj[q] >= 0 ? cj[q] : K];
+#pragma unroll
+                for (int q = 0; q < kMgBatch; q++) kj[q] = coordp[pj[q]];
+                bool end = false;
+#pragma unroll
+                for (int q = 0; q < kMgBatch; q++) {
+                    if (cj[q] < 0) {
+                        end = true;
+                        break;
+                    }
+                    const int P2 = pj[q];
+                    if (P2 == P) {
+                        s_bin[tid][W] += vj[q];
+                        continue;
+                    }
+                    const int sj = mg_stencil_of_cells<W>(key, kj[q], d);
+                    if (sj < 0) {
+                        atomicOr(err, 1);
+                        continue;
+                    }
+                    s_bin[tid][sj] += vj[q];
+                    s_nbr[a][sj] = P2;  // every writer of this place writes the same parent
                 }
-                s_bin[tid][sj] += v;
-                s_nbr[a][sj] = P2;  // every writer of this place writes the same parent
+                if (end) break;
             }
         }
     }

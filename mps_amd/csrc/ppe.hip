@@ -847,8 +847,8 @@ static bool all_done(const CgWork* cw) {
 // Runs chunks of CG iterations (each a captured graph, or enqueued directly)
 // until every segment is done; iterations past convergence return at once.
 // pred > 0 (the iterations this solve took in the previous step): chunks
-// cover the predicted remainder (even lengths, at most 64), then, once the
-// prediction is used up, 2, 4, 8, ... 64; else chunks of the check interval.
+// cover the prediction rounded down to even (at most 64 each), then, once it
+// is used up, 2, 4, 8, ... 64; else chunks of the check interval.
 template <typename T, typename IDX, bool WEIGHTED>
 static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* col, const T* val, T* y, T* r, T* p,
                               T* s, T* w, const mps_solver_options& o, int pred, hipStream_t st) {
@@ -875,7 +875,7 @@ static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* c
     for (;;) {
         int chunk = interval;
         if (pred > 0) {
-            const int rem = (pred - 1 - launched) & ~1;  // one less than predicted, even
+            const int rem = (pred - launched) & ~1;  // the prediction, rounded down to even
             if (rem >= 2) {
                 chunk = rem < 64 ? rem : 64;
             } else {

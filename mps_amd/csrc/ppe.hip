@@ -553,15 +553,35 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_step(Seg sg, T* __restrict__ y,
 }
 
 // One block per segment: sums its NV-interleaved partials into sc[seg].red.
-template <int NV>
+// Each segment's NV partial sums into red[OFF .. OFF + NV).
+template <int NV, int OFF = 0>
 __global__ void __launch_bounds__(kBlock) k_reduce_parts(Seg sg, const double* __restrict__ part, CgScalars* scs) {
+    static_assert(OFF + NV <= 3, "three result slots");
     __shared__ double red[3 * kBlock / 64];
     const int sgi = blockIdx.x;
     const int b0 = sg.blk_off[sgi], nb = sg.blk_off[sgi + 1] - b0;
     double o[NV];
     reduce_partials<NV>(part + (size_t) NV * b0, nb, o, red);
     if (threadIdx.x == 0)
-        for (int k = 0; k < NV; k++) scs[sgi].red[k] = o[k];
+        for (int k = 0; k < NV; k++) scs[sgi].red[OFF + k] = o[k];
+}
+
+// Inner-solve set-up on the device, so the host need not read ||r32||^2
+// first: segment s's threshold is ||r32||^2 (k_inner_init's partials) times
+// the reduction the host stored in thr; a segment whose residual is zero
+// does not iterate.
+__global__ void __launch_bounds__(kBlock) k_inner_prep(Seg sg, const double* __restrict__ part, CgScalars* scs) {
+    __shared__ double red[3 * kBlock / 64];
+    const int sgi = blockIdx.x;
+    const int b0 = sg.blk_off[sgi], nb = sg.blk_off[sgi + 1] - b0;
+    double o[1];
+    reduce_partials<1>(part + b0, nb, o, red);
+    if (threadIdx.x == 0) {
+        CgScalars* sc = scs + sgi;
+        sc->red[0] = o[0];
+        sc->thr = o[0] * sc->thr;
+        if (!(o[0] > 0.0)) sc->done = 1;
+    }
 }
 
 // FP64 init: yhat = x0 sqrt(d), bhat = S b; partial ||b||^2.
@@ -992,12 +1012,16 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
 
     hipLaunchKernelGGL(k_cg_init, gr, bl, 0, st, sg, A.row_slot, h.id, b_s, o.warm_start ? pprev_id : nullptr,
                        o.warm_start == 2 ? pprev2_id : nullptr, A.sdiag, cw->y, cw->bh, cw->part);
-    if ((e = reduce_and_read(cw, 1, st))) return e;
+    // ||b||^2 into red[2], then the initial residual: one read for both
+    hipLaunchKernelGGL((k_reduce_parts<1, 2>), dim3(nseg), dim3(kBlock), 0, st, sg, cw->part, cw->sc);
+    launch_residual(cw, A, st);
+    if ((e = reduce_and_read(cw, 2, st))) return e;
     std::vector<double> bn2(nseg), thr(nseg), rho2(nseg);
     std::vector<int> segrows(nseg), conv(nseg), total_it(nseg, 0), refinements(nseg, 0), max_it(nseg);
     for (int s = 0; s < nseg; s++) {
         segrows[s] = cw->h_row[s + 1] - cw->h_row[s];
-        bn2[s] = cw->host[s].red[0];
+        bn2[s] = cw->host[s].red[2];
+        rho2[s] = cw->host[s].red[1];
         stats[s].rows = segrows[s];
         stats[s].rhs_norm = sqrt(bn2[s]);
         thr[s] = o.tolerance * o.tolerance * bn2[s];
@@ -1015,12 +1039,12 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
                 if (bn2[s] == 0.0 && segrows[s] > 0)
                     if ((e = fill_double(cw->y + cw->h_row[s], 0.0, (size_t) segrows[s], st))) return e;
         }
+        launch_residual(cw, A, st);
+        if ((e = reduce_and_read(cw, 2, st))) return e;
+        for (int s = 0; s < nseg; s++) rho2[s] = cw->host[s].red[1];
     }
-    launch_residual(cw, A, st);
-    if ((e = reduce_and_read(cw, 2, st))) return e;
     bool all_conv = true;
     for (int s = 0; s < nseg; s++) {
-        rho2[s] = cw->host[s].red[1];
         conv[s] = bn2[s] == 0.0 || rho2[s] <= thr[s];
         stats[s].initial_residual = bn2[s] > 0 ? sqrt(rho2[s] / bn2[s]) : 0.0;
         all_conv = all_conv && conv[s];
@@ -1054,12 +1078,10 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
                 active = active || (!conv[s] && refinements[s] < max_ref && total_it[s] < max_it[s]);
             if (!active) break;
             hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, sg, cw->r64, cw->r32, cw->e32, cw->part);
-            if ((e = reduce_and_read(cw, 1, st))) return e;
             std::vector<int> run(nseg, 0);
             for (int s = 0; s < nseg; s++) {
-                const double init = cw->host[s].red[0];
                 CgScalars z{};
-                run[s] = !conv[s] && refinements[s] < max_ref && total_it[s] < max_it[s] && init > 0.0;
+                run[s] = !conv[s] && refinements[s] < max_ref && total_it[s] < max_it[s];
                 // reduce the inner residual by eta, or by what the outer loop
                 // still needs when that is less
                 double red2 = eta * eta;
@@ -1067,12 +1089,13 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
                     const double need = thr[s] / rho2[s];
                     if (need > red2) red2 = need * 0.25;
                 }
-                z.thr = init * red2;
+                z.thr = red2;  // times ||r32||^2 in k_inner_prep
                 z.max_iter = max_it[s] - total_it[s];
                 z.done = run[s] ? 0 : 1;
                 cw->host[s] = z;
             }
             if ((e = upload_scalars(cw, st))) return e;
+            hipLaunchKernelGGL(k_inner_prep, dim3(nseg), dim3(kBlock), 0, st, sg, cw->part, cw->sc);
             if ((e = run_cg<float, false>(cw, 1, round, A, A.val32, cw->e32, cw->r32, cw->p32, cw->s32, cw->w32, o,
                                           st)))
                 return e;

@@ -38,6 +38,7 @@ constexpr int kMgMaxLevels = 16;
 constexpr int kMgCoarse = 64;  // rows of the coarsest level at most
 
 static inline dim3 mg_grid(int n) { return dim3((unsigned) std::max(1, (n + kBlock - 1) / kBlock)); }
+static inline dim3 fill_grid_mg(int n) { return dim3((unsigned) std::min(4096, std::max(1, (n + kBlock - 1) / kBlock))); }
 
 // The cycle is skipped once every segment's CG has converged (a segment
 // still iterating needs it; converged segments ignore its output).
@@ -338,6 +339,97 @@ __global__ void __launch_bounds__(kBlock) k_mg_link(int n, const int* __restrict
     const int pr = pscan[mg_parent_key(K, coord, scene, d, dp, goffp, slot)];
     agg[K] = pr;
     child[(size_t) slot * capp + pr] = K;
+}
+
+// Tables for building the structure of all coarse levels in one pass each
+// (levels lo..hi; per level its scenes' grids and first dense indices, the
+// first cell of its grid in the joint flag array of levels 2.., the joint
+// scan's value there, the first row of its rows in the joint row list of
+// levels 1.., and its row arrays).
+struct MgTabs {
+    const MgDims* sdims[kMgMaxLevels + 1];
+    const int* goff[kMgMaxLevels + 1];
+    int foff[kMgMaxLevels + 2];
+    int pbase[kMgMaxLevels + 2];
+    int rbase[kMgMaxLevels + 2];
+    int cbase[kMgMaxLevels + 2];
+    int* coord[kMgMaxLevels + 1];
+    int* scene[kMgMaxLevels + 1];
+    int* agg[kMgMaxLevels + 1];
+    int* child[kMgMaxLevels + 1];
+    int cap[kMgMaxLevels + 1];
+    int lo, hi, nscene, C;
+};
+
+// The level (lo..hi) whose range [X[l], X[l + 1]) holds t.
+__device__ __forceinline__ int mg_level_of(const int* X, int lo, int hi, int t) {
+    int l = lo;
+    while (l < hi && t >= X[l + 1]) l++;
+    return l;
+}
+
+// Every level-1 cell flags the cell above it on each level 2..hi (level l
+// merges 2^(l-1) level-1 cells per axis: its coordinates are c >> (l - 1)).
+__global__ void __launch_bounds__(kBlock) k_mg_mark_all(int n1, const int* __restrict__ coord1,
+                                                        const int* __restrict__ scene1,
+                                                        const MgDims* __restrict__ d1, MgTabs T,
+                                                        int* __restrict__ flag) {
+    const int K = blockIdx.x * blockDim.x + threadIdx.x;
+    if (K >= n1) return;
+    const int sc = scene1[K];
+    int cx, cy, cz;
+    mg_decode(coord1[K], d1[sc], cx, cy, cz);
+    for (int l = 2; l <= T.hi; l++) {
+        const int sh = l - 1;
+        const MgDims q = T.sdims[l][sc];
+        flag[T.foff[l] + T.goff[l][sc] + (cx >> sh) + q.x * ((cy >> sh) + q.y * (cz >> sh))] = 1;
+    }
+}
+
+// Rows of each level 2..hi: differences of the joint scan at the grids' starts.
+__global__ void k_mg_sizes(const int* __restrict__ pscan, MgTabs T, int* __restrict__ out) {
+    const int l = 2 + (int) threadIdx.x;
+    if (l <= T.hi) out[l] = pscan[T.foff[l + 1]] - pscan[T.foff[l]];
+}
+
+// Each flagged grid cell of levels 2..hi becomes a row: its cell and scene.
+__global__ void __launch_bounds__(kBlock) k_mg_parents_all(const int* __restrict__ flag,
+                                                           const int* __restrict__ pscan, MgTabs T) {
+    const int P = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P >= T.foff[T.hi + 1] || !flag[P]) return;
+    const int l = mg_level_of(T.foff, 2, T.hi, P);
+    const int loc = P - T.foff[l];
+    const int sc = mg_bucket(T.goff[l], T.nscene, loc);
+    const int r = pscan[P] - T.pbase[l];
+    T.coord[l][r] = loc - T.goff[l][sc];
+    T.scene[l][r] = sc;
+}
+
+// Child tables of levels 2..hi to -1 (empty slots).
+__global__ void __launch_bounds__(kBlock) k_mg_clear_children(MgTabs T) {
+    const int tot = T.cbase[T.hi + 1];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
+        const int l = mg_level_of(T.cbase, 2, T.hi, i);
+        T.child[l][i - T.cbase[l]] = -1;
+    }
+}
+
+// Rows of levels lo..hi - 1: agg = parent row on the level above, and the
+// parent's child slot (cx&1) + 2(cy&1) + 4(cz&1) = the row.
+__global__ void __launch_bounds__(kBlock) k_mg_link_all(const int* __restrict__ pscan, MgTabs T) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T.rbase[T.hi]) return;
+    const int l = mg_level_of(T.rbase, T.lo, T.hi - 1, t);
+    const int K = t - T.rbase[l];
+    const int sc = T.scene[l][K];
+    int cx, cy, cz;
+    mg_decode(T.coord[l][K], T.sdims[l][sc], cx, cy, cz);
+    const int slot = (cx & 1) + 2 * (cy & 1) + 4 * (cz & 1);
+    const MgDims q = T.sdims[l + 1][sc];
+    const int key = T.goff[l + 1][sc] + (cx >> 1) + q.x * ((cy >> 1) + q.y * (cz >> 1));
+    const int pr = pscan[T.foff[l + 1] + key] - T.pbase[l + 1];
+    T.agg[l][K] = pr;
+    T.child[l + 1][(size_t) slot * T.cap[l + 1] + pr] = K;
 }
 
 // Galerkin level l + 1 from level l: parent P sums its children's rows.  A
@@ -896,6 +988,7 @@ struct MgWork {
     void* res0 = nullptr;
     void* xx0 = nullptr;
     int gcap = 0;                    // dense grid scratch
+    int* lsz = nullptr;              // device: rows per level from the joint scan
     int* flag = nullptr;
     int* pscan = nullptr;
     int* scan_tmp = nullptr;
@@ -963,7 +1056,7 @@ void mg_destroy(MgWork* m) {
         hipFree(m->d_goff[l]);
     }
     void* ps[] = {m->nd,  m->err,   m->cst1,  m->head,     m->hscan, m->rkey,
-                  m->res0, m->xx0, m->flag, m->pscan, m->scan_tmp, m->desc};
+                  m->res0, m->xx0, m->flag, m->pscan, m->scan_tmp, m->desc, m->lsz};
     for (void* p : ps) hipFree(p);
     mailbox_free(m->mb);
     delete m;
@@ -1056,6 +1149,8 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
     }
     if (!m->nd) {
         if ((e = mg_alloc(&m->nd, sizeof(int) * (kMgMaxLevels + 1)))) return e;
+        if ((e = mg_alloc(&m->lsz, sizeof(int) * (kMgMaxLevels + 1)))) return e;
+        if ((e = fill_int(m->lsz, 0, kMgMaxLevels + 1, st))) return e;
         if ((e = mg_alloc(&m->err, sizeof(int)))) return e;
         if ((e = mg_alloc(&m->desc, sizeof(MgLevelDesc) * (kMgMaxLevels + 1)))) return e;
     }
@@ -1118,23 +1213,51 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
         hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 27, 16>), dim3((n1 + 15) / 16), dim3(kAsm1Rows), 0, st, n1,
                            m->cst1, A.slice_off, fcol, fval, m->agg[0], m->coord[1], m->scene[1], m->d_sdims[1], m->cap[1],
                            m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1], m->err);
-    // stop at <= 64 rows, or once every scene's grid is a single cell
+    // Coarse levels 2..: level l + 1 merges 2x2 (2x2x2) cells of level l.
+    // Their structure is built for all levels at once: the grid dims of
+    // every level that still shrinks, one flag per grid cell set by the
+    // level-1 cells under it, one scan over all the grids and one read of
+    // the level sizes; then rows, child tables and parent links of the kept
+    // levels in one launch each, and the Galerkin products level by level.
+    // Levels stop at <= 64 rows, or once every scene's grid is a single cell.
     const int coarse = std::max(kMgCoarse, ns);
-    int l = 1;
-    while (m->n[l] > coarse && l < kMgMaxLevels) {
-        std::vector<MgDims>& dp = m->sdims[l + 1];
-        std::vector<int>& go = m->goff[l + 1];
-        dp.resize(ns);
-        go.assign(ns + 1, 0);
-        bool shrinks = false;
-        for (int k = 0; k < ns; k++) {
-            const MgDims d = m->sdims[l][k];
-            dp[k] = MgDims{(d.x + 1) / 2, (d.y + 1) / 2, (d.z + 1) / 2};
-            shrinks = shrinks || d.x > 1 || d.y > 1 || d.z > 1;
-            go[k + 1] = go[k] + dp[k].x * dp[k].y * dp[k].z;
+    MgTabs tb{};
+    tb.nscene = ns;
+    tb.C = m->C;
+    int lmax = 1;
+    {
+        int f = 0;
+        for (int k = 1; k < kMgMaxLevels; k++) {
+            std::vector<MgDims>& dp = m->sdims[k + 1];
+            std::vector<int>& go = m->goff[k + 1];
+            dp.resize(ns);
+            go.assign(ns + 1, 0);
+            bool shrinks = false;
+            for (int q = 0; q < ns; q++) {
+                const MgDims d = m->sdims[k][q];
+                dp[q] = MgDims{(d.x + 1) / 2, (d.y + 1) / 2, (d.z + 1) / 2};
+                shrinks = shrinks || d.x > 1 || d.y > 1 || d.z > 1;
+                go[q + 1] = go[q] + dp[q].x * dp[q].y * dp[q].z;
+            }
+            if (!shrinks) break;
+            if ((e = mg_upload(m->d_sdims[k + 1], dp, m->up_sdims[k + 1], st)) ||
+                (e = mg_upload(m->d_goff[k + 1], go, m->up_goff[k + 1], st)))
+                return e;
+            tb.foff[k + 1] = f;
+            f += go[ns];
+            lmax = k + 1;
         }
-        if (!shrinks) break;
-        const int G = go[ns];
+        tb.foff[lmax + 1] = f;
+    }
+    for (int k = 1; k <= lmax; k++) {
+        tb.sdims[k] = m->d_sdims[k];
+        tb.goff[k] = m->d_goff[k];
+    }
+    tb.lo = 2;
+    tb.hi = lmax;
+    int l = 1;
+    if (lmax >= 2 && n1 > coarse) {
+        const int G = tb.foff[lmax + 1];
         if (G + 1 > m->gcap) {
             hipFree(m->flag);
             hipFree(m->pscan);
@@ -1145,35 +1268,56 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
                 return e;
             m->gcap = cap;
         }
-        if ((e = mg_upload(m->d_sdims[l + 1], dp, m->up_sdims[l + 1], st)) ||
-            (e = mg_upload(m->d_goff[l + 1], go, m->up_goff[l + 1], st)))
-            return e;
         if ((e = fill_int(m->flag, 0, (size_t) G, st))) return e;
-        hipLaunchKernelGGL(k_mg_mark, mg_grid(m->n[l]), dim3(kBlock), 0, st, m->n[l], m->coord[l], m->scene[l],
-                           m->d_sdims[l], m->d_sdims[l + 1], m->d_goff[l + 1], m->flag);
+        hipLaunchKernelGGL(k_mg_mark_all, mg_grid(n1), dim3(kBlock), 0, st, n1, m->coord[1], m->scene[1],
+                           m->d_sdims[1], tb, m->flag);
         if ((e = mg_scan(m, m->flag, m->pscan, G, st))) return e;
-        int np = 0;
-        if ((e = mg_read_int(m, m->pscan + G, &np, st))) return e;
-        if ((e = mg_reserve_level(m, l + 1, np))) return e;
-        hipLaunchKernelGGL(k_mg_parents, mg_grid(G), dim3(kBlock), 0, st, G, m->flag, m->pscan, m->d_goff[l + 1], ns,
-                           m->coord[l + 1], m->scene[l + 1]);
-        if ((e = fill_int(m->child[l + 1], -1, (size_t) m->C * m->cap[l + 1], st))) return e;
-        hipLaunchKernelGGL(k_mg_link, mg_grid(m->n[l]), dim3(kBlock), 0, st, m->n[l], m->coord[l], m->scene[l],
-                           m->d_sdims[l], m->d_sdims[l + 1], m->d_goff[l + 1], m->pscan, m->agg[l],
-                           m->child[l + 1], m->cap[l + 1]);
-        const dim3 pg((np + kAsm1Rows / (dim == 2 ? 4 : 8) - 1) / (kAsm1Rows / (dim == 2 ? 4 : 8)));
-        if (dim == 2)
-            hipLaunchKernelGGL((k_mg_assemble<T, 9, 4>), pg, ab, 0, st, np, m->child[l + 1], m->cap[l + 1], m->col[l],
-                               (const T*) m->val[l], (const T*) m->diag[l], m->cap[l], m->agg[l], m->coord[l + 1],
-                               m->scene[l + 1], m->d_sdims[l + 1], m->col[l + 1], (T*) m->val[l + 1],
-                               (T*) m->diag[l + 1], (T*) m->dinv[l + 1], m->err);
-        else
-            hipLaunchKernelGGL((k_mg_assemble<T, 27, 8>), pg, ab, 0, st, np, m->child[l + 1], m->cap[l + 1], m->col[l],
-                               (const T*) m->val[l], (const T*) m->diag[l], m->cap[l], m->agg[l], m->coord[l + 1],
-                               m->scene[l + 1], m->d_sdims[l + 1], m->col[l + 1], (T*) m->val[l + 1],
-                               (T*) m->diag[l + 1], (T*) m->dinv[l + 1], m->err);
-        m->n[l + 1] = np;
-        l++;
+        hipLaunchKernelGGL(k_mg_sizes, dim3(1), dim3(64), 0, st, m->pscan, tb, m->lsz);
+        int sz[kMgMaxLevels + 1] = {};
+        {
+            const MailSrc src{m->lsz, (int) sizeof(int) * (kMgMaxLevels + 1)};
+            if ((e = mailbox_read(m->mb, &src, 1, sz, st))) return e;
+        }
+        for (int k = 2; k <= lmax; k++) m->n[k] = sz[k];
+        while (l < lmax && m->n[l] > coarse) l++;
+        const int L = l;
+        // the kept levels' rows, then their tables
+        for (int k = 2; k <= L; k++)
+            if ((e = mg_reserve_level(m, k, m->n[k]))) return e;
+        tb.hi = L;
+        tb.pbase[2] = 0;
+        tb.cbase[2] = 0;
+        for (int k = 2; k <= L; k++) {
+            tb.pbase[k + 1] = tb.pbase[k] + m->n[k];
+            tb.cbase[k + 1] = tb.cbase[k] + m->C * m->cap[k];
+        }
+        tb.lo = 1;
+        tb.rbase[1] = 0;
+        for (int k = 1; k <= L; k++) {
+            tb.rbase[k + 1] = tb.rbase[k] + m->n[k];
+            tb.coord[k] = m->coord[k];
+            tb.scene[k] = m->scene[k];
+            tb.agg[k] = m->agg[k];
+            tb.child[k] = m->child[k];
+            tb.cap[k] = m->cap[k];
+        }
+        hipLaunchKernelGGL(k_mg_parents_all, mg_grid(tb.foff[L + 1]), dim3(kBlock), 0, st, m->flag, m->pscan, tb);
+        hipLaunchKernelGGL(k_mg_clear_children, fill_grid_mg(tb.cbase[L + 1]), dim3(kBlock), 0, st, tb);
+        hipLaunchKernelGGL(k_mg_link_all, mg_grid(tb.rbase[L]), dim3(kBlock), 0, st, m->pscan, tb);
+        for (int k = 1; k < L; k++) {
+            const int np = m->n[k + 1];
+            const dim3 pg((np + kAsm1Rows / m->C - 1) / (kAsm1Rows / m->C));
+            if (dim == 2)
+                hipLaunchKernelGGL((k_mg_assemble<T, 9, 4>), pg, ab, 0, st, np, m->child[k + 1], m->cap[k + 1],
+                                   m->col[k], (const T*) m->val[k], (const T*) m->diag[k], m->cap[k], m->agg[k],
+                                   m->coord[k + 1], m->scene[k + 1], m->d_sdims[k + 1], m->col[k + 1],
+                                   (T*) m->val[k + 1], (T*) m->diag[k + 1], (T*) m->dinv[k + 1], m->err);
+            else
+                hipLaunchKernelGGL((k_mg_assemble<T, 27, 8>), pg, ab, 0, st, np, m->child[k + 1], m->cap[k + 1],
+                                   m->col[k], (const T*) m->val[k], (const T*) m->diag[k], m->cap[k], m->agg[k],
+                                   m->coord[k + 1], m->scene[k + 1], m->d_sdims[k + 1], m->col[k + 1],
+                                   (T*) m->val[k + 1], (T*) m->diag[k + 1], (T*) m->dinv[k + 1], m->err);
+        }
     }
     if (m->L != l) m->gen++;
     m->L = l;

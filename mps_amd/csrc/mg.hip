@@ -1346,11 +1346,10 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
             return e;
     }
     if ((e = hipMemcpyAsync(m->nd, m->n, sizeof(int) * (kMgMaxLevels + 1), hipMemcpyHostToDevice, st))) return e;
-    int bad = 0;
-    if ((e = mg_read_int(m, m->err, &bad, st))) return e;
-    if (bad) return hipErrorInvalidValue;  // a coarse row with more neighbours than the stencil allows
-    return hipGetLastError();
+    return hipGetLastError();  // m->err is read by the caller with its next read
 }
+
+const int* mg_error_flag(const MgWork* m) { return m->err; }
 
 hipError_t mg_setup(MgWork* m, const Sell& A, const Hot& h, int dim, int fp32, hipStream_t st) {
     if (fp32)

@@ -793,8 +793,12 @@ int mg_levels(const MgWork* m);
 void mg_set_params(MgWork* m, double omega, double cf, int sweeps);
 // Builds the hierarchy for the current system (every segment its own
 // grids); hipErrorInvalidValue when it cannot (the caller then solves
-// without it).
+// without it).  One check is left on the device: the int at mg_error_flag
+// is nonzero after the stream has run the setup if a coarse row had more
+// neighbours than the stencil allows, and the hierarchy must then not be
+// used.
 hipError_t mg_setup(MgWork* m, const Sell& A, const Hot& h, int dim, int fp32, hipStream_t st);
+const int* mg_error_flag(const MgWork* m);
 // count flags at stride ints from flag: the cycle is skipped once all are set
 struct MgDone {
     const int* flag;

@@ -32,6 +32,7 @@
 
 #include <math.h>
 #include <stdlib.h>
+#include <cstring>
 
 #include <algorithm>
 #include <vector>
@@ -1013,9 +1014,21 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
     hipLaunchKernelGGL(k_cg_init, gr, bl, 0, st, sg, A.row_slot, h.id, b_s, o.warm_start ? pprev_id : nullptr,
                        o.warm_start == 2 ? pprev2_id : nullptr, A.sdiag, cw->y, cw->bh, cw->part);
     // ||b||^2 into red[2], then the initial residual: one read for both
+    // (and for the multigrid setup's stencil check)
     hipLaunchKernelGGL((k_reduce_parts<1, 2>), dim3(nseg), dim3(kBlock), 0, st, sg, cw->part, cw->sc);
     launch_residual(cw, A, st);
-    if ((e = reduce_and_read(cw, 2, st))) return e;
+    hipLaunchKernelGGL(k_reduce_parts<2>, dim3(nseg), dim3(kBlock), 0, st, sg, cw->part, cw->sc);
+    {
+        int mg_bad = 0;
+        const MailSrc src[2] = {{cw->sc, (int) (sizeof(CgScalars) * (size_t) nseg)},
+                                {cw->mg_on ? (const void*) mg_error_flag(cw->mg) : (const void*) cw->sc,
+                                 (int) sizeof(int)}};
+        std::vector<CgScalars> buf((size_t) nseg + 1);
+        if ((e = mailbox_read(cw->mb, src, 2, buf.data(), st))) return e;
+        std::memcpy(cw->host, buf.data(), sizeof(CgScalars) * (size_t) nseg);
+        std::memcpy(&mg_bad, (const char*) buf.data() + sizeof(CgScalars) * (size_t) nseg, sizeof(int));
+        if (cw->mg_on && mg_bad) cw->mg_on = 0;  // a coarse row outside the stencil: Jacobi alone
+    }
     std::vector<double> bn2(nseg), thr(nseg), rho2(nseg);
     std::vector<int> segrows(nseg), conv(nseg), total_it(nseg, 0), refinements(nseg, 0), max_it(nseg);
     for (int s = 0; s < nseg; s++) {

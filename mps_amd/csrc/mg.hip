@@ -861,10 +861,23 @@ __global__ void __launch_bounds__(kMgTailThreads) k_mg_tail(const MgLevelDesc* _
                 if (K >= 0) {
                     const T bK = a.b[K];
                     T acc = a.diag[K] * (omega * bK * a.dinv[K]);
-                    for (int j = 0; j < W; j++) {
-                        const int c = a.col[j * n + K];
-                        if (c < 0) break;
-                        acc += a.val[j * n + K] * (omega * a.b[c] * a.dinv[c]);
+                    for (int j0 = 0; j0 < W; j0 += kMgBatch) {
+                        int c[kMgBatch];
+                        T vv[kMgBatch], xb[kMgBatch];
+#pragma unroll
+                        for (int q = 0; q < kMgBatch; q++) {
+                            c[q] = a.col[(j0 + q) * n + K];
+                            vv[q] = a.val[(j0 + q) * n + K];
+                        }
+                        if (W > kMgBatch && c[0] < 0) break;
+#pragma unroll
+                        for (int q = 0; q < kMgBatch; q++) {
+                            const int cc = c[q] >= 0 ? c[q] : K;
+                            xb[q] = omega * a.b[cc] * a.dinv[cc];
+                        }
+#pragma unroll
+                        for (int q = 0; q < kMgBatch; q++)
+                            if (c[q] >= 0) acc += vv[q] * xb[q];
                     }
                     v = bK - acc;
                 }
@@ -886,10 +899,20 @@ __global__ void __launch_bounds__(kMgTailThreads) k_mg_tail(const MgLevelDesc* _
             for (int K = tid; K < n; K += kMgTailThreads) {
                 const T xt = xa[K];
                 T acc = c.diag[K] * xt;
-                for (int j = 0; j < W; j++) {
-                    const int cc = c.col[j * n + K];
-                    if (cc < 0) break;
-                    acc += c.val[j * n + K] * xa[cc];
+                for (int j0 = 0; j0 < W; j0 += kMgBatch) {
+                    int cj[kMgBatch];
+                    T vv[kMgBatch], xv[kMgBatch];
+#pragma unroll
+                    for (int q = 0; q < kMgBatch; q++) {
+                        cj[q] = c.col[(j0 + q) * n + K];
+                        vv[q] = c.val[(j0 + q) * n + K];
+                    }
+                    if (W > kMgBatch && cj[0] < 0) break;
+#pragma unroll
+                    for (int q = 0; q < kMgBatch; q++) xv[q] = xa[cj[q] >= 0 ? cj[q] : K];
+#pragma unroll
+                    for (int q = 0; q < kMgBatch; q++)
+                        if (cj[q] >= 0) acc += vv[q] * xv[q];
                 }
                 xb[K] = xt + omega * c.dinv[K] * (c.b[K] - acc);
             }
@@ -911,10 +934,27 @@ __global__ void __launch_bounds__(kMgTailThreads) k_mg_tail(const MgLevelDesc* _
             const T bK = a.b[K], dK = a.dinv[K];
             const T xx = omega * bK * dK + cf * xc[a.agg[K]];
             T acc = a.diag[K] * xx;
-            for (int j = 0; j < W; j++) {
-                const int c = a.col[j * n + K];
-                if (c < 0) break;
-                acc += a.val[j * n + K] * (omega * a.b[c] * a.dinv[c] + cf * xc[a.agg[c]]);
+            for (int j0 = 0; j0 < W; j0 += kMgBatch) {
+                int c[kMgBatch], ag[kMgBatch];
+                T vv[kMgBatch], bb[kMgBatch], dd[kMgBatch], xv[kMgBatch];
+#pragma unroll
+                for (int q = 0; q < kMgBatch; q++) {
+                    c[q] = a.col[(j0 + q) * n + K];
+                    vv[q] = a.val[(j0 + q) * n + K];
+                }
+                if (W > kMgBatch && c[0] < 0) break;
+#pragma unroll
+                for (int q = 0; q < kMgBatch; q++) {
+                    const int cc = c[q] >= 0 ? c[q] : K;
+                    bb[q] = a.b[cc];
+                    dd[q] = a.dinv[cc];
+                    ag[q] = a.agg[cc];
+                }
+#pragma unroll
+                for (int q = 0; q < kMgBatch; q++) xv[q] = xc[ag[q]];
+#pragma unroll
+                for (int q = 0; q < kMgBatch; q++)
+                    if (c[q] >= 0) acc += vv[q] * (omega * bb[q] * dd[q] + cf * xv[q]);
             }
             a.x[K] = xx + omega * dK * (bK - acc);
         }

@@ -665,13 +665,16 @@ __global__ void __launch_bounds__(kBlock) k_mg_down(const int* __restrict__ npar
 // where read), then x = xx + omega D^-1 (b - A xx).  The row's own values
 // and entries are read in the round that reads the level size and the done
 // flags (in range for every K < cap; rows past cap read row 0 and store
-// nothing).
-template <typename T, int W>
+// nothing).  PRO (level 1): also the fine level's first step up for the
+// cell's rows cst1[K] .. cst1[K + 1] - 1: xxf = omega bf + cf x[K].
+template <typename T, int W, bool PRO>
 __global__ void __launch_bounds__(kBlock) k_mg_up(const int* __restrict__ nrow, const int* __restrict__ col,
                                                   const T* __restrict__ val, const T* __restrict__ diag,
                                                   const T* __restrict__ dinv, int cap, const int* __restrict__ agg,
                                                   const T* __restrict__ b, const T* __restrict__ xc,
-                                                  T* __restrict__ x, T omega, T cf, MgDone done) {
+                                                  T* __restrict__ x, T omega, T cf, MgDone done,
+                                                  const int* __restrict__ cst1, const T* __restrict__ bf,
+                                                  T* __restrict__ xxf) {
 #pragma clang fp contract(fast)
     const int K = blockIdx.x * blockDim.x + threadIdx.x;
     const int Kr = K < cap ? K : 0;
@@ -685,6 +688,11 @@ __global__ void __launch_bounds__(kBlock) k_mg_up(const int* __restrict__ nrow, 
     }
     const T bK = b[Kr], dK = dinv[Kr], gK = diag[Kr];
     const int aK = agg[Kr];
+    int f0 = 0, f1 = 0;
+    if (PRO) {
+        f0 = cst1[Kr];
+        f1 = cst1[Kr + 1];
+    }
     if (mg_done_wave(done) || K >= nr) return;
 #pragma unroll
     for (int j = 0; j < W; j++) {
@@ -701,7 +709,17 @@ __global__ void __launch_bounds__(kBlock) k_mg_up(const int* __restrict__ nrow, 
 #pragma unroll
     for (int j = 0; j < W; j++)
         if (c[j] >= 0) acc += v[j] * (omega * bb[j] * dd[j] + cf * xv[j]);
-    x[K] = xx + omega * dK * (bK - acc);
+    const T xK = xx + omega * dK * (bK - acc);
+    x[K] = xK;
+    if (PRO)
+        for (int r = f0; r < f1; r += 4) {  // four fine rows per round
+            T bv[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) bv[q] = bf[r + q < f1 ? r + q : f1 - 1];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (r + q < f1) xxf[r + q] = omega * bv[q] + cf * xK;
+        }
 }
 
 // Device table of one level's arrays, for the tail kernel.
@@ -1417,12 +1435,18 @@ static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval,
                            (const T*) m->dinv[l], m->cap[l], (const T*) m->b[l], (T*) m->b[l + 1], om, done);
     hipLaunchKernelGGL((k_mg_tail<T, W, C>), dim3(1), dim3(kMgTailThreads), kMgTailLds, st, m->desc, nd, lt, L, om,
                        cf, m->sweeps, done);
-    for (int l = lt - 1; l >= 1; l--)
-        hipLaunchKernelGGL((k_mg_up<T, W>), mg_grid(m->cap[l]), dim3(kBlock), 0, st, nd + l, m->col[l],
+    for (int l = lt - 1; l >= 2; l--)
+        hipLaunchKernelGGL((k_mg_up<T, W, false>), mg_grid(m->cap[l]), dim3(kBlock), 0, st, nd + l, m->col[l],
                            (const T*) m->val[l], (const T*) m->diag[l], (const T*) m->dinv[l], m->cap[l], m->agg[l],
-                           (const T*) m->b[l], (const T*) m->x[l + 1], (T*) m->x[l], om, cf, done);
-    hipLaunchKernelGGL((k_mg_prolong0<T>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, r, m->agg[0],
-                       (const T*) m->x[1], xx0, om, cf, done);
+                           (const T*) m->b[l], (const T*) m->x[l + 1], (T*) m->x[l], om, cf, done, nullptr, nullptr,
+                           nullptr);
+    if (lt >= 2)  // level 1's up pass also takes the fine level's first step up
+        hipLaunchKernelGGL((k_mg_up<T, W, true>), mg_grid(m->cap[1]), dim3(kBlock), 0, st, nd + 1, m->col[1],
+                           (const T*) m->val[1], (const T*) m->diag[1], (const T*) m->dinv[1], m->cap[1], m->agg[1],
+                           (const T*) m->b[1], (const T*) m->x[2], (T*) m->x[1], om, cf, done, m->cst1, r, xx0);
+    else
+        hipLaunchKernelGGL((k_mg_prolong0<T>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, r, m->agg[0],
+                           (const T*) m->x[1], xx0, om, cf, done);
     hipLaunchKernelGGL((k_mg_up0<T, IDX>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, A.slice_off, fcol, fval, r,
                        xx0, u, om, done);
 }

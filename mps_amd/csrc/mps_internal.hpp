@@ -185,6 +185,25 @@ __device__ __forceinline__ void scan_cell(const double* __restrict__ px, const d
     }
 }
 
+// Distances from a particle at p to the two neighbouring cells' boxes along
+// one axis (cell c spans [pmin + (c - 1) len, pmin + c len) on the axis; the
+// particle's cell is ic), less a margin far above the rounding of the cell
+// index and of the distance.  Every slot in a cell lies in the cell's box
+// when the walk reads the positions its bucket build keyed (which every walk
+// here does), so a cell whose box is farther than re from the particle holds
+// no neighbour and is skipped whole; the visit order of the other cells and
+// what f receives are unchanged.
+__device__ __forceinline__ void cell_gaps(double p, double pmin, double len, int ic, double& lo, double& hi) {
+    const double m = 1e-9 * len;
+    const double l = p - (pmin + (double) (ic - 1) * len) - m;
+    const double h = (pmin + (double) ic * len) - p - m;
+    lo = l > 0.0 ? l : 0.0;
+    hi = h > 0.0 ? h : 0.0;
+}
+__device__ __forceinline__ double cell_gap(int d, double lo, double hi) {  // d = offset + 1 in 0..2
+    return d == 0 ? lo : (d == 2 ? hi : 0.0);
+}
+
 // Visits the neighbours of slot s in the reference's order (src/mps.cpp:599-613):
 // jx outer, jy, jz inner, ascending id within a cell.  Calls f(q, dist, dx, dy, dz)
 // for every slot q != s with dist < re, dist = sqrt((dx*dx+dy*dy)+dz*dz) and
@@ -206,13 +225,18 @@ __device__ __forceinline__ void for_neighbors(const GridView& g, const double* _
         // correctly rounded and monotone, so for them sqrt(d2) >= re and the
         // reference's test dist < re fails too.
         const double re2 = re * re * (1.0 + 1e-12);
+        double gxl, gxh, gyl, gyh;
+        cell_gaps(xi, g.xmin, g.len, ix, gxl, gxh);
+        cell_gaps(yi, g.ymin, g.len, iy, gyl, gyh);
+        const double rr = re * re;
         int cb[9], ce[9];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
 #pragma unroll
             for (int b = 0; b < 3; b++) {
                 const int jx = ix - 1 + a, jy = iy - 1 + b;
-                const bool ok = jx >= 0 && jx < g.nx && jy >= 0 && jy < g.ny;
+                const double gx = cell_gap(a, gxl, gxh), gy = cell_gap(b, gyl, gyh);
+                const bool ok = jx >= 0 && jx < g.nx && jy >= 0 && jy < g.ny && gx * gx + gy * gy <= rr;
                 const int k = jx + jy * g.nx;
                 cb[3 * a + b] = ok ? cs[k] : 0;
                 ce[3 * a + b] = ok ? cs[k + 1] : 0;
@@ -232,16 +256,25 @@ __device__ __forceinline__ void for_neighbors(const GridView& g, const double* _
             iy = (key / g.nx) % g.ny;
             iz = key / (g.nx * g.ny);
         }
+        double gxl, gxh, gyl, gyh, gzl, gzh;
+        cell_gaps(xi, g.xmin, g.len, ix, gxl, gxh);
+        cell_gaps(yi, g.ymin, g.len, iy, gyl, gyh);
+        cell_gaps(zi, g.zmin, g.len, iz, gzl, gzh);
+        const double rr = re * re;
         for (int jx = ix - 1; jx <= ix + 1; jx++) {
             if (jx < 0 || jx >= g.nx) continue;
+            const double gx = cell_gap(jx - ix + 1, gxl, gxh);
             for (int jy = iy - 1; jy <= iy + 1; jy++) {
                 if (jy < 0 || jy >= g.ny) continue;
+                const double gy = cell_gap(jy - iy + 1, gyl, gyh);
                 for (int jz = iz - 1; jz <= iz + 1; jz++) {
                     if (g.flat) {
                         if (jz != 1) continue;  // layers 0 and 2 are empty
                     } else if (jz < 0 || jz >= g.nz) {
                         continue;
                     }
+                    const double gz = g.flat ? 0.0 : cell_gap(jz - iz + 1, gzl, gzh);
+                    if (gx * gx + gy * gy + gz * gz > rr) continue;
                     const int k = g.flat ? (jx + jy * g.nx) : (jx + jy * g.nx + jz * g.nx * g.ny);
                     scan_cell<3>(px, py, pz, cs[k], cs[k + 1], s, xi, yi, zi, re, re2, f);
                 }
@@ -392,13 +425,18 @@ __device__ __forceinline__ void for_neighbors(const NbStage<DIM>& st, const Grid
     if constexpr (DIM == 2) {
         const int ix = key % g.nx;
         const int iy = key / g.nx;
+        double gxl, gxh, gyl, gyh;  // cells out of reach are skipped (see cell_gaps)
+        cell_gaps(xi, g.xmin, g.len, ix, gxl, gxh);
+        cell_gaps(yi, g.ymin, g.len, iy, gyl, gyh);
+        const double rr = re * re;
         int cb[9], ce[9];
 #pragma unroll
         for (int a = 0; a < 3; a++) {
 #pragma unroll
             for (int b = 0; b < 3; b++) {
                 const int jx = ix - 1 + a, jy = iy - 1 + b;
-                const bool ok = jx >= 0 && jx < g.nx && jy >= 0 && jy < g.ny;
+                const double gx = cell_gap(a, gxl, gxh), gy = cell_gap(b, gyl, gyh);
+                const bool ok = jx >= 0 && jx < g.nx && jy >= 0 && jy < g.ny && gx * gx + gy * gy <= rr;
                 const int k = jx + jy * g.nx;
                 cb[3 * a + b] = ok ? cs[k] : 0;
                 ce[3 * a + b] = ok ? cs[k + 1] : 0;
@@ -415,12 +453,21 @@ __device__ __forceinline__ void for_neighbors(const NbStage<DIM>& st, const Grid
         const int ix = key % g.nx;
         const int iy = (key / g.nx) % g.ny;
         const int iz = key / (g.nx * g.ny);
+        double gxl, gxh, gyl, gyh, gzl, gzh;  // cells out of reach are skipped (see cell_gaps)
+        cell_gaps(xi, g.xmin, g.len, ix, gxl, gxh);
+        cell_gaps(yi, g.ymin, g.len, iy, gyl, gyh);
+        cell_gaps(zi, g.zmin, g.len, iz, gzl, gzh);
+        const double rr = re * re;
         for (int jx = ix - 1; jx <= ix + 1; jx++) {
             if (jx < 0 || jx >= g.nx) continue;
+            const double gx = cell_gap(jx - ix + 1, gxl, gxh);
             for (int jy = iy - 1; jy <= iy + 1; jy++) {
                 if (jy < 0 || jy >= g.ny) continue;
+                const double gy = cell_gap(jy - iy + 1, gyl, gyh);
                 for (int jz = iz - 1; jz <= iz + 1; jz++) {
                     if (jz < 0 || jz >= g.nz) continue;
+                    const double gz = cell_gap(jz - iz + 1, gzl, gzh);
+                    if (gx * gx + gy * gy + gz * gz > rr) continue;
                     const int k = jx + jy * g.nx + jz * g.nx * g.ny;
                     const int r = (jy - iy + 1) + 3 * (jz - iz + 1);
                     scan_cell_lds<3>(st.x, st.y, st.z, st.meta[1 + r], cs[k], cs[k + 1], s, xi, yi, zi, re, re2, f);
@@ -528,12 +575,21 @@ __device__ __forceinline__ void for_neighbors(const NbStageF& st, const GridView
     const int ix = key % g.nx;
     const int iy = (key / g.nx) % g.ny;
     const int iz = key / (g.nx * g.ny);
+    double gxl, gxh, gyl, gyh, gzl, gzh;  // cells out of reach are skipped (see cell_gaps)
+    cell_gaps(xi, g.xmin, g.len, ix, gxl, gxh);
+    cell_gaps(yi, g.ymin, g.len, iy, gyl, gyh);
+    cell_gaps(zi, g.zmin, g.len, iz, gzl, gzh);
+    const double rr = re * re;
     for (int jx = ix - 1; jx <= ix + 1; jx++) {
         if (jx < 0 || jx >= g.nx) continue;
+        const double gx = cell_gap(jx - ix + 1, gxl, gxh);
         for (int jy = iy - 1; jy <= iy + 1; jy++) {
             if (jy < 0 || jy >= g.ny) continue;
+            const double gy = cell_gap(jy - iy + 1, gyl, gyh);
             for (int jz = iz - 1; jz <= iz + 1; jz++) {
                 if (jz < 0 || jz >= g.nz) continue;
+                const double gz = cell_gap(jz - iz + 1, gzl, gzh);
+                if (gx * gx + gy * gy + gz * gz > rr) continue;
                 const int k = jx + jy * g.nx + jz * g.nx * g.ny;
                 const int off = st.meta[1 + (jy - iy + 1) + 3 * (jz - iz + 1)];
                 const int b = cs[k], e = cs[k + 1];

@@ -294,53 +294,6 @@ __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble1(int n1, const int* _
     }
 }
 
-// Parent (on the next level's dense index space) of level-l row K.
-__device__ __forceinline__ int mg_parent_key(int K, const int* __restrict__ coord, const int* __restrict__ scene,
-                                             const MgDims* __restrict__ d, const MgDims* __restrict__ dp,
-                                             const int* __restrict__ goffp, int& slot) {
-    const int sc = scene[K];
-    int cx, cy, cz;
-    mg_decode(coord[K], d[sc], cx, cy, cz);
-    slot = (cx & 1) + 2 * (cy & 1) + 4 * (cz & 1);
-    const MgDims q = dp[sc];
-    return goffp[sc] + (cx >> 1) + q.x * ((cy >> 1) + q.y * (cz >> 1));
-}
-
-__global__ void __launch_bounds__(kBlock) k_mg_mark(int n, const int* __restrict__ coord,
-                                                    const int* __restrict__ scene, const MgDims* __restrict__ d,
-                                                    const MgDims* __restrict__ dp, const int* __restrict__ goffp,
-                                                    int* __restrict__ flag) {
-    const int K = blockIdx.x * blockDim.x + threadIdx.x;
-    if (K >= n) return;
-    int slot;
-    flag[mg_parent_key(K, coord, scene, d, dp, goffp, slot)] = 1;
-}
-
-__global__ void __launch_bounds__(kBlock) k_mg_parents(int G, const int* __restrict__ flag,
-                                                       const int* __restrict__ pscan, const int* __restrict__ goffp,
-                                                       int nscene, int* __restrict__ coordp,
-                                                       int* __restrict__ scenep) {
-    const int P = blockIdx.x * blockDim.x + threadIdx.x;
-    if (P >= G || !flag[P]) return;
-    const int sc = mg_bucket(goffp, nscene, P);
-    coordp[pscan[P]] = P - goffp[sc];
-    scenep[pscan[P]] = sc;
-}
-
-// agg[K] = parent row; the parent's child slot (cx&1) + 2(cy&1) + 4(cz&1) = K.
-__global__ void __launch_bounds__(kBlock) k_mg_link(int n, const int* __restrict__ coord,
-                                                    const int* __restrict__ scene, const MgDims* __restrict__ d,
-                                                    const MgDims* __restrict__ dp, const int* __restrict__ goffp,
-                                                    const int* __restrict__ pscan, int* __restrict__ agg,
-                                                    int* __restrict__ child, int capp) {
-    const int K = blockIdx.x * blockDim.x + threadIdx.x;
-    if (K >= n) return;
-    int slot;
-    const int pr = pscan[mg_parent_key(K, coord, scene, d, dp, goffp, slot)];
-    agg[K] = pr;
-    child[(size_t) slot * capp + pr] = K;
-}
-
 // Tables for building the structure of all coarse levels in one pass each
 // (levels lo..hi; per level its scenes' grids and first dense indices, the
 // first cell of its grid in the joint flag array of levels 2.., the joint

@@ -411,7 +411,7 @@ static void free_all(mps_handle* h) {
                     h->w.vmax,      h->w.coll_cnt,    h->w.coll_off, h->w.coll_pairs, h->w.coll_dist, h->w.coll_level,
                     h->w.coll_last, h->x_s,           h->dbg_id,     h->tmp3a,        h->tmp3b,       h->tmp8,
                     h->A.slice_off, h->A.slice_w,     h->A.row_slot, h->A.sdiag,      h->A.dgl,       h->A.raw_off,
-                    h->A.col,       h->A.dcol,        h->A.maxdelta, h->A.val64,       h->A.val32,    h->A.raw_col,    h->A.raw_val,   h->dev_counters,
+                    h->A.col,       h->A.dcol,        h->A.anc,         h->A.maxdelta, h->A.val64,       h->A.val32,    h->A.raw_col,    h->A.raw_val,   h->dev_counters,
                     h->ev.ghost_ids, h->ev.unconn_ids, h->snap_pos,  h->snap_vel,     h->d_g,         h->d_c,
                     h->d_sid,       h->d_key0,        h->w.nb.cnt,   h->w.nb.q,       h->w.nb.d};
     for (void* p : ptrs) hipFree(p);

@@ -82,13 +82,6 @@ __device__ __forceinline__ int mg_bucket(const int* __restrict__ first, int coun
     return lo;
 }
 
-template <typename IDX>
-__device__ __forceinline__ int mg_col(const IDX* __restrict__ c, size_t idx, int row) {
-    if constexpr (sizeof(IDX) == 2)
-        return row + (int) c[idx];
-    else
-        return (int) c[idx];
-}
 
 // ---------------- setup ----------------
 
@@ -180,7 +173,8 @@ constexpr int kMgBatch = 9;     // ELL entries per round of loads in the coarse 
 template <typename T, typename IDX, int W, int A>
 __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble1(int n1, const int* __restrict__ cst1,
                                                             const int* __restrict__ slice_pos,
-                                                            const IDX* __restrict__ col, const T* __restrict__ val,
+                                                            const IDX* __restrict__ col, const int* __restrict__ anc,
+                                                            const T* __restrict__ val,
                                                             const int* __restrict__ agg0,
                                                             const int* __restrict__ coord1,
                                                             const int* __restrict__ scene1,
@@ -224,6 +218,8 @@ __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble1(int n1, const int* _
             const int sl = r >> 6, lane = r & 63;
             const int sp = slice_pos[sl], w = slice_pos[sl + 1] - sp;
             const size_t base = (size_t) sp * 64 + lane;
+            int alo, ahi;
+            sell_anchors<IDX>(anc, r, alo, ahi);
             // four entries per round: columns and values, then their
             // aggregates, then the aggregates' cells, each round's loads
             // issued together (entries past w re-read the last and are dropped)
@@ -233,7 +229,7 @@ __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble1(int n1, const int* _
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const size_t idx = base + (size_t) (k0 + q < w ? k0 + q : w - 1) * 64;
-                    K2[q] = mg_col(col, idx, r);
+                    K2[q] = sell_col(col, idx, r, alo, ahi);
                     v[q] = (double) val[idx];
                 }
 #pragma unroll
@@ -499,16 +495,19 @@ __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble(int np, const int* __
 // before any is used.
 template <typename T, typename IDX>
 __device__ __forceinline__ T mg_fine_row(const int* __restrict__ slice_pos, const IDX* __restrict__ col,
-                                         const T* __restrict__ val, const T* __restrict__ x, int r, T acc) {
+                                         const int* __restrict__ anc, const T* __restrict__ val,
+                                         const T* __restrict__ x, int r, T acc) {
 #pragma clang fp contract(fast)
     const int sl = r >> 6, lane = r & 63;
     const int s0 = slice_pos[sl], w = slice_pos[sl + 1] - s0;
     const size_t base = (size_t) s0 * 64 + lane;
+    int lo, hi;
+    sell_anchors<IDX>(anc, r, lo, hi);
     int k = 0;
     for (; k + 4 <= w; k += 4) {
         const size_t i0 = base + (size_t) k * 64;
-        const int c0 = mg_col(col, i0, r), c1 = mg_col(col, i0 + 64, r);
-        const int c2 = mg_col(col, i0 + 128, r), c3 = mg_col(col, i0 + 192, r);
+        const int c0 = sell_col(col, i0, r, lo, hi), c1 = sell_col(col, i0 + 64, r, lo, hi);
+        const int c2 = sell_col(col, i0 + 128, r, lo, hi), c3 = sell_col(col, i0 + 192, r, lo, hi);
         const T v0 = val[i0], v1 = val[i0 + 64], v2 = val[i0 + 128], v3 = val[i0 + 192];
         const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
         acc += v0 * x0;
@@ -518,7 +517,7 @@ __device__ __forceinline__ T mg_fine_row(const int* __restrict__ slice_pos, cons
     }
     for (; k < w; k++) {
         const size_t idx = base + (size_t) k * 64;
-        acc += val[idx] * x[mg_col(col, idx, r)];
+        acc += val[idx] * x[sell_col(col, idx, r, lo, hi)];
     }
     return acc;
 }
@@ -526,7 +525,8 @@ __device__ __forceinline__ T mg_fine_row(const int* __restrict__ slice_pos, cons
 // Fine level, down: x0 = omega b (unit diagonal); res = b - Ahat x0.
 template <typename T, typename IDX>
 __global__ void __launch_bounds__(kBlock) k_mg_down0(const int* __restrict__ nrow, const int* __restrict__ slice_pos,
-                                                     const IDX* __restrict__ col, const T* __restrict__ val,
+                                                     const IDX* __restrict__ col, const int* __restrict__ anc,
+                                                     const T* __restrict__ val,
                                                      const T* __restrict__ b, T* __restrict__ res, T omega,
                                                      MgDone done) {
 #pragma clang fp contract(fast)
@@ -534,7 +534,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_down0(const int* __restrict__ nro
     const int nr = *nrow;
     if (mg_done_wave(done) || r >= nr) return;
     const T br = b[r];
-    const T acc = mg_fine_row<T, IDX>(slice_pos, col, val, b, r, br);
+    const T acc = mg_fine_row<T, IDX>(slice_pos, col, anc, val, b, r, br);
     res[r] = br - omega * acc;
 }
 
@@ -951,7 +951,8 @@ __global__ void __launch_bounds__(kBlock) k_mg_prolong0(const int* __restrict__ 
 // ... then u = xx + omega (b - Ahat xx).
 template <typename T, typename IDX>
 __global__ void __launch_bounds__(kBlock) k_mg_up0(const int* __restrict__ nrow, const int* __restrict__ slice_pos,
-                                                   const IDX* __restrict__ col, const T* __restrict__ val,
+                                                   const IDX* __restrict__ col, const int* __restrict__ anc,
+                                                   const T* __restrict__ val,
                                                    const T* __restrict__ b, const T* __restrict__ xx,
                                                    T* __restrict__ u, T omega, MgDone done) {
 #pragma clang fp contract(fast)
@@ -959,7 +960,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_up0(const int* __restrict__ nrow,
     const int nr = *nrow;
     if (mg_done_wave(done) || r >= nr) return;
     const T xr = xx[r], br = b[r];
-    const T acc = mg_fine_row<T, IDX>(slice_pos, col, val, xx, r, xr);
+    const T acc = mg_fine_row<T, IDX>(slice_pos, col, anc, val, xx, r, xr);
     u[r] = xr + omega * (br - acc);
 }
 
@@ -1218,12 +1219,13 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
     const dim3 ab(kAsm1Rows);
     if (dim == 2)
         hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 9, 48>), dim3((n1 + 47) / 48), dim3(kAsm1Rows), 0, st, n1, m->cst1,
-                           A.slice_off, fcol, fval, m->agg[0], m->coord[1], m->scene[1], m->d_sdims[1], m->cap[1],
-                           m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1], m->err);
+                           A.slice_off, fcol, A.anc, fval, m->agg[0], m->coord[1], m->scene[1], m->d_sdims[1],
+                           m->cap[1], m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1], m->err);
     else
         hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 27, 16>), dim3((n1 + 15) / 16), dim3(kAsm1Rows), 0, st, n1,
-                           m->cst1, A.slice_off, fcol, fval, m->agg[0], m->coord[1], m->scene[1], m->d_sdims[1], m->cap[1],
-                           m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1], m->err);
+                           m->cst1, A.slice_off, fcol, A.anc, fval, m->agg[0], m->coord[1], m->scene[1],
+                           m->d_sdims[1], m->cap[1], m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1],
+                           m->err);
     // Coarse levels 2..: level l + 1 merges 2x2 (2x2x2) cells of level l.
     // Their structure is built for all levels at once: the grid dims of
     // every level that still shrinks, one flag per grid cell set by the
@@ -1363,9 +1365,13 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
 const int* mg_error_flag(const MgWork* m) { return m->err; }
 
 hipError_t mg_setup(MgWork* m, const Sell& A, const Hot& h, int dim, int fp32, hipStream_t st) {
-    if (fp32)
+    const unsigned short* acol = (const unsigned short*) A.dcol;  // 3D codes (see sell_col)
+    if (fp32) {
+        if (A.use16 && A.dim == 3) return mg_setup_t<float, unsigned short>(m, A, h, dim, A.val32, acol, st);
         return A.use16 ? mg_setup_t<float, short>(m, A, h, dim, A.val32, A.dcol, st)
                        : mg_setup_t<float, int>(m, A, h, dim, A.val32, A.col, st);
+    }
+    if (A.use16 && A.dim == 3) return mg_setup_t<double, unsigned short>(m, A, h, dim, A.val64, acol, st);
     return A.use16 ? mg_setup_t<double, short>(m, A, h, dim, A.val64, A.dcol, st)
                    : mg_setup_t<double, int>(m, A, h, dim, A.val64, A.col, st);
 }
@@ -1378,8 +1384,8 @@ static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval,
     int* nd = m->nd;
     T* res0 = (T*) m->res0;
     T* xx0 = (T*) m->xx0;
-    hipLaunchKernelGGL((k_mg_down0<T, IDX>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, A.slice_off, fcol, fval,
-                       r, res0, om, done);
+    hipLaunchKernelGGL((k_mg_down0<T, IDX>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, A.slice_off, fcol, A.anc,
+                       fval, r, res0, om, done);
     hipLaunchKernelGGL((k_mg_restrict1<T>), mg_grid(m->cap[1]), dim3(kBlock), 0, st, nd + 1, m->cst1, res0,
                        (T*) m->b[1], done);
     for (int l = 1; l < lt; l++)
@@ -1400,8 +1406,8 @@ static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval,
     else
         hipLaunchKernelGGL((k_mg_prolong0<T>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, r, m->agg[0],
                            (const T*) m->x[1], xx0, om, cf, done);
-    hipLaunchKernelGGL((k_mg_up0<T, IDX>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, A.slice_off, fcol, fval, r,
-                       xx0, u, om, done);
+    hipLaunchKernelGGL((k_mg_up0<T, IDX>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, A.slice_off, fcol, A.anc,
+                       fval, r, xx0, u, om, done);
 }
 
 template <typename T, typename IDX>
@@ -1421,5 +1427,9 @@ template void mg_cycle<double, short>(MgWork*, const Sell&, const short*, const 
                                       MgDone, int, hipStream_t);
 template void mg_cycle<double, int>(MgWork*, const Sell&, const int*, const double*, const double*, double*, MgDone,
                                     int, hipStream_t);
+template void mg_cycle<float, unsigned short>(MgWork*, const Sell&, const unsigned short*, const float*, const float*,
+                                              float*, MgDone, int, hipStream_t);
+template void mg_cycle<double, unsigned short>(MgWork*, const Sell&, const unsigned short*, const double*,
+                                               const double*, double*, MgDone, int, hipStream_t);
 
 }  // namespace mps

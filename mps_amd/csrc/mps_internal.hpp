@@ -819,8 +819,10 @@ struct Sell {
     int* slice_off;      // nslices + 1 (in entries)
     int* slice_w;        // nslices (max nnz of the slice)
     int* col;            // entries: column (compact row index)
-    short* dcol;         // entries: column - row, when use16
-    int use16;           // every |column - row| fits 16 bits
+    short* dcol;         // entries: 16-bit column codes, when use16 (see sell_col)
+    int use16;           // the 16-bit codes hold every column
+    int* anc;            // 3D, 2 per row: row offsets of the cells below and above the row's cell
+    size_t anc_cap;      // rows anc holds
     int* maxdelta;       // device: max |column - row| of the last assembly
     int wide_hint;       // host: the last assembly needed 32-bit columns (write them in the first pass)
     double* val64;       // scaled off-diagonal values (FP64)
@@ -835,6 +837,37 @@ struct Sell {
     double* raw_val;
     size_t raw_cap;
 };
+
+// Column of SELL entry idx of row `row`.  int: the column itself.  short
+// (2D): column - row.  unsigned short (3D): the row's neighbours lie in
+// three z layers of cells, each a compact run of rows, so the code holds
+// the layer (0 the row's own, 1 below, 2 above; top 2 bits) and the offset
+// + 8192 (14 bits) from the row itself in its own layer, else from the row's
+// anchor in that layer (lo, hi: the first row of the cell directly below
+// or above the row's cell, minus the row).
+template <typename IDX>
+__device__ __forceinline__ int sell_col(const IDX* __restrict__ c, size_t idx, int row, int lo, int hi) {
+    if constexpr (sizeof(IDX) == 4) {
+        return (int) c[idx];
+    } else if constexpr (static_cast<IDX>(-1) < 0) {
+        return row + (int) c[idx];
+    } else {
+        const unsigned v = c[idx];
+        const unsigned l = v >> 14;
+        return row + ((int) (v & 16383u) - 8192) + (l == 1u ? lo : (l == 2u ? hi : 0));
+    }
+}
+// The row's anchors (unsigned short codes only).
+template <typename IDX>
+__device__ __forceinline__ void sell_anchors(const int* __restrict__ anc, int row, int& lo, int& hi) {
+    if constexpr (sizeof(IDX) == 2 && !(static_cast<IDX>(-1) < 0)) {
+        lo = anc[2 * (size_t) row];
+        hi = anc[2 * (size_t) row + 1];
+    } else {
+        lo = hi = 0;
+    }
+}
+constexpr int kAnchMax = 8191;  // largest |offset| an unsigned short code holds
 
 hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,
                          const StepScratch& w, Sell& A, int want_val32, int want_raw, int* err,

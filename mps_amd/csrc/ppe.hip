@@ -109,9 +109,18 @@ __global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneT
                                                   double* __restrict__ val64, float* __restrict__ val32,
                                                   double* __restrict__ sdiag, double* __restrict__ dgl,
                                                   const int* __restrict__ raw_off, int* __restrict__ raw_col,
-                                                  double* __restrict__ raw_val, int write_col, NbList nb) {
+                                                  double* __restrict__ raw_val, int write_col, NbList nb,
+                                                  int* __restrict__ anc) {
+    // 3D: 16-bit codes relative to the row's anchors in the cell layers below
+    // and above (see sell_col); staged entries keep (column - row) * 4 + layer
+    constexpr bool kAnch = DIM == 3;
     __shared__ double s_val[kFillStage][64];
     __shared__ int s_dl[kFillStage][64];
+    int lo = 0, hi = 0;
+    auto code16 = [&](int dl, int layer) {
+        const int off = dl - (layer == 1 ? lo : (layer == 2 ? hi : 0));
+        return kAnch ? (short) ((layer << 14) | (off + 8192)) : (short) dl;
+    };
     const int lane = threadIdx.x;
     const int sl = blockIdx.x;
     const int r = sl * 64 + lane;
@@ -133,13 +142,29 @@ __global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneT
         int ro = raw_off ? raw_off[s] : 0;
         const double re = c.re_p;
         const double zi = DIM == 3 ? h.pz[s] : 0.0;
+        // the row's layer and anchors: first rows of the cells directly below
+        // and above its own (a flat grid has one layer)
+        const int nxy = g.nx * g.ny;
+        int iz = 0;
+        if (kAnch && !g.flat) {
+            const int* __restrict__ csl = g.cs + g.key0;
+            const int kl = h.key[s] - g.key0;
+            iz = kl / nxy;
+            if (iz > 0) lo = row_of[csl[kl - nxy]] - r;
+            if (iz < g.nz - 1) hi = row_of[csl[kl + nxy]] - r;
+        }
+        if (kAnch) {
+            anc[2 * (size_t) r] = lo;
+            anc[2 * (size_t) r + 1] = hi;
+        }
         struct Nbr {
             int bc;      // boundary class
             double isq;  // 1 / sqrt(diag_s[q])
             int row;     // compacted row
+            int key;     // cell key
         };
         for_listed_g<DIM>(nb, g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
-                          [&](int q) { return Nbr{(int) bc_s[q], isq_s[q], row_of[q]}; },
+                          [&](int q) { return Nbr{(int) bc_s[q], isq_s[q], row_of[q], kAnch ? h.key[q] : 0}; },
                           [&](int q, double dist, double, double, double, const Nbr& nq) {
                                if (nq.bc != MPS_BC_INNER) return;
                                // the off-diagonal value of src/mps.cpp:343-348
@@ -149,14 +174,20 @@ __global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneT
                                const double sv = v * (si * sj);
                                const int cq = nq.row;
                                const int dl = cq - r;
-                               const int ad = dl < 0 ? -dl : dl;
+                               int layer = 0;
+                               if (kAnch && !g.flat) {
+                                   const int zq = (nq.key - g.key0) / nxy;
+                                   layer = zq < iz ? 1 : (zq > iz ? 2 : 0);
+                               }
+                               const int off = dl - (layer == 1 ? lo : (layer == 2 ? hi : 0));
+                               const int ad = off < 0 ? -off : off;
                                md = ad > md ? ad : md;
                                if (k < kFillStage) {
                                    s_val[k][lane] = sv;
-                                   s_dl[k][lane] = dl;
+                                   s_dl[k][lane] = dl * 4 + layer;
                                } else {
                                    const size_t idx = base + (size_t) k * 64;
-                                   dcol[idx] = (short) dl;
+                                   dcol[idx] = code16(dl, layer);
                                    if (write_col) col[idx] = cq;
                                    val64[idx] = sv;
                                    if (val32) val32[idx] = (float) sv;
@@ -174,16 +205,17 @@ __global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneT
     for (int e = 0; e < ws; e++) {
         const size_t idx = base + (size_t) e * 64;
         const bool have = e < k;
-        const int dl = have ? s_dl[e][lane] : 0;
+        const int dq = have ? s_dl[e][lane] : 0;
+        const int dl = dq >> 2, layer = dq & 3;  // padding: own row, own layer
         const double sv = have ? s_val[e][lane] : 0.0;
-        dcol[idx] = (short) dl;
+        dcol[idx] = code16(dl, layer);
         if (write_col) col[idx] = r + dl;
         val64[idx] = sv;
         if (val32) val32[idx] = (float) sv;
     }
     for (int e = k > kFillStage ? k : kFillStage; e < width; e++) {
         const size_t idx = base + (size_t) e * 64;
-        dcol[idx] = 0;
+        dcol[idx] = code16(0, 0);
         if (write_col) col[idx] = r;
         val64[idx] = 0.0;
         if (val32) val32[idx] = 0.0f;
@@ -270,13 +302,22 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
     // them (the particle order changes little between steps); a second pass
     // walks the neighbours again only when that guess was wrong.
     const dim3 fg((rows + 63) / 64), fb(64);
+    const int lim16 = dim == 3 ? kAnchMax : 32767;
+    if (dim == 3 && (size_t) rows > A.anc_cap) {
+        hipFree(A.anc);
+        A.anc = nullptr;
+        A.anc_cap = 0;
+        const size_t cap = (size_t) rows + rows / 4 + 64;
+        if ((e = hipMalloc(&A.anc, 2 * cap * sizeof(int))) != hipSuccess) return e;
+        A.anc_cap = cap;
+    }
     int md = 0;
     for (int pass = 0; pass < 2; pass++) {
         const int write_col = pass || A.wide_hint;
 #define FILL_ARGS                                                                                                  \
     g, c, tab, h, rows_p, A.row_slot, w.row_of, w.bc_s, w.diag_s, w.isq_s, A.slice_off, A.col, A.dcol, A.maxdelta, A.val64, \
         want_val32 ? A.val32 : nullptr, A.sdiag, A.dgl, (want_raw && !pass) ? A.raw_off : nullptr,                \
-        (want_raw && !pass) ? A.raw_col : nullptr, (want_raw && !pass) ? A.raw_val : nullptr, write_col, w.nb
+        (want_raw && !pass) ? A.raw_col : nullptr, (want_raw && !pass) ? A.raw_val : nullptr, write_col, w.nb, A.anc
         if (dim == 2 && tab.g)
             hipLaunchKernelGGL((k_fill_sell<2, true>), fg, fb, 0, st, FILL_ARGS);
         else if (dim == 2)
@@ -286,12 +327,14 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
         else
             hipLaunchKernelGGL((k_fill_sell<3, false>), fg, fb, 0, st, FILL_ARGS);
 #undef FILL_ARGS
-        // 16-bit column offsets when every row's neighbours lie within +-32767 rows
+        // 16-bit column codes when every offset fits: within +-32767 rows of
+        // the row (2D), within +-8191 of the row's anchor in the entry's
+        // layer (3D)
         const MailSrc src{A.maxdelta, (int) sizeof(int)};
         if ((e = mailbox_read(mb, &src, 1, &md, st)) != hipSuccess) return e;
-        if (md <= 32767 || write_col) break;
+        if (md <= lim16 || write_col) break;
     }
-    A.use16 = md <= 32767 ? 1 : 0;
+    A.use16 = md <= lim16 ? 1 : 0;
     A.wide_hint = A.use16 ? 0 : 1;
     return hipGetLastError();
 }
@@ -433,28 +476,22 @@ __device__ __forceinline__ int block_uniform_done(CgScalars* sc) {
     return s_done;
 }
 
-// Column of SELL entry idx of row `row`: absolute (int) or 16-bit offset.
-template <typename IDX>
-__device__ __forceinline__ int col_of(const IDX* __restrict__ c, size_t idx, int row) {
-    if constexpr (sizeof(IDX) == 2)
-        return row + (int) c[idx];
-    else
-        return (int) c[idx];
-}
-
 // acc + sum_k val[k] x[col[k]] over the row's SELL entries, gathers issued in
 // batches of 4 for memory-level parallelism.
 template <typename T, typename IDX>
 __device__ __forceinline__ T sell_row(const int* __restrict__ slice_pos, const IDX* __restrict__ col,
-                                      const T* __restrict__ val, const T* __restrict__ x, int row, T acc) {
+                                      const int* __restrict__ anc, const T* __restrict__ val,
+                                      const T* __restrict__ x, int row, T acc) {
     const int sl = row >> 6, lane = row & 63;
     const int s0 = slice_pos[sl], w = slice_pos[sl + 1] - s0;
     const size_t base = (size_t) s0 * 64 + lane;
+    int lo, hi;
+    sell_anchors<IDX>(anc, row, lo, hi);
     int k = 0;
     for (; k + 4 <= w; k += 4) {
         const size_t i0 = base + (size_t) k * 64;
-        const int c0 = col_of(col, i0, row), c1 = col_of(col, i0 + 64, row);
-        const int c2 = col_of(col, i0 + 128, row), c3 = col_of(col, i0 + 192, row);
+        const int c0 = sell_col(col, i0, row, lo, hi), c1 = sell_col(col, i0 + 64, row, lo, hi);
+        const int c2 = sell_col(col, i0 + 128, row, lo, hi), c3 = sell_col(col, i0 + 192, row, lo, hi);
         const T v0 = val[i0], v1 = val[i0 + 64], v2 = val[i0 + 128], v3 = val[i0 + 192];
         const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
         acc += v0 * x0;
@@ -464,7 +501,7 @@ __device__ __forceinline__ T sell_row(const int* __restrict__ slice_pos, const I
     }
     for (; k < w; k++) {
         const size_t i0 = base + (size_t) k * 64;
-        acc += val[i0] * x[col_of(col, i0, row)];
+        acc += val[i0] * x[sell_col(col, i0, row, lo, hi)];
     }
     return acc;
 }
@@ -473,7 +510,8 @@ __device__ __forceinline__ T sell_row(const int* __restrict__ slice_pos, const I
 // diagonal); partials r.u, w.u and the stopping norm of r.
 template <typename T, typename IDX, bool WEIGHTED, bool PRE>
 __global__ void __launch_bounds__(kCgBlock) k_cg_apply(Seg sg, const int* __restrict__ slice_pos,
-                                                     const IDX* __restrict__ col, const T* __restrict__ val,
+                                                     const IDX* __restrict__ col, const int* __restrict__ anc,
+                                                     const T* __restrict__ val,
                                                      const T* __restrict__ r, const T* __restrict__ u,
                                                      T* __restrict__ w, const double* __restrict__ dgl,
                                                      double* __restrict__ part, CgScalars* scs) {
@@ -488,7 +526,7 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_apply(Seg sg, const int* __rest
     for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kCgBlock) {
         const T rr = r[row];
         const T uu = PRE ? x[row] : rr;
-        const T acc = sell_row<T, IDX>(slice_pos, col, val, x, row, uu);
+        const T acc = sell_row<T, IDX>(slice_pos, col, anc, val, x, row, uu);
         w[row] = acc;
         const double r2 = (double) rr * (double) rr;
         g += (double) rr * (double) uu;
@@ -621,7 +659,8 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_init(Seg sg, const int* __restr
 // FP64 residual r = bhat - Ahat y (unit diagonal); partials r.r, sum d r^2.
 template <typename IDX>
 __global__ void __launch_bounds__(kCgBlock) k_residual64(Seg sg, const int* __restrict__ slice_pos,
-                                                       const IDX* __restrict__ col, const double* __restrict__ val,
+                                                       const IDX* __restrict__ col, const int* __restrict__ anc,
+                                                       const double* __restrict__ val,
                                                        const double* __restrict__ y, const double* __restrict__ bh,
                                                        const double* __restrict__ dgl, double* __restrict__ r,
                                                        double* __restrict__ part) {
@@ -630,7 +669,7 @@ __global__ void __launch_bounds__(kCgBlock) k_residual64(Seg sg, const int* __re
     const BlockRows br = block_rows(sg);
     double rr = 0.0, rw = 0.0;
     for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kCgBlock) {
-        const double acc = sell_row<double, IDX>(slice_pos, col, val, y, row, y[row]);
+        const double acc = sell_row<double, IDX>(slice_pos, col, anc, val, y, row, y[row]);
         const double rv = bh[row] - acc;
         r[row] = rv;
         rr += rv * rv;
@@ -846,12 +885,14 @@ static void enqueue_chunk(CgWork* cw, const Sell& A, const IDX* col, const T* va
         if (cw->mg_on) {
             const MgDone dn{&cw->sc[0].done, (int) (sizeof(CgScalars) / sizeof(int)), cw->nseg};
             mg_cycle<T, IDX>(cw->mg, A, col, val, r, u, dn, A.dim, st);
-            hipLaunchKernelGGL((k_cg_apply<T, IDX, WEIGHTED, true>), gr, bl, 0, st, sg, A.slice_off, col, val, r, u, w,
+            hipLaunchKernelGGL((k_cg_apply<T, IDX, WEIGHTED, true>), gr, bl, 0, st, sg, A.slice_off, col, A.anc, val, r,
+                               u, w,
                                A.dgl, cw->part, cw->sc);
             hipLaunchKernelGGL((k_cg_step<T, WEIGHTED, true>), gr, bl, 0, st, sg, y, r, p, s, w, u, cw->part, cw->sc,
                                k & 1);
         } else {
-            hipLaunchKernelGGL((k_cg_apply<T, IDX, WEIGHTED, false>), gr, bl, 0, st, sg, A.slice_off, col, val, r, r,
+            hipLaunchKernelGGL((k_cg_apply<T, IDX, WEIGHTED, false>), gr, bl, 0, st, sg, A.slice_off, col, A.anc, val, r,
+                               r,
                                w, A.dgl, cw->part, cw->sc);
             hipLaunchKernelGGL((k_cg_step<T, WEIGHTED, false>), gr, bl, 0, st, sg, y, r, p, s, w, r, cw->part, cw->sc,
                                k & 1);
@@ -874,7 +915,7 @@ template <typename T, typename IDX, bool WEIGHTED>
 static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* col, const T* val, T* y, T* r, T* p,
                               T* s, T* w, const mps_solver_options& o, int pred, hipStream_t st) {
     const int interval = std::max(2, (o.check_interval + 1) & ~1);
-    const int idx16 = sizeof(IDX) == 2 ? 1 : 0;
+    const int idx16 = sizeof(IDX) == 4 ? 0 : (static_cast<IDX>(-1) < 0 ? 1 : 2);
     const size_t sbytes = sizeof(CgScalars) * (size_t) cw->nseg;
     hipError_t e;
     if (o.use_graph) {
@@ -952,12 +993,15 @@ static hipError_t reduce_and_read(CgWork* cw, int nv, hipStream_t st) {
 static void launch_residual(CgWork* cw, const Sell& A, hipStream_t st) {
     const dim3 gr(cw->total_blocks), bl(kCgBlock);
     const Seg sg = cw->seg();
-    if (A.use16)
-        hipLaunchKernelGGL(k_residual64<short>, gr, bl, 0, st, sg, A.slice_off, A.dcol, A.val64, cw->y, cw->bh, A.dgl,
-                           cw->r64, cw->part);
+    if (A.use16 && A.dim == 3)
+        hipLaunchKernelGGL(k_residual64<unsigned short>, gr, bl, 0, st, sg, A.slice_off, (const unsigned short*) A.dcol,
+                           A.anc, A.val64, cw->y, cw->bh, A.dgl, cw->r64, cw->part);
+    else if (A.use16)
+        hipLaunchKernelGGL(k_residual64<short>, gr, bl, 0, st, sg, A.slice_off, A.dcol, A.anc, A.val64, cw->y, cw->bh,
+                           A.dgl, cw->r64, cw->part);
     else
-        hipLaunchKernelGGL(k_residual64<int>, gr, bl, 0, st, sg, A.slice_off, A.col, A.val64, cw->y, cw->bh, A.dgl,
-                           cw->r64, cw->part);
+        hipLaunchKernelGGL(k_residual64<int>, gr, bl, 0, st, sg, A.slice_off, A.col, A.anc, A.val64, cw->y, cw->bh,
+                           A.dgl, cw->r64, cw->part);
 }
 
 // `solve`: index of this solve within the step (0 for the FP64 CG, the
@@ -970,7 +1014,10 @@ static hipError_t run_cg(CgWork* cw, int which, int solve, const Sell& A, const 
     int* const last = &cw->pred[cw->pred_bank][which][slot];
     const int pred = cw->predict ? *last : 0;
     hipError_t e;
-    if (A.use16)
+    if (A.use16 && A.dim == 3)
+        e = run_cg_loop<T, unsigned short, WEIGHTED>(cw, which, A, (const unsigned short*) A.dcol, val, y, r, p, s, w, o,
+                                                     pred, st);
+    else if (A.use16)
         e = run_cg_loop<T, short, WEIGHTED>(cw, which, A, A.dcol, val, y, r, p, s, w, o, pred, st);
     else
         e = run_cg_loop<T, int, WEIGHTED>(cw, which, A, A.col, val, y, r, p, s, w, o, pred, st);
@@ -1153,16 +1200,16 @@ static void time_launch(CgWork* cw, const Sell& A, const IDX* col, int kind, hip
     const Seg sg = cw->seg();
     switch (kind) {
         case 0:
-            hipLaunchKernelGGL((k_cg_apply<double, IDX, true, false>), gr, bl, 0, st, sg, A.slice_off, col, A.val64,
-                               cw->r64, cw->r64, cw->w64, A.dgl, cw->part, cw->sc);
+            hipLaunchKernelGGL((k_cg_apply<double, IDX, true, false>), gr, bl, 0, st, sg, A.slice_off, col, A.anc,
+                               A.val64, cw->r64, cw->r64, cw->w64, A.dgl, cw->part, cw->sc);
             break;
         case 1:
             hipLaunchKernelGGL((k_cg_step<double, true, false>), gr, bl, 0, st, sg, cw->y, cw->r64, cw->p64, cw->s64,
                                cw->w64, cw->r64, cw->part, cw->sc, 0);
             break;
         case 2:
-            hipLaunchKernelGGL((k_cg_apply<float, IDX, false, false>), gr, bl, 0, st, sg, A.slice_off, col, A.val32,
-                               cw->r32, cw->r32, cw->w32, A.dgl, cw->part, cw->sc);
+            hipLaunchKernelGGL((k_cg_apply<float, IDX, false, false>), gr, bl, 0, st, sg, A.slice_off, col, A.anc,
+                               A.val32, cw->r32, cw->r32, cw->w32, A.dgl, cw->part, cw->sc);
             break;
         default:
             hipLaunchKernelGGL((k_cg_step<float, false, false>), gr, bl, 0, st, sg, cw->e32, cw->r32, cw->p32, cw->s32,
@@ -1210,7 +1257,9 @@ hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, h
         // one untimed launch, then reps timed launches
         for (int rep = -1; rep < reps; rep++) {
             if (rep == 0) hipEventRecord(a, st);
-            if (A.use16)
+            if (A.use16 && A.dim == 3)
+                time_launch<unsigned short>(cw, A, (const unsigned short*) A.dcol, kind, st);
+            else if (A.use16)
                 time_launch<short>(cw, A, A.dcol, kind, st);
             else
                 time_launch<int>(cw, A, A.col, kind, st);
@@ -1226,11 +1275,12 @@ hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, h
     if (e) return e;
     const long long Rl = rows;
     const long long ib = A.use16 ? 2 : 4;
+    const long long ab = A.use16 && A.dim == 3 ? 8 : 0;  // the row's two anchors
     // apply: matrix entries + r read + w written (+ d when weighted)
-    bytes[0] = nnz * (ib + 8) + Rl * (8 + 8 + 8);
+    bytes[0] = nnz * (ib + 8) + Rl * (8 + 8 + 8 + ab);
     // step: r, w, p, s, y read; p, s, y, r written
     bytes[1] = Rl * 9 * 8;
-    bytes[2] = nnz * (ib + 4) + Rl * (4 + 4);
+    bytes[2] = nnz * (ib + 4) + Rl * (4 + 4 + ab);
     bytes[3] = Rl * 9 * 4;
     return hipGetLastError();
 }

@@ -482,7 +482,12 @@ __device__ __forceinline__ void for_neighbors(const NbStage<DIM>& st, const Grid
 // They only skip candidates that are certainly beyond re: slack covers the
 // rounding of the offsets (half an FP32 ulp of the largest offset, bounded by
 // the grid's diagonal) and of the FP32 distance; every candidate not skipped
-// is tested in FP64 from global memory exactly as before.
+// is tested in FP64 from global memory exactly as before.  The walk tests a
+// cell's candidates (64 at a time) in FP32 first and keeps the passes as a
+// bit mask, then loads the passes' FP64 positions kNbBatch at a time: the
+// FP64 test and f run in rounds of kNbBatch passes of the wavefront's
+// busiest lane instead of at every candidate that any lane passes, and
+// candidates still reach them in slot order.
 struct NbStageF {
     int ok;
     const int* meta;  // as NbStage<3>
@@ -593,17 +598,47 @@ __device__ __forceinline__ void for_neighbors(const NbStageF& st, const GridView
                 const int k = jx + jy * g.nx + jz * g.nx * g.ny;
                 const int off = st.meta[1 + (jy - iy + 1) + 3 * (jz - iz + 1)];
                 const int b = cs[k], e = cs[k + 1];
-                for (int q = b; q < e; q++) {
-                    const float dxf = st.x[q + off] - xf, dyf = st.y[q + off] - yf, dzf = st.z[q + off] - zf;
-                    if (dxf * dxf + dyf * dyf + dzf * dzf > rf2) continue;
-                    const unsigned boff = (unsigned) q * 8u;
-                    const double dx = *reinterpret_cast<const double*>(bx + boff) - xi;
-                    const double dy = *reinterpret_cast<const double*>(by + boff) - yi;
-                    const double dz = *reinterpret_cast<const double*>(bz + boff) - zi;
-                    const double d2 = (dx * dx + dy * dy) + dz * dz;
-                    if (d2 > re2) continue;
-                    const double dist = sqrt(d2);
-                    if (q != s && dist < re) f(q, dist, dx, dy, dz);
+                for (int c0 = b; c0 < e; c0 += 64) {
+                    const int c1 = e - c0 < 64 ? e : c0 + 64;
+                    uint64_t m = 0;
+                    for (int q = c0; q < c1; q++) {
+                        const float dxf = st.x[q + off] - xf, dyf = st.y[q + off] - yf, dzf = st.z[q + off] - zf;
+                        if (dxf * dxf + dyf * dyf + dzf * dzf <= rf2) m |= 1ull << (q - c0);
+                    }
+                    while (m) {
+                        int qq[kNbBatch];
+                        int np = 0;
+#pragma unroll
+                        for (int i = 0; i < kNbBatch; i++) {  // past the last pass: the first again
+                            if (i == 0 || m) {
+                                qq[i] = c0 + __builtin_ctzll(m);
+                                m &= m - 1;
+                                np = i + 1;
+                            } else {
+                                qq[i] = qq[0];
+                            }
+                        }
+                        double x[kNbBatch], y[kNbBatch], z[kNbBatch];
+#pragma unroll
+                        for (int i = 0; i < kNbBatch; i++) {
+                            const unsigned boff = (unsigned) qq[i] * 8u;
+                            x[i] = *reinterpret_cast<const double*>(bx + boff);
+                            y[i] = *reinterpret_cast<const double*>(by + boff);
+                            z[i] = *reinterpret_cast<const double*>(bz + boff);
+                        }
+#pragma unroll
+                        for (int i = 0; i < kNbBatch; i++) {
+                            if (i >= np) break;
+                            const int q = qq[i];
+                            const double dx = x[i] - xi;
+                            const double dy = y[i] - yi;
+                            const double dz = z[i] - zi;
+                            const double d2 = (dx * dx + dy * dy) + dz * dz;
+                            if (d2 > re2) continue;
+                            const double dist = sqrt(d2);
+                            if (q != s && dist < re) f(q, dist, dx, dy, dz);
+                        }
+                    }
                 }
             }
         }

@@ -57,8 +57,12 @@ __device__ __forceinline__ double ndens(const GridView& g, const Hot& h, int s, 
 // LDS staging of a block's candidate positions (see nb_stage, nb_stage_f):
 // capacity in slots per block; every thread of the block declares and fills
 // it first.  2D: FP64, 3 ranges of ~280 slots for 256-thread blocks (24 KB).
-// 3D: FP32 offsets, 9 ranges of ~320 slots (36 KB); in FP64 (72 KB) two
-// blocks fit a CU and the walks were slower than from global memory.
+// 3D: FP32 offsets, 9 ranges of ~330 slots: 3392 slots (40.7 KB with the
+// metadata) is the most that still lets four 256-thread blocks share a CU's
+// 160 KB; on the 3.6M-particle dam break at t = 0 one block in ten needs
+// more and walks global memory (at 3072 slots, four in ten did).  In FP64
+// (72 KB) two blocks fit a CU and the walks were slower than from global
+// memory.
 template <int DIM>
 struct NbStageOf {
     using type = NbStage<2>;
@@ -68,7 +72,7 @@ struct NbStageOf<3> {
     using type = NbStageF;
 };
 template <int DIM>
-constexpr int kStageCap = DIM == 2 ? 1536 : 3072;
+constexpr int kStageCap = DIM == 2 ? 1536 : 3392;
 template <int DIM>
 using StageT = typename std::conditional<DIM == 2, double, float>::type;
 

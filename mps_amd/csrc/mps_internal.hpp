@@ -378,29 +378,32 @@ __device__ __forceinline__ NbStage<DIM> nb_stage(const GridView& g0, const Scene
     return NbStage<DIM>{meta[0], meta, sx, sy, sz};
 }
 
-// scan_cell on the LDS copy: candidate slot q sits at index q + off.
+// scan_cell on the LDS copy (candidate slot q sits at index q + off), in two
+// passes: the candidates' prefilter results go into a 64-bit mask first (64
+// candidates at a time), then the exact test, the square root and f run once
+// per set bit, in slot order.  A wavefront then
+// runs them as often as its busiest lane has passes, not at every candidate
+// that any lane passes.
 template <int DIM, typename F>
-__device__ __forceinline__ void scan_cell_lds(const double* __restrict__ sx, const double* __restrict__ sy,
-                                              const double* __restrict__ sz, int off, int b, int e, int s,
-                                              double xi, double yi, double zi, double re, double re2, F& f) {
-    for (int q0 = b; q0 < e; q0 += kNbBatch) {
-        double x[kNbBatch], y[kNbBatch], z[kNbBatch];
-#pragma unroll
-        for (int i = 0; i < kNbBatch; i++) {
-            const int q = q0 + i < e ? q0 + i : e - 1;
-            x[i] = sx[q + off];
-            y[i] = sy[q + off];
-            if constexpr (DIM == 3) z[i] = sz[q + off];
-        }
-#pragma unroll
-        for (int i = 0; i < kNbBatch; i++) {
-            const int q = q0 + i;
-            if (q >= e) break;
-            const double dx = x[i] - xi;
-            const double dy = y[i] - yi;
-            const double dz = DIM == 3 ? z[i] - zi : 0.0;
+__device__ __forceinline__ void scan_cell_lds_mask(const double* __restrict__ sx, const double* __restrict__ sy,
+                                                   const double* __restrict__ sz, int off, int b, int e, int s,
+                                                   double xi, double yi, double zi, double re, double re2, F& f) {
+    for (int c0 = b; c0 < e; c0 += 64) {
+        const int c1 = e - c0 < 64 ? e : c0 + 64;
+        uint64_t m = 0;
+        for (int q = c0; q < c1; q++) {
+            const double dx = sx[q + off] - xi;
+            const double dy = sy[q + off] - yi;
+            const double dz = DIM == 3 ? sz[q + off] - zi : 0.0;
             const double d2f = DIM == 3 ? fma(dz, dz, fma(dy, dy, dx * dx)) : fma(dy, dy, dx * dx);
-            if (d2f > re2) continue;
+            if (d2f <= re2) m |= 1ull << (q - c0);
+        }
+        while (m) {
+            const int q = c0 + __builtin_ctzll(m);
+            m &= m - 1;
+            const double dx = sx[q + off] - xi;
+            const double dy = sy[q + off] - yi;
+            const double dz = DIM == 3 ? sz[q + off] - zi : 0.0;
             const double d2 = DIM == 3 ? (dx * dx + dy * dy) + dz * dz : dx * dx + dy * dy;
             if (d2 > re2) continue;
             const double dist = sqrt(d2);
@@ -446,8 +449,8 @@ __device__ __forceinline__ void for_neighbors(const NbStage<DIM>& st, const Grid
 #pragma unroll
         for (int c = 0; c < 9; c++) {
             const int b = c % 3;
-            scan_cell_lds<2>(st.x, st.y, st.z, b == 0 ? o0 : (b == 1 ? o1 : o2), cb[c], ce[c], s, xi, yi, 0.0, re,
-                             re2, f);
+            scan_cell_lds_mask<2>(st.x, st.y, st.z, b == 0 ? o0 : (b == 1 ? o1 : o2), cb[c], ce[c], s, xi, yi, 0.0,
+                                  re, re2, f);
         }
     } else {
         const int ix = key % g.nx;
@@ -470,7 +473,8 @@ __device__ __forceinline__ void for_neighbors(const NbStage<DIM>& st, const Grid
                     if (gx * gx + gy * gy + gz * gz > rr) continue;
                     const int k = jx + jy * g.nx + jz * g.nx * g.ny;
                     const int r = (jy - iy + 1) + 3 * (jz - iz + 1);
-                    scan_cell_lds<3>(st.x, st.y, st.z, st.meta[1 + r], cs[k], cs[k + 1], s, xi, yi, zi, re, re2, f);
+                    scan_cell_lds_mask<3>(st.x, st.y, st.z, st.meta[1 + r], cs[k], cs[k + 1], s, xi, yi, zi, re, re2,
+                                          f);
                 }
             }
         }

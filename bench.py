@@ -82,6 +82,25 @@ def cpu_baseline(settings, scene, n_fluid, threads):
                        f"{platform.processor() or platform.machine()}")
 
 
+def copy_probe(mib):
+    """A plain device copy of known size (one elementwise kernel reading and
+    writing mib MiB per launch), run under the same HBM counters as the
+    solver kernels to turn the counters' units into bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(4):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"copy_probe_mib": mib, "launches": 4,
+                      "read_write_gbs": 4 * 2 * n * 4 / dt / 1e9}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,7 +118,13 @@ def main():
                     help="CG start: 0 zero, 1 the last solution, 2 extrapolated from the last two")
     ap.add_argument("--inner-tol", type=float, default=1e-4,
                     help="mixed path: residual reduction asked of each FP32 inner solve")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB between two device buffers 4 times and exit: the known-size "
+                         "copy that scales the FETCH_SIZE / WRITE_SIZE counters to bytes")
     a = ap.parse_args()
+    if a.copy_probe > 0:
+        copy_probe(a.copy_probe)
+        return
     precond = capi.MPS_PRECOND_MULTIGRID if a.precond == "multigrid" else capi.MPS_PRECOND_JACOBI
     if a.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {a.gpus} is not supported (the MPS step here is a single-GPU "
@@ -156,9 +181,11 @@ def main():
     ms, by = mixed["kernels"]["spmv32"]
     achieved = by / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     # HBM bytes per launch from the committed, copy-calibrated FETCH_SIZE /
-    # WRITE_SIZE runs of the same kernel, scaled to this run's row count
+    # WRITE_SIZE runs of the same kernel on the same workload, scaled to this
+    # run's row count (the ensemble's 2D scenes use the 2M dam break's)
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "pmc_cg_apply_dambreak2d_2M.json")
+    pmc_of = "dambreak2d_2M" if a.workload == "ensemble64" else a.workload
+    pmc = os.path.join(ROOT, "profiles", f"pmc_cg_apply_{pmc_of}.json")
     if os.path.exists(pmc):
         with open(pmc) as f:
             t = json.load(f)

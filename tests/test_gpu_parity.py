@@ -229,6 +229,57 @@ def test_3d_one_step():
     assert np.linalg.norm(gp["x"] - op["x"]) <= P_RTOL * np.linalg.norm(op["x"])
 
 
+def _crowd(s, sc, centre, count, seed):
+    """count extra fluid particles scattered over the bucket cell (side
+    re_max) that holds centre, so that cell has more than 64 candidates."""
+    d = s.dim
+    l0 = s.particle_distance
+    side = max(s.ratio_pressure, s.ratio_viscosity, s.ratio_surface) * l0
+    lo = np.array([r[0] for r in (s.domain_x, s.domain_y, s.domain_z)][:d])
+    c0 = lo + np.floor((np.asarray(centre[:d]) - lo) / side) * side
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((count, 3))
+    pos[:, :d] = c0 + rng.uniform(0.05, 0.95, size=(count, d)) * side
+    return configs.Scene(np.concatenate([sc.type, np.full(count, configs.FLUID, dtype=sc.type.dtype)]),
+                         np.concatenate([sc.pos, pos]), np.concatenate([sc.vel, np.zeros((count, 3))]))
+
+
+def _crowded_case(dim):
+    if dim == 2:
+        s, sc, _ = configs.workload("dambreak2d_584")
+        return s, _crowd(s, _jitter(s, sc, 3), (0.12, 0.2, 0.0), 100, 4)
+    s = configs.dambreak_settings(0.02, 1e-3, 1.0, dim=3)
+    sc = configs.lattice_3d(0.02, (0.3, 0.3, 0.1), (0.0, 0.1), (0.0, 0.2), (0.0, 0.1), 1, 2)
+    s.domain_x, s.domain_y, s.domain_z = (-0.1, 0.4), (-0.1, 0.4), (-0.1, 0.2)
+    return s, _crowd(s, _jitter(s, sc, 5, amp=0.1), (0.05, 0.1, 0.05), 100, 6)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_crowded_cell_bit_exact(dim):
+    """One cell holding over 100 particles: the walks test candidates in
+    chunks of 64 and their lists overflow, and the state entering the
+    pressure stage (viscosity, move, collisions), the boundary conditions
+    and the pressure system must still equal the oracle's bit for bit."""
+    s, sc = _crowded_case(dim)
+    o, g = _pair(s, sc)
+    o.load_initial_state()
+    g.load_initial_state()
+    most = int(np.diff(g.neighbors()[0]).max())
+    assert most > 64, most
+    g.set_snapshot(True)
+    o.step(True)
+    g.step(True)
+    opos, ovel = o.snapshot()
+    gpos, gvel = g.snapshot()
+    assert (gpos == opos).all() and (gvel == ovel).all()
+    np.testing.assert_array_equal(g.state()["bc"], o.state()["bc"])
+    op, gp = o.pressure_system(), g.pressure_system()
+    assert (gp["b"] == op["b"]).all() and (gp["diag"] == op["diag"]).all()
+    np.testing.assert_array_equal(gp["row_ptr"], op["row_ptr"])
+    np.testing.assert_array_equal(gp["col"], op["col"])
+    assert (gp["val"] == op["val"]).all()
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_neighbour_list_capacity_changes_nothing(dim, monkeypatch):
     """The pressure stages read the neighbour lists the BC pass records;

@@ -234,6 +234,10 @@ static bool scene_grid(const mps_settings& s, int dim, GridView& g, std::string&
     g.key0 = 0;
     g.ghost_key = g.ncell;
     g.cs = nullptr;
+    // MPS_AMD_STAGE_CAP lowers the walks' LDS staging capacity (0: every walk
+    // reads global memory); results do not depend on it
+    g.stage_cap = 1 << 30;
+    if (const char* e = getenv("MPS_AMD_STAGE_CAP")) g.stage_cap = std::max(atoi(e), 0);
     return true;
 }
 

@@ -33,6 +33,7 @@ struct GridView {
     int key0;           // first key of this scene in the handle's key space
     int ghost_key;      // key of the ghost segment (after every scene's cells)
     const int* cs;      // cell starts over the whole key space (+ ghost segment end)
+    int stage_cap;      // most slots a block copies into LDS (the kernels' LDS size caps it further)
 };
 
 // Hot particle arrays in slot order (one of two ping-pong sets).
@@ -493,7 +494,7 @@ __device__ __forceinline__ void for_neighbors(const NbStage<DIM>& st, const Grid
 // busiest lane instead of at every candidate that any lane passes, and
 // candidates still reach them in slot order.
 struct NbStageF {
-    int ok;
+    int ok;           // bit r: stencil row r is in LDS
     const int* meta;  // as NbStage<3>
     const float* x;
     const float* y;
@@ -518,6 +519,9 @@ __device__ __forceinline__ NbStageF nb_stage_f(const GridView& g0, const SceneTa
             if (k_lo >= 0 && k_hi < g.ncell) {
                 const int* cs = g.cs + g.key0;
                 const int nx = g.nx, nxy = g.nx * g.ny;
+                // rows are staged in order while they fit; a row that does
+                // not is left in global memory (the walk reads its cells
+                // there) and later rows may still fit
                 int tot = 0;
                 for (int r = 0; r < NR; r++) {
                     const int dy = r % 3 - 1, dz = r / 3 - 1;
@@ -528,10 +532,12 @@ __device__ __forceinline__ NbStageF nb_stage_f(const GridView& g0, const SceneTa
                     first[r] = a;
                     base[r] = tot;
                     meta[1 + r] = tot - a;
-                    tot += b - a;
+                    if (tot + (b - a) <= cap) {
+                        tot += b - a;
+                        ok |= 1 << r;
+                    }
                 }
                 base[NR] = tot;
-                ok = tot <= cap ? 1 : 0;
                 // every offset is at most the grid's diagonal; an FP32 offset
                 // is off by half an ulp of that, a difference of two by one
                 // ulp per axis
@@ -600,8 +606,13 @@ __device__ __forceinline__ void for_neighbors(const NbStageF& st, const GridView
                 const double gz = cell_gap(jz - iz + 1, gzl, gzh);
                 if (gx * gx + gy * gy + gz * gz > rr) continue;
                 const int k = jx + jy * g.nx + jz * g.nx * g.ny;
-                const int off = st.meta[1 + (jy - iy + 1) + 3 * (jz - iz + 1)];
+                const int r = (jy - iy + 1) + 3 * (jz - iz + 1);
                 const int b = cs[k], e = cs[k + 1];
+                if (!((st.ok >> r) & 1)) {  // row left in global memory
+                    scan_cell<3>(px, py, pz, b, e, s, xi, yi, zi, re, re2, f);
+                    continue;
+                }
+                const int off = st.meta[1 + r];
                 for (int c0 = b; c0 < e; c0 += 64) {
                     const int c1 = e - c0 < 64 ? e : c0 + 64;
                     uint64_t m = 0;

@@ -81,10 +81,11 @@ __device__ __forceinline__ typename NbStageOf<DIM>::type nb_stage_any(const Grid
                                                                       const Hot& h, int n, int* meta,
                                                                       StageT<DIM>* sx, StageT<DIM>* sy,
                                                                       StageT<DIM>* sz) {
+    const int cap = kStageCap<DIM> < g.stage_cap ? kStageCap<DIM> : g.stage_cap;
     if constexpr (DIM == 2)
-        return nb_stage<2>(g, tab, h, n, kStageCap<2>, meta, sx, sy, sz);
+        return nb_stage<2>(g, tab, h, n, cap, meta, sx, sy, sz);
     else
-        return nb_stage_f(g, tab, h, n, kStageCap<3>, meta, sx, sy, sz);
+        return nb_stage_f(g, tab, h, n, cap, meta, sx, sy, sz);
 }
 
 #define NB_STAGE(DIM, GRID, HOT, N)                                                                        \

@@ -316,6 +316,43 @@ def test_neighbour_list_capacity_changes_nothing(dim, monkeypatch):
             assert (np.asarray(a) == np.asarray(b)).all()
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_stage_capacity_changes_nothing(dim, monkeypatch):
+    """The walks read a block's candidates from LDS when they fit the staging
+    capacity; a 3D block stages the stencil rows that fit and reads the others
+    from global memory.  The default capacity (every block staged here), 900
+    slots (3D: two to six of a block's nine rows staged, some after a row
+    left out) and 0 (every walk in global memory) must give the same bits
+    over 6 steps."""
+    if dim == 2:
+        s, sc, _ = configs.workload("dambreak2d_584")
+        sc = _jitter(s, sc, 23, vamp=0.3)
+    else:
+        s = configs.dambreak_settings(0.02, 1e-3, 1.0, dim=3)
+        sc = configs.lattice_3d(0.02, (0.3, 0.3, 0.1), (0.0, 0.1), (0.0, 0.2), (0.0, 0.1), 1, 2)
+        s.domain_x, s.domain_y, s.domain_z = (-0.1, 0.4), (-0.1, 0.4), (-0.1, 0.2)
+        sc = _jitter(s, sc, 24, amp=0.1, vamp=0.3)
+    res = []
+    for cap in (None, "900", "0"):
+        if cap is None:
+            monkeypatch.delenv("MPS_AMD_STAGE_CAP", raising=False)
+        else:
+            monkeypatch.setenv("MPS_AMD_STAGE_CAP", cap)
+        g = capi.Simulation(s, sc)
+        g.set_solver(precision=capi.MPS_PREC_FP64, tolerance=1e-12, warm_start=True)
+        g.load_initial_state()
+        for _ in range(6):
+            g.step()
+        res.append((g.state(), g.debug_events()))
+        g.close()
+    monkeypatch.delenv("MPS_AMD_STAGE_CAP", raising=False)
+    for st, ev in res[1:]:
+        for k in ("type", "pos", "vel", "p", "ndr", "bc", "pmin"):
+            assert (st[k] == res[0][0][k]).all(), k
+        for a, b in zip(ev, res[0][1]):
+            assert (np.asarray(a) == np.asarray(b)).all()
+
+
 def test_graph_and_stream_paths_identical():
     s, sc, _ = configs.workload("dambreak2d_584")
     res = []

@@ -220,12 +220,17 @@ __global__ void __launch_bounds__(64) k_fill_sell(GridView g0, Consts c0, SceneT
         val64[idx] = 0.0;
         if (val32) val32[idx] = 0.0f;
     }
-    // one atomic per wavefront (the block is one wavefront; every lane is here)
+    // at most one atomic per wavefront (the block is one wavefront; every
+    // lane is here).  Every wavefront aims at the same word, so the atomic is
+    // skipped when the word already holds md: it only grows during the
+    // launch, so a stale read is too low, never too high, and costs at most
+    // an atomic that was not needed.
     for (int o = 32; o > 0; o >>= 1) {
         const int u = __shfl_xor(md, o, 64);
         md = u > md ? u : md;
     }
-    if (lane == 0 && md > 0) atomicMax(maxdelta, md);
+    if (lane == 0 && md > 0 && __hip_atomic_load(maxdelta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < md)
+        atomicMax(maxdelta, md);
 }
 
 hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, const Hot& h,

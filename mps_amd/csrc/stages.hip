@@ -491,6 +491,9 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_diag_join(GridView g0, Consts
                                    }
                                }
                                if (q <= s) return;
+                               // parents only ever move to ancestors, so equal
+                               // parents (even read stale) share a component
+                               if (parent[q] == parent[s]) return;
                                int a = cc_find(vp, s);
                                int b = cc_find(vp, q);
                                while (a != b) {

@@ -30,8 +30,13 @@ extern "C" {
 #define MPS_E_HIP 2       /* HIP runtime error */
 #define MPS_E_NOMEM 3     /* device allocation failed */
 #define MPS_E_RANGE 4     /* a particle fell outside the bucket grid */
-#define MPS_E_SOLVER 5    /* PPE solve did not reach its tolerance */
-#define MPS_E_ASYM 6      /* PPE matrix not symmetric (src/mps.cpp:418-443) */
+#define MPS_E_SOLVER 5    /* the step completed, but a PPE solve stopped above its
+                             tolerance (the true FP64 relative residual); the
+                             reference ignores its solver's status
+                             (src/mps.cpp:457-460) and so may the caller */
+/* The reference's asymmetric-matrix exit (src/mps.cpp:418-443) has no status:
+ * the matrix is assembled symmetric by construction (one density per scene,
+ * the same distance both ways). */
 
 #define MPS_GHOST (-1)
 #define MPS_FLUID 0
@@ -102,7 +107,9 @@ typedef struct mps_solver_options {
 typedef struct mps_solver_stats {
     int32_t rows;              /* Inner rows in the system */
     int32_t iterations;        /* CG iterations (mixed: inner iterations summed) */
-    int32_t refinements;       /* mixed: outer refinement steps (FP64 residuals) */
+    int32_t refinements;       /* mixed: outer refinement steps (FP64 residuals);
+                                  FP64: restarts from the iterate after its true
+                                  residual missed the tolerance */
     int32_t converged;         /* 1 if the true relative residual met the tolerance */
     double rhs_norm;           /* ||b||_2 */
     double rel_residual;       /* ||b-Ax||_2 / ||b||_2, true FP64 residual */
@@ -153,11 +160,15 @@ int mps_get_solver_stats(const mps_handle* h, mps_solver_stats* out);
 int mps_load_initial_state(mps_handle* h);
 
 /* Replaces MPS::stepForward (src/mps.cpp:111-130). Writes the Courant number
- * (src/mps.cpp:547-571) to *courant when courant is not NULL. */
+ * (src/mps.cpp:547-571) to *courant when courant is not NULL.  Returns
+ * MPS_E_SOLVER after a completed step whose pressure solve stopped above its
+ * tolerance (iteration or refinement cap, or a tolerance below what FP64
+ * rounding of A x allows); the step used the solution as it stood. */
 int mps_step(mps_handle* h, int32_t is_export, double* courant);
 
 /* Runs n_steps steps with no export and no host synchronisation between
- * them; writes each step's Courant number to courants[n_steps] if not NULL. */
+ * them; writes each step's Courant number to courants[n_steps] if not NULL.
+ * Every step runs; MPS_E_SOLVER names the first whose solve missed. */
 int mps_run_steps(mps_handle* h, int32_t n_steps, double* courants);
 
 /* Copies the particle state out in id order (the fields Exporter reads,
@@ -186,6 +197,14 @@ int mps_debug_events(const mps_handle* h, int64_t counts[3],
 int mps_clear_debug_events(mps_handle* h);
 
 /* ---- test and inspection entry points ---- */
+
+/* Internal counters (no counterpart in the reference; for tests):
+ *   out[0] = times a solver's captured CG graphs were dropped because a
+ *            pointer or launch shape they bake in changed,
+ *   out[1] = CG vector reallocations,
+ *   out[2] = pressure-matrix storage reallocations (entries or 3D anchors).
+ * Writes min(cap, 3) values. */
+int mps_debug_counters(const mps_handle* h, int64_t* out, int32_t cap);
 
 /* Bucket grid geometry (src/bucket.hpp:11-14). */
 int mps_bucket_shape(const mps_handle* h, int32_t dims[4] /* num,numX,numY,numZ */,
@@ -285,7 +304,8 @@ int mps_snapshot_end(mps_handle* h, int32_t slot, const int8_t** type, const dou
  * resumed from it continues bit for bit as if uninterrupted.  The image is
  * mps_checkpoint_size() bytes (a 64-byte header, then 12 doubles and 2 bytes
  * per particle); mps_checkpoint_load accepts only an image of a handle of the
- * same shape (particle count, dim, scenes) and validates it first.  Debug
+ * same shape (particle count, dim, scenes, each scene's particle count and
+ * settings) and validates it first.  Debug
  * events and solver statistics are not part of the state. */
 int64_t mps_checkpoint_size(const mps_handle* h);
 int mps_checkpoint_save(mps_handle* h, void* buf, int64_t cap);

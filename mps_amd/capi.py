@@ -21,8 +21,8 @@ MPS_PREC_MIXED = 1
 MPS_PRECOND_JACOBI = 0
 MPS_PRECOND_MULTIGRID = 1
 
-_ERRORS = {1: "MPS_E_INVALID", 2: "MPS_E_HIP", 3: "MPS_E_NOMEM", 4: "MPS_E_RANGE", 5: "MPS_E_SOLVER",
-           6: "MPS_E_ASYM"}
+MPS_E_SOLVER = 5
+_ERRORS = {1: "MPS_E_INVALID", 2: "MPS_E_HIP", 3: "MPS_E_NOMEM", 4: "MPS_E_RANGE", 5: "MPS_E_SOLVER"}
 
 EXPORTED = [
     "mps_last_error", "mps_create", "mps_destroy", "mps_set_solver", "mps_get_solver_stats",
@@ -32,7 +32,7 @@ EXPORTED = [
     "mps_time_pressure_kernels", "mps_set_block_sizes", "mps_autotune",
     "mps_create_ensemble", "mps_scene_count", "mps_ensemble_stats",
     "mps_adjoint_record", "mps_adjoint_backward", "mps_snapshot_begin", "mps_snapshot_end",
-    "mps_checkpoint_size", "mps_checkpoint_save", "mps_checkpoint_load",
+    "mps_checkpoint_size", "mps_checkpoint_save", "mps_checkpoint_load", "mps_debug_counters",
 ]
 
 
@@ -120,6 +120,7 @@ def lib():
     L.mps_checkpoint_size.argtypes = [vp]
     L.mps_checkpoint_save.argtypes = [vp, vp, ctypes.c_int64]
     L.mps_checkpoint_load.argtypes = [vp, vp, ctypes.c_int64]
+    L.mps_debug_counters.argtypes = [vp, vp, i32]
     for name in EXPORTED:
         if name not in ("mps_last_error", "mps_checkpoint_size"):
             getattr(L, name).restype = i32
@@ -167,6 +168,8 @@ class Simulation:
             raise MpsError(rc, self.L.mps_last_error(None).decode())
         self.h = h
         self.nscene = len(self.offsets) - 1
+        self.strict_solver = False  # True: a step whose solve missed its tolerance raises
+        self.solver_warning = None  # message of the last such step
 
     def scene_stats(self):
         """Per-scene solver statistics and Courant numbers of the last step."""
@@ -242,6 +245,12 @@ class Simulation:
         o.preconditioner = int(preconditioner)
         self._check(self.L.mps_set_solver(self.h, ctypes.byref(o)))
 
+    def debug_counters(self):
+        """{graph_rebuilds, cg_reallocs, matrix_reallocs} (internal counters)."""
+        out = np.zeros(3, np.int64)
+        self._check(self.L.mps_debug_counters(self.h, _p(out), 3))
+        return dict(graph_rebuilds=int(out[0]), cg_reallocs=int(out[1]), matrix_reallocs=int(out[2]))
+
     def solver_stats(self):
         s = SolverStats()
         self._check(self.L.mps_get_solver_stats(self.h, ctypes.byref(s)))
@@ -250,14 +259,25 @@ class Simulation:
     def load_initial_state(self):
         self._check(self.L.mps_load_initial_state(self.h))
 
+    def _check_step(self, rc):
+        # MPS_E_SOLVER: the step completed with a solve that stopped above its
+        # tolerance; like the reference (which ignores the solver's status,
+        # src/mps.cpp:457-460) the run goes on, and the message is kept
+        if rc == MPS_E_SOLVER:
+            self.solver_warning = self.L.mps_last_error(self.h).decode()
+            if self.strict_solver:
+                raise MpsError(rc, self.solver_warning)
+            return
+        self._check(rc)
+
     def step(self, is_export=False):
         c = ctypes.c_double()
-        self._check(self.L.mps_step(self.h, int(bool(is_export)), ctypes.byref(c)))
+        self._check_step(self.L.mps_step(self.h, int(bool(is_export)), ctypes.byref(c)))
         return c.value
 
     def run_steps(self, k):
         out = np.zeros(max(k, 1))
-        self._check(self.L.mps_run_steps(self.h, k, _p(out)))
+        self._check_step(self.L.mps_run_steps(self.h, k, _p(out)))
         return out[:k]
 
     def state(self):

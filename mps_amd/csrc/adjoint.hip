@@ -16,6 +16,12 @@
 //   w'(r) = -re / r^2
 //   phi(r) = w(r) / r^2,  phi'(r) / r = 2 / r^4 - 3 re / r^5
 //   dr/dx_j = d / r = -dr/dx_i
+//
+// No multiply-add is fused here, as in the forward stages: the neighbour
+// walks recompute distances, and a pair within an ulp of a radius must take
+// the same side of dist < re as the recorded forward run did.
+#pragma clang fp contract(off)
+
 #include "mps_internal.hpp"
 
 namespace mps {

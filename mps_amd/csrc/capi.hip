@@ -112,6 +112,7 @@ struct mps_handle {
     double* vmax_dst = nullptr;  // where the next drain puts the scenes' largest speeds
     double* pinned_d = nullptr;
     double* vmax = nullptr;
+    std::string solve_msg;  // set when a pressure solve of the last step stopped above its tolerance
     int want_raw = 0;
     int want_snap = 0;
     int have_system = 0;
@@ -560,6 +561,7 @@ static int step_core(mps_handle* h, int is_export, double* vmax_out, StepTap* ta
     tend(h);
     std::string msg;
     HIPCHK(h, cg_solve(h->cg, h->A, c, n, h->w.b_s, h->cold.pprev, h->cold.pprev2, h->x_s, h->opt, h->sstats.data(), h->st, msg));
+    h->solve_msg = msg;
     tbeg(h, F_POUT);
     HIPCHK(h, launch_pressure_out(h->dim, g, tab(h), n, h->c, c, h->x_s, h->cold, h->w, h->st, h->bs[F_POUT]));
     tend(h);
@@ -1096,6 +1098,22 @@ int mps_load_initial_state(mps_handle* h) {
     return drain(h);
 }
 
+// Message of a solve that stopped above its tolerance, naming the scenes.
+static std::string solve_failure(const mps_handle* h) {
+    std::string m = h->solve_msg;
+    if (h->nscene > 1) {
+        m += " (scenes";
+        for (int k = 0; k < h->nscene; k++)
+            if (!h->sstats[k].converged) m += " " + std::to_string(k);
+        m += ")";
+    }
+    char buf[96];
+    double worst = 0.0;
+    for (int k = 0; k < h->nscene; k++) worst = std::max(worst, h->sstats[k].rel_residual);
+    std::snprintf(buf, sizeof buf, ": relative residual %.3e > %.3e", worst, h->opt.tolerance);
+    return m + buf;
+}
+
 int mps_step(mps_handle* h, int32_t is_export, double* courant) {
     if (!h) return MPS_E_INVALID;
     int rc = step_core(h, is_export, h->pinned_d);
@@ -1104,18 +1122,26 @@ int mps_step(mps_handle* h, int32_t is_export, double* courant) {
     if (rc) return rc;
     const double cn = courant_of(h, h->pinned_d);
     if (courant) *courant = cn;
+    if (!h->solve_msg.empty()) return fail(h, MPS_E_SOLVER, solve_failure(h));
     return MPS_OK;
 }
 
 int mps_run_steps(mps_handle* h, int32_t n_steps, double* courants) {
     if (!h || n_steps < 0) return MPS_E_INVALID;
+    int first_bad = -1;
+    std::string bad;
     for (int32_t k = 0; k < n_steps; k++) {
         int rc = step_core(h, 0, h->pinned_d);
         if (rc) return rc;
         if ((rc = drain(h))) return rc;
         const double cn = courant_of(h, h->pinned_d);
         if (courants) courants[k] = cn;
+        if (!h->solve_msg.empty() && first_bad < 0) {
+            first_bad = k;
+            bad = solve_failure(h);
+        }
     }
+    if (first_bad >= 0) return fail(h, MPS_E_SOLVER, "step " + std::to_string(first_bad) + ": " + bad);
     return MPS_OK;
 }
 
@@ -1248,6 +1274,15 @@ int mps_debug_events(const mps_handle* hc, int64_t counts[3], int32_t* ghost_ids
         for (size_t k = 0; k < a.size() && (int32_t) k < cap; k++) ghost_ids[k] = a[k];
     if (unconnected_ids)
         for (size_t k = 0; k < b.size() && (int32_t) k < cap; k++) unconnected_ids[k] = b[k];
+    return MPS_OK;
+}
+
+int mps_debug_counters(const mps_handle* h, int64_t* out, int32_t cap) {
+    if (!h || (cap > 0 && !out)) return MPS_E_INVALID;
+    long long cg[2];
+    cg_debug_counts(h->cg, cg);
+    const int64_t v[3] = {cg[0], cg[1], h->A.grows};
+    for (int k = 0; k < cap && k < 3; k++) out[k] = v[k];
     return MPS_OK;
 }
 
@@ -1612,7 +1647,8 @@ int mps_snapshot_end(mps_handle* h, int32_t slot, const int8_t** type, const dou
 }
 
 // ---- lossless checkpoints ----
-// Image: 64-byte header {magic "MPSAMDCK", version 1, n, dim, nscene, 0...},
+// Image: 64-byte header {magic "MPSAMDCK", version 2, n, dim, nscene, sizes,
+// a hash of every scene's particle count and settings, 0...},
 // then the saved state (kStateDoubles x n doubles, kStateBytes x n bytes) in
 // the layout of save_state: per particle id x y z vx vy vz, warm-start
 // solution, pressure, minimum pressure, number density ratio, a ghost's
@@ -1621,9 +1657,33 @@ struct CkHeader {
     char magic[8];
     int32_t version, n, dim, nscene;
     int64_t doubles, bytes;
-    char pad[24];
+    uint64_t shape_hash;  // ck_shape_hash of the handle that saved it
+    char pad[16];
 };
 static_assert(sizeof(CkHeader) == 64, "checkpoint header is 64 bytes");
+constexpr int32_t kCkVersion = 2;
+
+// FNV-1a over each scene's particle count and settings (every field but the
+// reserved one): an image loads only into a handle whose scenes hold as many
+// particles each and step under the same settings.
+static uint64_t ck_shape_hash(const mps_handle* h) {
+    uint64_t x = 1469598103934665603ull;
+    auto mix = [&x](const void* p, size_t len) {
+        const unsigned char* b = (const unsigned char*) p;
+        for (size_t i = 0; i < len; i++) {
+            x ^= b[i];
+            x *= 1099511628211ull;
+        }
+    };
+    for (int k = 0; k < h->nscene; k++) {
+        const SceneInfo& sc = h->scenes[k];
+        const int32_t ints[4] = {sc.n, sc.s.dim, sc.s.qc_on, sc.s.hos_on};
+        mix(ints, sizeof ints);
+        const char* d0 = (const char*) &sc.s.particle_distance;
+        mix(d0, (size_t) ((const char*) (&sc.s.restitution + 1) - d0));
+    }
+    return x;
+}
 
 int64_t mps_checkpoint_size(const mps_handle* h) {
     if (!h) return -1;
@@ -1648,7 +1708,8 @@ int mps_checkpoint_save(mps_handle* h, void* buf, int64_t cap) {
     if (rc) return rc;
     CkHeader hd{};
     memcpy(hd.magic, "MPSAMDCK", 8);
-    hd.version = 1;
+    hd.version = kCkVersion;
+    hd.shape_hash = ck_shape_hash(h);
     hd.n = h->n;
     hd.dim = h->dim;
     hd.nscene = h->nscene;
@@ -1677,10 +1738,13 @@ int mps_checkpoint_load(mps_handle* h, const void* buf, int64_t size) {
     if (size < (int64_t) sizeof(CkHeader)) return fail(h, MPS_E_INVALID, "checkpoint image too short");
     CkHeader hd;
     memcpy(&hd, buf, sizeof hd);
-    if (memcmp(hd.magic, "MPSAMDCK", 8) != 0 || hd.version != 1)
+    if (memcmp(hd.magic, "MPSAMDCK", 8) != 0 || hd.version != kCkVersion)
         return fail(h, MPS_E_INVALID, "not a checkpoint image of this library");
     if (hd.n != h->n || hd.dim != h->dim || hd.nscene != h->nscene)
         return fail(h, MPS_E_INVALID, "checkpoint was taken from a handle of another shape");
+    if (hd.shape_hash != ck_shape_hash(h))
+        return fail(h, MPS_E_INVALID,
+                    "checkpoint was taken from a handle with other per-scene particle counts or settings");
     if (size < mps_checkpoint_size(h) || hd.doubles != (int64_t) kStateDoubles * h->n ||
         hd.bytes != (int64_t) kStateBytes * h->n)
         return fail(h, MPS_E_INVALID, "checkpoint image truncated");

@@ -886,7 +886,12 @@ struct Sell {
     int* raw_col;        // slots
     double* raw_val;
     size_t raw_cap;
+    long long grows = 0;  // times the entry or anchor storage was reallocated
 };
+
+// MPS_AMD_TIGHT_CAPS=1 (tests): capacities grow to exactly what a system
+// needs, so every larger system reallocates its storage.
+bool tight_caps();
 
 // Column of SELL entry idx of row `row`.  int: the column itself.  short
 // (2D): column - row.  unsigned short (3D): the row's neighbours lie in
@@ -956,6 +961,8 @@ void cg_destroy(CgWork* cw);
 // the step's own solves, 1 for the adjoint's.
 void cg_prediction_bank(CgWork* cw, int bank);
 hipError_t cg_reserve(CgWork* cw, int rows);
+// [0] times a solver's cached graphs were dropped, [1] CG vector reallocations
+void cg_debug_counts(const CgWork* cw, long long out[2]);
 // Solves A x = b for the Inner rows.  b_s / x_s in slot order; x0_id (id order,
 // may be NULL) gives the warm start.  Fills stats.
 // stats: one entry per segment.

@@ -263,7 +263,8 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
     A.rows_dev = w.row_of + n;
     A.nslices = (rows + 63) / 64;
     if (entries > A.cap) {
-        size_t cap = entries + entries / 4 + 64 * 64;
+        size_t cap = tight_caps() ? entries : entries + entries / 4 + 64 * 64;
+        A.grows++;
         hipFree(A.col);
         hipFree(A.dcol);
         hipFree(A.val64);
@@ -312,7 +313,8 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
         hipFree(A.anc);
         A.anc = nullptr;
         A.anc_cap = 0;
-        const size_t cap = (size_t) rows + rows / 4 + 64;
+        A.grows++;
+        const size_t cap = tight_caps() ? (size_t) rows : (size_t) rows + rows / 4 + 64;
         if ((e = hipMalloc(&A.anc, 2 * cap * sizeof(int))) != hipSuccess) return e;
         A.anc_cap = cap;
     }
@@ -373,6 +375,9 @@ constexpr int kCgBlock = 512;
 // iterations it saves (measured: 584-particle dam break), so small systems
 // keep the Jacobi scaling alone.
 constexpr int kMgMinRows = 8192;
+// FP64 CG restarts from the current iterate when the true residual of the
+// recurrence's answer misses the tolerance.
+constexpr int kMaxRestarts = 3;
 
 static int cg_blocks(int rows) {
     int nb = (rows + kBlock - 1) / kBlock;
@@ -386,12 +391,12 @@ struct CgScalars {
     double thr;        // stop when rho2 <= thr
     double rho2;       // latest residual^2 (unscaled sum a_ii r_i^2, or r.r for inner solves)
     double red[3];     // results of k_reduce_parts
-    int iter;          // apply launches run (= completed updates once done)
+    int iter;          // apply launches run (read by every block of k_cg_step)
     int max_iter;
     int done;
     int converged;
     int frozen;        // timing mode: kernels leave every scalar as it is
-    int pad;
+    int fin;           // completed updates, written once when the segment stops
 };
 
 struct Seg {
@@ -566,11 +571,13 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_step(Seg sg, T* __restrict__ y,
     } else {
         const bool conv = rho <= sc->thr;
         if (conv || it > sc->max_iter || !(rho == rho)) {
+            // iter itself stays: the other blocks of this launch read it
+            // and must all see the same count to take the same branch
             if (lead) {
                 sc->done = 1;
                 sc->converged = conv ? 1 : 0;
                 sc->rho2 = rho;
-                sc->iter = it - 1;
+                sc->fin = it - 1;
             }
             return;
         }
@@ -755,16 +762,20 @@ struct CgWork {
     hipGraphExec_t graph[2][kGraphSlots] = {};
     int graph_chunk[2][kGraphSlots] = {};
     int graph_next[2] = {0, 0};  // slot to replace next
-    const void* graph_key[2] = {nullptr, nullptr};
+    // everything a captured chunk bakes in: every device pointer its kernels
+    // receive, the launch shape and the multigrid hierarchy's generation.
+    // Any difference (a buffer reallocated by a larger system, another
+    // column width) drops the cached graphs of that solver.
+    static constexpr int kSig = 24;
+    uintptr_t graph_sig[2][kSig] = {};
     static constexpr int kPredSolves = 4;
     bool predict = true;  // MPS_AMD_CG_PREDICT=0: fixed chunks of the check interval
     // last step's iterations per solve of a step; bank 1 holds the adjoint
     // solves', whose counts (zero start) differ from the forward ones'
     int pred[2][2][kPredSolves] = {};
     int pred_bank = 0;
-    int graph_idx16[2] = {-1, -1};
-    int graph_blocks[2] = {0, 0};
-    int graph_mg[2] = {-2, -2};  // multigrid generation captured (-1: none)
+    long long graph_rebuilds = 0;  // times a solver's cached graphs were dropped
+    long long vec_reallocs = 0;    // times the CG vectors were reallocated
     Seg seg() const { return Seg{nseg, blk_off, row_off}; }
 };
 
@@ -819,9 +830,20 @@ static hipError_t cg_reserve_segs(CgWork* cw, int nseg) {
     return hipSuccess;
 }
 
+bool tight_caps() {
+    const char* v = getenv("MPS_AMD_TIGHT_CAPS");
+    return v && atoi(v) != 0;
+}
+
+void cg_debug_counts(const CgWork* cw, long long out[2]) {
+    out[0] = cw->graph_rebuilds;
+    out[1] = cw->vec_reallocs;
+}
+
 hipError_t cg_reserve(CgWork* cw, int rows) {
     if (rows <= cw->cap && cw->y) return hipSuccess;
-    const int cap = rows + rows / 8 + 1024;
+    const int cap = tight_caps() ? rows : rows + rows / 8 + 1024;
+    cw->vec_reallocs++;
     cg_free_vectors(cw);
     hipError_t e;
     const size_t d = (size_t) cap * sizeof(double), f = (size_t) cap * sizeof(float);
@@ -924,18 +946,21 @@ static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* c
     const size_t sbytes = sizeof(CgScalars) * (size_t) cw->nseg;
     hipError_t e;
     if (o.use_graph) {
-        const int mgk = cw->mg_on ? mg_generation(cw->mg) : -1;
-        if (cw->graph_key[which] != (const void*) val || cw->graph_idx16[which] != idx16 ||
-            cw->graph_blocks[which] != cw->total_blocks || cw->graph_mg[which] != mgk) {
+        const uintptr_t sig[CgWork::kSig] = {
+            (uintptr_t) val,         (uintptr_t) col,          (uintptr_t) A.anc,       (uintptr_t) A.slice_off,
+            (uintptr_t) A.dgl,       (uintptr_t) y,            (uintptr_t) r,           (uintptr_t) p,
+            (uintptr_t) s,           (uintptr_t) w,            (uintptr_t) u_vec<T>(cw), (uintptr_t) cw->part,
+            (uintptr_t) cw->sc,      (uintptr_t) cw->blk_off,  (uintptr_t) cw->row_off, (uintptr_t) A.rows_dev,
+            (uintptr_t) cw->nseg,    (uintptr_t) cw->total_blocks, (uintptr_t) idx16,
+            (uintptr_t) (cw->mg_on ? mg_generation(cw->mg) + 1 : 0), (uintptr_t) cw->mg_on, 0, 0, 0};
+        if (std::memcmp(sig, cw->graph_sig[which], sizeof(sig)) != 0) {
             for (int j = 0; j < CgWork::kGraphSlots; j++) {
                 if (cw->graph[which][j]) hipGraphExecDestroy(cw->graph[which][j]);
                 cw->graph[which][j] = nullptr;
                 cw->graph_chunk[which][j] = 0;
             }
-            cw->graph_key[which] = val;
-            cw->graph_idx16[which] = idx16;
-            cw->graph_blocks[which] = cw->total_blocks;
-            cw->graph_mg[which] = mgk;
+            std::memcpy(cw->graph_sig[which], sig, sizeof(sig));
+            cw->graph_rebuilds++;
         }
     }
     int launched = 0, grow = 2;
@@ -1028,7 +1053,7 @@ static hipError_t run_cg(CgWork* cw, int which, int solve, const Sell& A, const 
         e = run_cg_loop<T, int, WEIGHTED>(cw, which, A, A.col, val, y, r, p, s, w, o, pred, st);
     if (e) return e;
     int it = 0;  // the segment that took longest sets the chunk
-    for (int k = 0; k < cw->nseg; k++) it = std::max(it, cw->host[k].iter);
+    for (int k = 0; k < cw->nseg; k++) it = std::max(it, cw->host[k].fin);
     *last = it;
     return hipSuccess;
 }
@@ -1082,7 +1107,8 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
         if (cw->mg_on && mg_bad) cw->mg_on = 0;  // a coarse row outside the stencil: Jacobi alone
     }
     std::vector<double> bn2(nseg), thr(nseg), rho2(nseg);
-    std::vector<int> segrows(nseg), conv(nseg), total_it(nseg, 0), refinements(nseg, 0), max_it(nseg);
+    std::vector<int> segrows(nseg), conv(nseg), total_it(nseg, 0), refinements(nseg, 0), max_it(nseg),
+        restarts(nseg, 0);
     for (int s = 0; s < nseg; s++) {
         segrows[s] = cw->h_row[s + 1] - cw->h_row[s];
         bn2[s] = cw->host[s].red[2];
@@ -1108,30 +1134,47 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
         if ((e = reduce_and_read(cw, 2, st))) return e;
         for (int s = 0; s < nseg; s++) rho2[s] = cw->host[s].red[1];
     }
-    bool all_conv = true;
     for (int s = 0; s < nseg; s++) {
         conv[s] = bn2[s] == 0.0 || rho2[s] <= thr[s];
         stats[s].initial_residual = bn2[s] > 0 ? sqrt(rho2[s] / bn2[s]) : 0.0;
-        all_conv = all_conv && conv[s];
     }
 
     if (o.precision == MPS_PREC_FP64) {
-        if (!all_conv) {
-            // part holds the initial residual partials for the first apply
+        // CG from the current iterate (r64 holds its true residual), then the
+        // true residual of the result decides convergence: the recurrence's
+        // residual drifts from it by rounding, so a segment whose true
+        // residual misses the tolerance restarts from its iterate while
+        // iterations remain and restarts still halve it (a tolerance below
+        // the rounding floor of A x stops after one restart)
+        std::vector<int> stalled(nseg, 0);
+        for (int pass = 0; pass <= kMaxRestarts; pass++) {
+            std::vector<int> ran(nseg);
+            bool any = false;
             for (int s = 0; s < nseg; s++) {
+                ran[s] = !conv[s] && !stalled[s] && total_it[s] < max_it[s];
+                any = any || ran[s];
                 CgScalars z{};
                 z.thr = thr[s];
-                z.max_iter = max_it[s];
-                z.done = conv[s];
+                z.max_iter = max_it[s] - total_it[s];
+                z.done = !ran[s];
                 cw->host[s] = z;
             }
+            if (!any) break;
             if ((e = upload_scalars(cw, st))) return e;
-            if ((e = run_cg<double, true>(cw, 0, 0, A, A.val64, cw->y, cw->r64, cw->p64, cw->s64, cw->w64, o, st)))
+            if ((e = run_cg<double, true>(cw, 0, pass, A, A.val64, cw->y, cw->r64, cw->p64, cw->s64, cw->w64, o,
+                                          st)))
                 return e;
+            for (int s = 0; s < nseg; s++)
+                if (ran[s]) total_it[s] += cw->host[s].fin;
+            launch_residual(cw, A, st);
+            if ((e = reduce_and_read(cw, 2, st))) return e;
             for (int s = 0; s < nseg; s++) {
-                if (conv[s]) continue;
-                total_it[s] = cw->host[s].iter;
-                conv[s] = cw->host[s].converged != 0;
+                if (!ran[s]) continue;
+                const double before = rho2[s];
+                rho2[s] = cw->host[s].red[1];
+                conv[s] = rho2[s] <= thr[s];
+                restarts[s] += pass > 0;
+                stalled[s] = pass > 0 && !(rho2[s] <= 0.25 * before);
             }
         }
     } else {
@@ -1165,7 +1208,7 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
                                           st)))
                 return e;
             for (int s = 0; s < nseg; s++)
-                if (run[s]) total_it[s] += cw->host[s].iter;
+                if (run[s]) total_it[s] += cw->host[s].fin;
             // segments that did not run keep e = 0 from k_inner_init
             hipLaunchKernelGGL(k_add_correction, gr, bl, 0, st, sg, cw->y, cw->e32);
             launch_residual(cw, A, st);
@@ -1178,18 +1221,12 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
             }
         }
     }
-    // true FP64 residual of the final iterate: the refinement loop (and a
-    // solve that needed no iteration) already computed it from this iterate
-    const bool fresh = o.precision != MPS_PREC_FP64 || all_conv;
-    if (!fresh) {
-        launch_residual(cw, A, st);
-        if ((e = reduce_and_read(cw, 2, st))) return e;
-        for (int s = 0; s < nseg; s++) rho2[s] = cw->host[s].red[1];
-    }
+    // rho2 now holds the true FP64 residual of the final iterate: both
+    // branches (and a solve that needed no iteration) computed it from it
     bool all_ok = true;
     for (int s = 0; s < nseg; s++) {
         stats[s].iterations = total_it[s];
-        stats[s].refinements = refinements[s];
+        stats[s].refinements = o.precision == MPS_PREC_FP64 ? restarts[s] : refinements[s];
         stats[s].rel_residual = bn2[s] > 0 ? sqrt(rho2[s] / bn2[s]) : 0.0;
         stats[s].converged = conv[s] ? 1 : 0;
         all_ok = all_ok && conv[s];

@@ -140,7 +140,13 @@ void MPS::reportEvents(double courant) {
 // src/mps.cpp:111-130
 double MPS::stepForward(const bool isTimeToExport) {
     double courant = 0.0;
-    check(mps_step(h, isTimeToExport ? 1 : 0, &courant), "mps_step");
+    const int rc = mps_step(h, isTimeToExport ? 1 : 0, &courant);
+    // the reference ignores its solver's status (src/mps.cpp:457-460): a
+    // solve that stopped above its tolerance is logged and the run goes on
+    if (rc == MPS_E_SOLVER)
+        mpslog::debug(mps_last_error(h));
+    else
+        check(rc, "mps_step");
     reportEvents(courant);
     if (isTimeToExport && syncOnExport) syncParticles();
     lastCourant = courant;

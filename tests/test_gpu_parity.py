@@ -6,6 +6,7 @@ state entering the pressure stage.  Within stated FP64 tolerances: the
 pressure solution (two different CG summation orders, both run to near
 machine precision) and the state after the step.
 """
+import ctypes
 import dataclasses
 
 import numpy as np
@@ -367,6 +368,75 @@ def test_graph_and_stream_paths_identical():
         assert (res[0][k] == res[1][k]).all()
 
 
+@pytest.mark.parametrize("case", ["3d_jacobi", "2d_multigrid_mixed"])
+def test_graph_replay_after_storage_grows(case, monkeypatch):
+    """With every capacity exactly the last system's size
+    (MPS_AMD_TIGHT_CAPS=1), each larger system reallocates the matrix storage
+    (3D: also the row anchors) and the CG vectors between steps.  The cached
+    CG graphs bake those pointers in, so they must be dropped and captured
+    again: graph replay and direct launches give the same bits over 10 steps,
+    and the counters show that storage did grow under the graphs."""
+    monkeypatch.setenv("MPS_AMD_TIGHT_CAPS", "1")
+    if case == "3d_jacobi":
+        s = configs.dambreak_settings(0.02, 1e-3, 1.0, dim=3)
+        sc = configs.lattice_3d(0.02, (0.3, 0.3, 0.1), (0.0, 0.1), (0.0, 0.2), (0.0, 0.1), 1, 2)
+        s.domain_x, s.domain_y, s.domain_z = (-0.1, 0.4), (-0.1, 0.4), (-0.1, 0.2)
+        sc = _jitter(s, sc, 25, amp=0.1, vamp=0.3)
+        opts = dict(precision=capi.MPS_PREC_FP64, tolerance=1e-12, warm_start=True)
+    else:
+        s, sc, _ = configs.workload("dambreak2d_35k")
+        sc = _jitter(s, sc, 26, amp=0.1, vamp=0.3)
+        opts = dict(precision=capi.MPS_PREC_MIXED, tolerance=1e-9, warm_start=2,
+                    preconditioner=capi.MPS_PRECOND_MULTIGRID)
+    res, counters = [], []
+    for use_graph in (True, False):
+        g = capi.Simulation(s, sc)
+        g.set_solver(use_graph=use_graph, check_interval=4, **opts)
+        g.load_initial_state()
+        for _ in range(10):
+            g.step()
+        res.append(g.state())
+        counters.append(g.debug_counters())
+        g.close()
+    monkeypatch.delenv("MPS_AMD_TIGHT_CAPS", raising=False)
+    for k in ("type", "pos", "vel", "p", "ndr", "bc", "pmin"):
+        assert (res[0][k] == res[1][k]).all(), k
+    c = counters[0]
+    assert c["matrix_reallocs"] >= 2 and c["cg_reallocs"] >= 2, c
+    assert c["graph_rebuilds"] >= 2, c
+
+
+@pytest.mark.parametrize("precision", [capi.MPS_PREC_FP64, capi.MPS_PREC_MIXED])
+def test_capped_solve_reports_solver_status(precision):
+    """A solve capped below what its tolerance needs completes the step and
+    returns MPS_E_SOLVER with the residual in the message; the statistics say
+    it did not converge, and an uncapped solve of the same step converges."""
+    s, sc, _ = configs.workload("dambreak2d_584")
+    sc = _jitter(s, sc, 27)
+    g = capi.Simulation(s, sc)
+    g.set_solver(precision=precision, tolerance=1e-12, max_iterations=3)
+    g.load_initial_state()
+    c = ctypes.c_double()
+    rc = g.L.mps_step(g.h, 0, ctypes.byref(c))
+    assert rc == capi.MPS_E_SOLVER
+    msg = g.L.mps_last_error(g.h).decode()
+    assert "tolerance" in msg and "relative residual" in msg
+    st = g.solver_stats()
+    assert st["converged"] == 0 and st["rel_residual"] > 1e-12 and st["iterations"] <= 3
+    assert c.value > 0.0  # the step ran to its Courant number
+    g.strict_solver = True
+    with pytest.raises(capi.MpsError, match="MPS_E_SOLVER"):
+        g.step()
+    g.strict_solver = False
+    g.step()  # not strict: the run goes on, the message is kept
+    assert g.solver_warning and "tolerance" in g.solver_warning
+    g.set_solver(precision=precision, tolerance=1e-12)
+    g.solver_warning = None
+    g.step()
+    st = g.solver_stats()
+    assert st["converged"] == 1 and st["rel_residual"] <= 1e-12 and g.solver_warning is None
+
+
 def test_autotuner_changes_speed_not_results():
     s, sc, _ = configs.workload("dambreak2d_584")
     sc = _jitter(s, sc, 11)
@@ -454,3 +524,21 @@ def test_checkpoint_resume_is_bit_identical(warm):
     other = capi.Simulation(*configs.workload("dambreak2d_35k")[:2])
     with pytest.raises(capi.MpsError):
         other.restore(img)
+    # same particle count, dim and scene count, other settings: refused
+    s2 = dataclasses.replace(s, dt=s.dt * 0.5)
+    with pytest.raises(capi.MpsError, match="settings"):
+        capi.Simulation(s2, sc).restore(img)
+    # ensembles: the same totals with other per-scene counts are refused
+    small = configs.lattice_2d(s.particle_distance, (1.0, 0.6), (0.0, 0.15), (0.0, 0.25), 2, 2)
+    assert small.n != sc.n
+    e1 = capi.Simulation([(s, sc), (s, small)])
+    e1.load_initial_state()
+    e1.step()
+    eimg = e1.checkpoint()
+    with pytest.raises(capi.MpsError, match="per-scene"):
+        capi.Simulation([(s, small), (s, sc)]).restore(eimg)
+    e2 = capi.Simulation([(s, sc), (s, small)])
+    e2.restore(eimg)
+    se1, se2 = e1.state(), e2.state()
+    for k in se1:
+        assert (se1[k] == se2[k]).all(), k

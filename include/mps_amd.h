@@ -246,7 +246,9 @@ int mps_autotune(mps_handle* h, int32_t steps_per_candidate, int32_t block_sizes
 
 /* Times the pressure-solver kernels on the system of the last step, with HIP
  * events on the handle's stream, averaged over reps launches each (no
- * counterpart in the reference; a measurement hook).  out_ms / out_bytes:
+ * counterpart in the reference; a measurement hook).  The variants timed are
+ * the ones the last solve ran (with the multigrid preconditioner: the apply
+ * that gathers u = M r and the step that reads it).  out_ms / out_bytes:
  *   [0] FP64 SpMV, [1] FP64 vector update, [2] FP32 SpMV, [3] FP32 update,
  * bytes = algorithmic bytes of one launch (matrix entries, column indices and
  * vectors each read or written once).  Leaves the solver scratch undefined. */

@@ -1236,28 +1236,38 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
     return hipGetLastError();
 }
 
-template <typename IDX>
+// The variants the solver itself runs: PRE when the last solve used the
+// multigrid preconditioner (the apply gathers u = M r), else plain.
+template <typename IDX, bool PRE>
 static void time_launch(CgWork* cw, const Sell& A, const IDX* col, int kind, hipStream_t st) {
     const dim3 gr(cw->total_blocks), bl(kCgBlock);
     const Seg sg = cw->seg();
     switch (kind) {
         case 0:
-            hipLaunchKernelGGL((k_cg_apply<double, IDX, true, false>), gr, bl, 0, st, sg, A.slice_off, col, A.anc,
-                               A.val64, cw->r64, cw->r64, cw->w64, A.dgl, cw->part, cw->sc);
+            hipLaunchKernelGGL((k_cg_apply<double, IDX, true, PRE>), gr, bl, 0, st, sg, A.slice_off, col, A.anc,
+                               A.val64, cw->r64, PRE ? cw->u64 : cw->r64, cw->w64, A.dgl, cw->part, cw->sc);
             break;
         case 1:
-            hipLaunchKernelGGL((k_cg_step<double, true, false>), gr, bl, 0, st, sg, cw->y, cw->r64, cw->p64, cw->s64,
-                               cw->w64, cw->r64, cw->part, cw->sc, 0);
+            hipLaunchKernelGGL((k_cg_step<double, true, PRE>), gr, bl, 0, st, sg, cw->y, cw->r64, cw->p64, cw->s64,
+                               cw->w64, PRE ? cw->u64 : cw->r64, cw->part, cw->sc, 0);
             break;
         case 2:
-            hipLaunchKernelGGL((k_cg_apply<float, IDX, false, false>), gr, bl, 0, st, sg, A.slice_off, col, A.anc,
-                               A.val32, cw->r32, cw->r32, cw->w32, A.dgl, cw->part, cw->sc);
+            hipLaunchKernelGGL((k_cg_apply<float, IDX, false, PRE>), gr, bl, 0, st, sg, A.slice_off, col, A.anc,
+                               A.val32, cw->r32, PRE ? cw->u32 : cw->r32, cw->w32, A.dgl, cw->part, cw->sc);
             break;
         default:
-            hipLaunchKernelGGL((k_cg_step<float, false, false>), gr, bl, 0, st, sg, cw->e32, cw->r32, cw->p32, cw->s32,
-                               cw->w32, cw->r32, cw->part, cw->sc, 0);
+            hipLaunchKernelGGL((k_cg_step<float, false, PRE>), gr, bl, 0, st, sg, cw->e32, cw->r32, cw->p32, cw->s32,
+                               cw->w32, PRE ? cw->u32 : cw->r32, cw->part, cw->sc, 0);
             break;
     }
+}
+
+template <typename IDX>
+static void time_launch(CgWork* cw, const Sell& A, const IDX* col, int kind, bool pre, hipStream_t st) {
+    if (pre)
+        time_launch<IDX, true>(cw, A, col, kind, st);
+    else
+        time_launch<IDX, false>(cw, A, col, kind, st);
 }
 
 hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, hipStream_t st, double ms[4],
@@ -1275,10 +1285,12 @@ hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, h
     const Seg sg = cw->seg();
     // finite vector contents
     const size_t R = (size_t) rows;
+    const bool pre = cw->mg_on != 0;
     if ((e = fill_double(cw->r64, 1e-3, R, st)) || (e = fill_double(cw->p64, 0.0, R, st)) ||
         (e = fill_double(cw->s64, 0.0, R, st)) || (e = fill_double(cw->w64, 0.0, R, st)) ||
-        (e = fill_double(cw->y, 0.0, R, st)))
+        (e = fill_double(cw->y, 0.0, R, st)) || (e = fill_double(cw->u64, 1e-3, R, st)))
         return e;
+    hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, sg, cw->r64, cw->u32, cw->e32, cw->part);
     hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, sg, cw->r64, cw->r32, cw->e32, cw->part);
     hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, sg, cw->p64, cw->p32, cw->s32, cw->part);
     hipLaunchKernelGGL(k_inner_init, gr, bl, 0, st, sg, cw->p64, cw->w32, cw->e32, cw->part);
@@ -1300,11 +1312,11 @@ hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, h
         for (int rep = -1; rep < reps; rep++) {
             if (rep == 0) hipEventRecord(a, st);
             if (A.use16 && A.dim == 3)
-                time_launch<unsigned short>(cw, A, (const unsigned short*) A.dcol, kind, st);
+                time_launch<unsigned short>(cw, A, (const unsigned short*) A.dcol, kind, pre, st);
             else if (A.use16)
-                time_launch<short>(cw, A, A.dcol, kind, st);
+                time_launch<short>(cw, A, A.dcol, kind, pre, st);
             else
-                time_launch<int>(cw, A, A.col, kind, st);
+                time_launch<int>(cw, A, A.col, kind, pre, st);
         }
         hipEventRecord(b, st);
         if ((e = hipEventSynchronize(b))) break;
@@ -1318,12 +1330,13 @@ hipError_t cg_time_kernels(CgWork* cw, const Sell& A, int reps, long long nnz, h
     const long long Rl = rows;
     const long long ib = A.use16 ? 2 : 4;
     const long long ab = A.use16 && A.dim == 3 ? 8 : 0;  // the row's two anchors
-    // apply: matrix entries + r read + w written (+ d when weighted)
-    bytes[0] = nnz * (ib + 8) + Rl * (8 + 8 + 8 + ab);
-    // step: r, w, p, s, y read; p, s, y, r written
-    bytes[1] = Rl * 9 * 8;
-    bytes[2] = nnz * (ib + 4) + Rl * (4 + 4 + ab);
-    bytes[3] = Rl * 9 * 4;
+    const long long pu = pre ? 1 : 0;  // u = M r read besides r
+    // apply: matrix entries + r (+ u) read + w written (+ d when weighted)
+    bytes[0] = nnz * (ib + 8) + Rl * (8 * (3 + pu) + ab);
+    // step: r, w, (u,) p, s, y read; p, s, y, r written
+    bytes[1] = Rl * (9 + pu) * 8;
+    bytes[2] = nnz * (ib + 4) + Rl * (4 * (2 + pu) + ab);
+    bytes[3] = Rl * (9 + pu) * 4;
     return hipGetLastError();
 }
 

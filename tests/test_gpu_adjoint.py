@@ -32,7 +32,11 @@ def _assert_close(r, h):
 def test_adjoint_matches_central_differences_2d():
     r = gradcheck.check("dambreak2d_584", steps=20, interval=7, ndir=8, hs=(1e-5,), tol=EPS)
     _assert_close(r, 1e-5)
-    assert r["adjoint_stats"]["converged"] == 1 and r["adjoint_stats"]["rel_residual"] < 1e-13
+    # the adjoint solves run with the recorded run's tolerance, machine
+    # epsilon, which the true FP64 residual of A x cannot reach (converged
+    # reports the true residual against the tolerance): they stop at the
+    # rounding floor, far below what the FD check resolves
+    assert r["adjoint_stats"]["rel_residual"] < 1e-13
 
 
 def test_adjoint_through_collisions():

@@ -220,6 +220,17 @@ int mps_dump_bucket(mps_handle* h, int32_t* first, int32_t* last, int32_t* next)
  * list here (the reference keeps a stale one). */
 int mps_dump_neighbors(mps_handle* h, int32_t* offsets, int32_t* ids,
                        double* dists);
+/* mps_dump_neighbors without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The lists
+ * are staged in a static device array of the library (48 MiB per GPU, held
+ * until the process ends) instead of buffers allocated per call, so a
+ * system with more than 4,194,304 neighbour entries is refused (MPS_E_INVALID,
+ * the message says so; mps_dump_neighbors takes any size).  Every handle on
+ * one GPU shares that array: calls that overlap on one GPU (other host
+ * threads, other handles) race.  Same arguments and output as
+ * mps_dump_neighbors. */
+int mps_dump_neighbors_noalloc(mps_handle* h, int32_t* offsets, int32_t* ids, double* dists);
 /* Pressure system of the last step in id order: source term b (N), matrix
  * diagonal after the Dirichlet-connection doubling (N, 0 for non-Inner rows),
  * and the off-diagonal entries in CSR by id (row_ptr N+1, col, val: rows

@@ -33,6 +33,7 @@ EXPORTED = [
     "mps_create_ensemble", "mps_scene_count", "mps_ensemble_stats",
     "mps_adjoint_record", "mps_adjoint_backward", "mps_snapshot_begin", "mps_snapshot_end",
     "mps_checkpoint_size", "mps_checkpoint_save", "mps_checkpoint_load", "mps_debug_counters",
+    "mps_dump_neighbors_noalloc",
 ]
 
 
@@ -121,6 +122,7 @@ def lib():
     L.mps_checkpoint_save.argtypes = [vp, vp, ctypes.c_int64]
     L.mps_checkpoint_load.argtypes = [vp, vp, ctypes.c_int64]
     L.mps_debug_counters.argtypes = [vp, vp, i32]
+    L.mps_dump_neighbors_noalloc.argtypes = [vp, vp, vp, vp]
     for name in EXPORTED:
         if name not in ("mps_last_error", "mps_checkpoint_size"):
             getattr(L, name).restype = i32

@@ -12,8 +12,9 @@
 #include <string.h>
 
 #include <algorithm>
-#include <string>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "mps_internal.hpp"
@@ -113,6 +114,7 @@ struct mps_handle {
     double* pinned_d = nullptr;
     double* vmax = nullptr;
     std::string solve_msg;  // set when a pressure solve of the last step stopped above its tolerance
+    int dev = 0;            // device the handle was created on (its scratch copy)
     int want_raw = 0;
     int want_snap = 0;
     int have_system = 0;
@@ -436,6 +438,38 @@ static void free_all(mps_handle* h) {
     if (h->pinned_d) hipHostFree(h->pinned_d);
     if (h->cg) cg_destroy(h->cg);
     if (h->st) hipStreamDestroy(h->st);
+}
+
+// ---------------- allocation-free inspection: static device scratch ----------------
+//
+// mps_dump_neighbors_noalloc stages the neighbour lists in this file-scope
+// device array instead of buffers it allocates.  Each device holds its own
+// copy for the life of the process; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device (the array's name in
+// host code is a host-side stand-in, not a device address).
+constexpr size_t kNoAllocMaxEntries = (size_t) 1 << 22;  // 4,194,304 neighbour entries
+constexpr size_t kNoAllocScratchBytes = kNoAllocMaxEntries * (sizeof(int32_t) + sizeof(double));  // 48 MiB
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_dev[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Looks up the scratch array's address on the current device once per
+// device (the handle's set-up calls it with its device current).
+static hipError_t scratch_setup(int* dev_out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    *dev_out = dev;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[dev]) return hipSuccess;
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!ptr) return hipErrorInvalidSymbol;
+    g_scratch_dev[dev] = ptr;
+    return hipSuccess;
 }
 
 // ---------------- internal stage drivers ----------------
@@ -943,6 +977,12 @@ static int create_impl(int32_t nscene, const mps_settings* sset, const int32_t* 
         delete h;
         return MPS_E_HIP;
     }
+    if ((e = scratch_setup(&h->dev)) != hipSuccess) {
+        g_create_error = std::string("device scratch lookup (hipGetSymbolAddress): ") + hipGetErrorString(e);
+        hipStreamDestroy(h->st);
+        delete h;
+        return MPS_E_HIP;
+    }
     int rc = alloc_all(h);
     if (rc) {
         g_create_error = h->err;
@@ -1363,6 +1403,42 @@ int mps_dump_neighbors(mps_handle* h, int32_t* offsets, int32_t* ids, double* di
     hipFree(d_ids);
     hipFree(d_dist);
     if (e != hipSuccess) return hip_fail(h, e, "mps_dump_neighbors");
+    return MPS_OK;
+}
+
+int mps_dump_neighbors_noalloc(mps_handle* h, int32_t* offsets, int32_t* ids, double* dists) {
+    if (!h || !offsets) return MPS_E_INVALID;
+    void* const base = h->dev >= 0 && h->dev < kMaxDevices ? g_scratch_dev[h->dev] : nullptr;
+    if (!base) return fail(h, MPS_E_INVALID, "device scratch was not set up for this handle's device");
+    int rc = do_build(h);
+    if (rc) return rc;
+    if ((rc = drain(h))) return rc;
+    const int n = h->n;
+    GridView g = gview(h);
+    Hot c = cur(h);
+    int* cnt = h->w.coll_cnt;
+    int* off = h->w.coll_off;
+    HIPCHK(h, dump_neighbor_counts(h->dim, g, tab(h), n, c, cnt, h->s.re_max, h->st));
+    HIPCHK(h, scan_exclusive(cnt, off, n, h->bb.scan_tmp, h->st));
+    HIPCHK(h, hipMemcpyAsync(offsets, off, ((size_t) n + 1) * sizeof(int), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    if (!ids && !dists) return MPS_OK;
+    const size_t tot = (size_t) offsets[n];
+    if (tot > kNoAllocMaxEntries)
+        return fail(h, MPS_E_INVALID,
+                    "mps_dump_neighbors_noalloc: " + std::to_string(tot) + " neighbour entries exceed the " +
+                        std::to_string(kNoAllocScratchBytes >> 20) + " MiB device scratch (at most " +
+                        std::to_string(kNoAllocMaxEntries) + " entries); use mps_dump_neighbors");
+    // ids in the first 16 MiB, distances in the 32 MiB after them
+    int32_t* d_ids = reinterpret_cast<int32_t*>(base);
+    double* d_dist = reinterpret_cast<double*>(static_cast<char*>(base) + kNoAllocMaxEntries * sizeof(int32_t));
+    hipError_t e = dump_neighbor_lists(h->dim, g, tab(h), n, c, off, d_ids, d_dist, h->s.re_max, h->st);
+    if (e == hipSuccess && ids && tot)
+        e = hipMemcpyAsync(ids, d_ids, tot * sizeof(int32_t), hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess && dists && tot)
+        e = hipMemcpyAsync(dists, d_dist, tot * sizeof(double), hipMemcpyDeviceToHost, h->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    if (e != hipSuccess) return hip_fail(h, e, "mps_dump_neighbors_noalloc");
     return MPS_OK;
 }
 

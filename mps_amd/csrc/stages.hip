@@ -223,6 +223,103 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_coll_emit(GridView g0, Consts
 }
 
 constexpr int kResolveThreads = 1024;
+// Pair lists up to this long get their levels in parallel out of LDS; longer
+// ones on one thread.
+constexpr int kCollLds = 4096;
+
+// Levels of np <= kCollLds pairs, in parallel.  Pairs come grouped by the
+// id of their first (fluid) particle i, ascending, and a fluid partner j has
+// a smaller id than i, so it appears only in its own, earlier group and as
+// the partner of later pairs.  The latest earlier pair touching i is
+// therefore the pair before p when that has the same i; the latest earlier
+// pair touching j is the previous pair with partner j (found by sorting the
+// (j, p) keys), or else the last pair of j's own group.  level[p] = 1 + the
+// higher level of those two (0 for none); rounds of that update over every
+// pair reach the same values as the sequential assignment once no level
+// changes (each round settles at least one more link of every chain).
+__device__ void coll_levels_parallel(int np, const Hot& h, const int* __restrict__ off,
+                                     const int* __restrict__ pairs, int* __restrict__ level, int& lmax) {
+    __shared__ unsigned long long s_key[kCollLds];
+    __shared__ int s_prev[kCollLds];
+    __shared__ short s_lev[kCollLds];
+    __shared__ int s_changed, s_max;
+    int n2 = 1;
+    while (n2 < np) n2 <<= 1;
+    for (int p = threadIdx.x; p < n2; p += blockDim.x) {
+        unsigned long long k = ~0ull;
+        if (p < np) {
+            const int j = pairs[2 * p + 1];
+            if (h.type[j] == MPS_FLUID) k = ((unsigned long long) (unsigned) j << 32) | (unsigned) p;
+            s_prev[p] = -1;
+            s_lev[p] = 1;
+        }
+        s_key[p] = k;
+    }
+    __syncthreads();
+    for (int k = 2; k <= n2; k <<= 1) {  // bitonic sort, ascending
+        for (int jj = k >> 1; jj > 0; jj >>= 1) {
+            for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+                const int ixj = i ^ jj;
+                if (ixj > i) {
+                    const unsigned long long a = s_key[i], b = s_key[ixj];
+                    if ((a > b) == ((i & k) == 0)) {
+                        s_key[i] = b;
+                        s_key[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = threadIdx.x; k < n2; k += blockDim.x) {
+        const unsigned long long key = s_key[k];
+        if (key == ~0ull) continue;
+        const int j = (int) (key >> 32), p = (int) (key & 0xffffffffu);
+        int prev;
+        if (k > 0 && (int) (s_key[k - 1] >> 32) == j) {
+            prev = (int) (s_key[k - 1] & 0xffffffffu);
+        } else {
+            const int gid = h.id[j];
+            const int lo = off[gid], hi = off[gid + 1];
+            prev = hi > lo ? hi - 1 : -1;
+        }
+        s_prev[p] = prev;
+    }
+    __syncthreads();
+    volatile short* lev = s_lev;
+    for (;;) {
+        if (threadIdx.x == 0) s_changed = 0;
+        __syncthreads();
+        int ch = 0;
+        for (int p = threadIdx.x; p < np; p += blockDim.x) {
+            const int pi = p > 0 && pairs[2 * (p - 1)] == pairs[2 * p] ? p - 1 : -1;
+            const int pj = s_prev[p];
+            int L = 1;
+            if (pi >= 0 && lev[pi] + 1 > L) L = lev[pi] + 1;
+            if (pj >= 0 && lev[pj] + 1 > L) L = lev[pj] + 1;
+            if (L != lev[p]) {
+                lev[p] = (short) L;
+                ch = 1;
+            }
+        }
+        if (ch) s_changed = 1;
+        __syncthreads();
+        if (!s_changed) break;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) s_max = 0;
+    __syncthreads();
+    int mx = 0;
+    for (int p = threadIdx.x; p < np; p += blockDim.x) {
+        const int L = lev[p];
+        level[p] = L;
+        mx = L > mx ? L : mx;
+    }
+    atomicMax(&s_max, mx);
+    __syncthreads();
+    lmax = s_max;
+    __syncthreads();
+}
 
 template <bool MULTI>
 __global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts c0, SceneTab tab, int dim, Hot h,
@@ -235,7 +332,15 @@ __global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts 
     int np = off[n];
     if (np > cap) np = cap;
     if (np == 0) return;
-    if (threadIdx.x == 0) {
+    const bool par = np <= kCollLds;
+    if (par) {
+        int mx = 0;
+        coll_levels_parallel(np, h, off, pairs, level, mx);
+        if (threadIdx.x == 0) {
+            s_max = mx;
+            if (tp.nlev) *tp.nlev = mx;
+        }
+    } else if (threadIdx.x == 0) {
         int mx = 0;
         for (int p = 0; p < np; p++) {
             const int i = pairs[2 * p], j = pairs[2 * p + 1];
@@ -326,10 +431,11 @@ __global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts 
         __threadfence();
         __syncthreads();
     }
-    for (int p = threadIdx.x; p < np; p += blockDim.x) {
-        last[pairs[2 * p]] = 0;
-        last[pairs[2 * p + 1]] = 0;
-    }
+    if (!par)
+        for (int p = threadIdx.x; p < np; p += blockDim.x) {
+            last[pairs[2 * p]] = 0;
+            last[pairs[2 * p + 1]] = 0;
+        }
 }
 
 hipError_t launch_collision(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, Hot& h,

@@ -115,15 +115,17 @@ def _oracle_steps(settings, scene, steps, tol, duplicate):
     o.load_initial_state()
     t0 = time.perf_counter()
     its = []
-    for _ in range(steps):
+    for k in range(steps):
         o.step()
         its.append(o.cg_stats()[0])
+        print(f"[bench] cpu step {k + 1}/{steps} (N={scene.n}, tol {tol:g}{', solved twice' if duplicate else ''}): "
+              f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
     o.close()
     return dt, its
 
 
-def cpu_baseline(settings, scene, n_fluid, mid_state, full=False):
+def cpu_baseline(settings, scene, n_fluid, mid_state, full=False, workload="dambreak2d_2M"):
     """The FP64 CPU oracle (the reference's algorithm restated in C, OpenMP on
     the stage loops the reference parallelises and on the SpMV) on this
     host, timed on the same workload.  Default (a bounded sample): one step
@@ -148,7 +150,7 @@ def cpu_baseline(settings, scene, n_fluid, mid_state, full=False):
         t_tot += t_b
         n_steps += k
     out = dict(value=n_fluid * n_steps / t_tot, unit="particle-steps/s", cores=threads, kind="port",
-               sample=f"dambreak2d_2M (N={scene.n}), FP64 oracle, CG to {TOL:g} relative residual, single solve: "
+               sample=f"{workload} (N={scene.n}), FP64 oracle, CG to {TOL:g} relative residual, single solve: "
                       + "; ".join(parts) + f"; on {threads} threads of {info['model']}",
                host=info)
     s0, sc0, steps0 = configs.workload("dambreak2d_584")
@@ -162,7 +164,7 @@ def cpu_baseline(settings, scene, n_fluid, mid_state, full=False):
         t_r, it_r = _oracle_steps(settings, scene, k, eps, True)
         out["reference_faithful_2M"] = dict(
             value=n_fluid * k / t_r, unit="particle-steps/s", seconds_per_step=t_r / k,
-            sample=f"{k} steps of dambreak2d_2M from t=0, CG to machine epsilon solved twice per step, "
+            sample=f"{k} steps of {workload} from t=0, CG to machine epsilon solved twice per step, "
                    f"iterations per solve {it_r}")
     return out
 
@@ -399,7 +401,7 @@ def main():
         if mid is not None:
             mid = dict(mid, step=a.mid_state_step)
         print("[bench] cpu baseline (FP64 oracle on the host)", file=sys.stderr, flush=True)
-        res["cpu_baseline"] = cpu_baseline(settings, scene, n_fluid, mid, full=a.cpu_full)
+        res["cpu_baseline"] = cpu_baseline(settings, scene, n_fluid, mid, full=a.cpu_full, workload=a.workload)
     print(json.dumps(res))
 
 

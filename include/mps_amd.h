@@ -202,8 +202,13 @@ int mps_clear_debug_events(mps_handle* h);
  *   out[0] = times a solver's captured CG graphs were dropped because a
  *            pointer or launch shape they bake in changed,
  *   out[1] = CG vector reallocations,
- *   out[2] = pressure-matrix storage reallocations (entries or 3D anchors).
- * Writes min(cap, 3) values. */
+ *   out[2] = pressure-matrix storage reallocations (entries or 3D anchors),
+ *   out[3..6] = blocks of the LDS-staged neighbour walks on this device since
+ *            the process started, counted only for handles created with
+ *            MPS_AMD_STAGE_STATS=1: all of them, those that staged every
+ *            stencil row, those that staged some rows but not all, and of
+ *            those the ones that left a row out before a later staged row.
+ * Writes min(cap, 7) values. */
 int mps_debug_counters(const mps_handle* h, int64_t* out, int32_t cap);
 
 /* Bucket grid geometry (src/bucket.hpp:11-14). */

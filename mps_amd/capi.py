@@ -248,10 +248,13 @@ class Simulation:
         self._check(self.L.mps_set_solver(self.h, ctypes.byref(o)))
 
     def debug_counters(self):
-        """{graph_rebuilds, cg_reallocs, matrix_reallocs} (internal counters)."""
-        out = np.zeros(3, np.int64)
-        self._check(self.L.mps_debug_counters(self.h, _p(out), 3))
-        return dict(graph_rebuilds=int(out[0]), cg_reallocs=int(out[1]), matrix_reallocs=int(out[2]))
+        """Internal counters: graph rebuilds, CG and matrix reallocations, and
+        the LDS staging counts of the walks (MPS_AMD_STAGE_STATS=1 handles)."""
+        out = np.zeros(7, np.int64)
+        self._check(self.L.mps_debug_counters(self.h, _p(out), 7))
+        keys = ("graph_rebuilds", "cg_reallocs", "matrix_reallocs", "staged_blocks", "staged_full",
+                "staged_partial", "staged_partial_gap")
+        return {k: int(v) for k, v in zip(keys, out)}
 
     def solver_stats(self):
         s = SolverStats()

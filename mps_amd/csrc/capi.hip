@@ -241,6 +241,8 @@ static bool scene_grid(const mps_settings& s, int dim, GridView& g, std::string&
     // reads global memory); results do not depend on it
     g.stage_cap = 1 << 30;
     if (const char* e = getenv("MPS_AMD_STAGE_CAP")) g.stage_cap = std::max(atoi(e), 0);
+    g.stage_stats = 0;
+    if (const char* e = getenv("MPS_AMD_STAGE_STATS")) g.stage_stats = atoi(e) != 0;
     return true;
 }
 
@@ -1319,10 +1321,14 @@ int mps_debug_events(const mps_handle* hc, int64_t counts[3], int32_t* ghost_ids
 
 int mps_debug_counters(const mps_handle* h, int64_t* out, int32_t cap) {
     if (!h || (cap > 0 && !out)) return MPS_E_INVALID;
-    long long cg[2];
+    long long cg[2], stg[4] = {0, 0, 0, 0};
     cg_debug_counts(h->cg, cg);
-    const int64_t v[3] = {cg[0], cg[1], h->A.grows};
-    for (int k = 0; k < cap && k < 3; k++) out[k] = v[k];
+    if (cap > 3) {
+        const hipError_t e = stage_stats_read(stg);
+        if (e != hipSuccess) return hip_fail(const_cast<mps_handle*>(h), e, "mps_debug_counters");
+    }
+    const int64_t v[7] = {cg[0], cg[1], h->A.grows, stg[0], stg[1], stg[2], stg[3]};
+    for (int k = 0; k < cap && k < 7; k++) out[k] = v[k];
     return MPS_OK;
 }
 

@@ -34,6 +34,7 @@ struct GridView {
     int ghost_key;      // key of the ghost segment (after every scene's cells)
     const int* cs;      // cell starts over the whole key space (+ ghost segment end)
     int stage_cap;      // most slots a block copies into LDS (the kernels' LDS size caps it further)
+    int stage_stats;    // MPS_AMD_STAGE_STATS=1: count staged blocks by how much they staged (tests)
 };
 
 // Hot particle arrays in slot order (one of two ping-pong sets).
@@ -960,6 +961,11 @@ void cg_destroy(CgWork* cw);
 // Selects which record of last iteration counts sizes the CG chunks: 0 for
 // the step's own solves, 1 for the adjoint's.
 void cg_prediction_bank(CgWork* cw, int bank);
+// stages.hip: counts of the LDS staging of the walks since the process
+// started, kept when GridView::stage_stats is set: [0] blocks, [1] blocks
+// that staged every stencil row, [2] some but not all rows, [3] of those,
+// blocks that left a row out before a later staged one.
+hipError_t stage_stats_read(long long out[4]);
 hipError_t cg_reserve(CgWork* cw, int rows);
 // [0] times a solver's cached graphs were dropped, [1] CG vector reallocations
 void cg_debug_counts(const CgWork* cw, long long out[2]);

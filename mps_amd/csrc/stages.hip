@@ -76,16 +76,43 @@ constexpr int kStageCap = DIM == 2 ? 1536 : 3392;
 template <int DIM>
 using StageT = typename std::conditional<DIM == 2, double, float>::type;
 
+// Staging counts for tests (see stage_stats_read); only touched when the
+// grid's stage_stats flag is set, one atomic per block.
+static __device__ unsigned long long g_stage_stats[4];
+
+__device__ __forceinline__ void stage_stats_record(int mask, int full) {
+    atomicAdd(&g_stage_stats[0], 1ull);
+    if (mask == full) {
+        atomicAdd(&g_stage_stats[1], 1ull);
+    } else if (mask != 0) {
+        atomicAdd(&g_stage_stats[2], 1ull);
+        const int top = 31 - __clz(mask);           // highest staged row
+        if (mask != (1 << (top + 1)) - 1) atomicAdd(&g_stage_stats[3], 1ull);  // a row left out below it
+    }
+}
+
+hipError_t stage_stats_read(long long out[4]) {
+    unsigned long long v[4];
+    const hipError_t e = hipMemcpyFromSymbol(v, HIP_SYMBOL(g_stage_stats), sizeof v, 0, hipMemcpyDeviceToHost);
+    for (int k = 0; k < 4; k++) out[k] = e == hipSuccess ? (long long) v[k] : 0;
+    return e;
+}
+
 template <int DIM>
 __device__ __forceinline__ typename NbStageOf<DIM>::type nb_stage_any(const GridView& g, const SceneTab* tab,
                                                                       const Hot& h, int n, int* meta,
                                                                       StageT<DIM>* sx, StageT<DIM>* sy,
                                                                       StageT<DIM>* sz) {
     const int cap = kStageCap<DIM> < g.stage_cap ? kStageCap<DIM> : g.stage_cap;
-    if constexpr (DIM == 2)
-        return nb_stage<2>(g, tab, h, n, cap, meta, sx, sy, sz);
-    else
-        return nb_stage_f(g, tab, h, n, cap, meta, sx, sy, sz);
+    if constexpr (DIM == 2) {
+        const auto st = nb_stage<2>(g, tab, h, n, cap, meta, sx, sy, sz);
+        if (g.stage_stats && threadIdx.x == 0) stage_stats_record(st.ok ? 1 : 0, 1);
+        return st;
+    } else {
+        const auto st = nb_stage_f(g, tab, h, n, cap, meta, sx, sy, sz);
+        if (g.stage_stats && threadIdx.x == 0) stage_stats_record(st.ok, 0x1ff);
+        return st;
+    }
 }
 
 #define NB_STAGE(DIM, GRID, HOT, N)                                                                        \

@@ -4,10 +4,13 @@
 // reference's serial insertion.
 #pragma once
 
+#include <memory>
 #include <vector>
 
 #include "particle.hpp"
 #include "settings.hpp"
+
+struct mps_handle;
 
 class Bucket {
 public:
@@ -20,4 +23,11 @@ public:
     Bucket(const double& reMax, const Domain& domain, const int& particleSize);
     // Throws std::runtime_error if the GPU library fails.
     void storeParticles(std::vector<Particle>& particles);
+
+private:
+    // The device grid of the last call, reused while the particle count stays
+    // the same (later calls only upload positions and types).  Copies of a
+    // Bucket share it, so one Bucket and its copies are driven by one thread.
+    std::shared_ptr<mps_handle> dev;
+    int dev_n = -1;
 };

@@ -38,6 +38,9 @@ extern "C" {
 
 // Bucket(reMax, domain, N).storeParticles(particles of a CSV): dims4 gets
 // num/numX/numY/numZ; first/last (num) and next (N) are filled when not NULL.
+// The lists come from a second call: the first stores the particles shifted
+// by a third of a bucket along x, so the second reuses the device grid with
+// new positions.
 int mps_host_bucket_from_csv(const char* csv, double re_max, const double* dmin, const double* dmax, int* dims4,
                              int* first, int* last, int* next) {
     std::vector<Particle> ps;
@@ -51,6 +54,9 @@ int mps_host_bucket_from_csv(const char* csv, double re_max, const double* dmin,
     dims4[3] = b.numZ;
     if (!first) return 0;
     try {
+        std::vector<Particle> shifted = ps;
+        for (auto& p : shifted) p.position = Vec3(p.position.x() + re_max / 3, p.position.y(), p.position.z());
+        b.storeParticles(shifted);
         b.storeParticles(ps);
     } catch (const std::exception&) {
         return 2;

@@ -1745,9 +1745,10 @@ struct CkHeader {
 static_assert(sizeof(CkHeader) == 64, "checkpoint header is 64 bytes");
 constexpr int32_t kCkVersion = 2;
 
-// FNV-1a over each scene's particle count and settings (every field but the
-// reserved one): an image loads only into a handle whose scenes hold as many
-// particles each and step under the same settings.
+// FNV-1a over each scene's particle count and the settings the step depends
+// on (all but the finish time, the output interval and the CFL limit, so a
+// run can be resumed towards a later finish): an image loads only into a
+// handle whose scenes hold as many particles each and step the same way.
 static uint64_t ck_shape_hash(const mps_handle* h) {
     uint64_t x = 1469598103934665603ull;
     auto mix = [&x](const void* p, size_t len) {
@@ -1761,7 +1762,9 @@ static uint64_t ck_shape_hash(const mps_handle* h) {
         const SceneInfo& sc = h->scenes[k];
         const int32_t ints[4] = {sc.n, sc.s.dim, sc.s.qc_on, sc.s.hos_on};
         mix(ints, sizeof ints);
-        const char* d0 = (const char*) &sc.s.particle_distance;
+        mix(&sc.s.particle_distance, sizeof(double));
+        mix(&sc.s.dt, sizeof(double));
+        const char* d0 = (const char*) &sc.s.domain_min[0];
         mix(d0, (size_t) ((const char*) (&sc.s.restitution + 1) - d0));
     }
     return x;

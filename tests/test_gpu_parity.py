@@ -595,6 +595,8 @@ def test_checkpoint_resume_is_bit_identical(warm):
     s2 = dataclasses.replace(s, dt=s.dt * 0.5)
     with pytest.raises(capi.MpsError, match="settings"):
         capi.Simulation(s2, sc).restore(img)
+    # a later finish time (resuming towards a longer run) is accepted
+    capi.Simulation(dataclasses.replace(s, finish_time=2 * s.finish_time), sc).restore(img)
     # ensembles: the same totals with other per-scene counts are refused
     small = configs.lattice_2d(s.particle_distance, (1.0, 0.6), (0.0, 0.15), (0.0, 0.25), 2, 2)
     assert small.n != sc.n

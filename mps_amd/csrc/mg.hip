@@ -690,6 +690,7 @@ struct MgLevelDesc {
 };
 
 constexpr int kMgTailRows = 1024;       // levels this small may run inside the tail kernel
+constexpr size_t kMgAdditiveEntries = 22u << 20;  // fine entries from which single scenes take the additive cycle
 constexpr int kMgTailThreads = 1024;
 constexpr int kMgTailLds = 156 * 1024;  // dynamic LDS of the tail kernel
 
@@ -1024,6 +1025,21 @@ struct MgWork {
     double omega = 1.0, cf = 1.4;
     int sweeps = 1;
     bool user_params = false;        // set by mg_set_params / MPS_AMD_MG
+    // Fine level of the cycle.  Multiplicative (the symmetric V(1,1)): Jacobi
+    // from zero, restriction of the residual, coarse cycle, prolongation,
+    // Jacobi: u = x + omega (r - Ahat x), x = omega r + cf P c(R (r - omega Ahat r)),
+    // two fine SpMVs.  Additive: u = omega r + cf P c(R r), no fine SpMV;
+    // still symmetric positive definite (the sum of omega I and cf P C P^T
+    // with C the symmetric coarse cycle), so the CG stays a CG.  The additive
+    // cycle needs 1.5-1.9x the iterations and saves two fine SpMVs each.
+    // Measured (single dam breaks, steps 5-35): it is 2-6 % faster at the 2D
+    // 2M (25M matrix entries) and 7-9 % at the 3D 3.6M (85M), even at 3D
+    // 1.1-1.5M (25-35M), 4-20 % slower at 2D 0.8-1.6M (10-21M) and 3D 0.6M
+    // (15M), and 7-11 % slower on the 64-scene ensemble.  The default takes it for
+    // single scenes whose fine matrix holds kMgAdditiveEntries or more;
+    // MPS_AMD_MG_FINE=additive|multiplicative overrides.
+    int fine_mode = 0;               // 0: by size, 1: additive, 2: multiplicative
+    bool fine_additive = false;      // the current hierarchy's choice
     int tail_rows = kMgTailRows;
 };
 
@@ -1047,6 +1063,8 @@ MgWork* mg_create() {
         if (sscanf(v, "%lf,%lf,%d,%d", &om, &cf, &sw, &tr) >= 1) mg_set_params(m, om, cf, sw);
         if (tr > 0) m->tail_rows = tr;
     }
+    if (const char* v = getenv("MPS_AMD_MG_FINE"))
+        m->fine_mode = std::strcmp(v, "additive") == 0 ? 1 : (std::strcmp(v, "multiplicative") == 0 ? 2 : 0);
     return m;
 }
 
@@ -1141,6 +1159,17 @@ static hipError_t mg_allow_tail_lds() {
     return once;
 }
 
+// The fine level's form for this system (see MgWork::fine_mode).  A system
+// that took the additive cycle keeps it down to 7/8 of the threshold, so a
+// size near it does not switch the cycle, and with it the captured CG
+// graphs, from one step to the next.
+static bool mg_fine_additive(const MgWork* m, const Sell& A) {
+    if (m->fine_mode) return m->fine_mode == 1;
+    if (A.nseg != 1) return false;
+    const size_t lim = m->fine_additive ? kMgAdditiveEntries - kMgAdditiveEntries / 8 : kMgAdditiveEntries;
+    return A.entries >= lim;
+}
+
 template <typename T, typename IDX>
 static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, const T* fval, const IDX* fcol,
                              hipStream_t st) {
@@ -1151,8 +1180,13 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
     const int R = A.rows;
     m->W = dim == 2 ? 9 : 27;
     m->C = dim == 2 ? 4 : 8;
+    const bool add = mg_fine_additive(m, A);
+    if (add != m->fine_additive) m->gen++;
+    m->fine_additive = add;
     if (!m->user_params) {
-        const double cf = dim == 2 ? 1.4 : 1.3;
+        // the additive cycle prefers 1.2 in 3D (3.6M dam break: 1.2, 1.3, 1.5
+        // gave 20.9, 21.5, 25.7 iterations)
+        const double cf = dim == 2 ? 1.4 : (add ? 1.2 : 1.3);
         const int sw = dim == 2 ? 1 : 8;
         if (cf != m->cf || sw != m->sweeps || m->omega != 1.0) m->gen++;
         m->omega = 1.0;
@@ -1384,10 +1418,13 @@ static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval,
     int* nd = m->nd;
     T* res0 = (T*) m->res0;
     T* xx0 = (T*) m->xx0;
-    hipLaunchKernelGGL((k_mg_down0<T, IDX>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, A.slice_off, fcol, A.anc,
-                       fval, r, res0, om, done);
-    hipLaunchKernelGGL((k_mg_restrict1<T>), mg_grid(m->cap[1]), dim3(kBlock), 0, st, nd + 1, m->cst1, res0,
-                       (T*) m->b[1], done);
+    const bool add = m->fine_additive;
+    if (!add)
+        hipLaunchKernelGGL((k_mg_down0<T, IDX>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, A.slice_off, fcol,
+                           A.anc, fval, r, res0, om, done);
+    hipLaunchKernelGGL((k_mg_restrict1<T>), mg_grid(m->cap[1]), dim3(kBlock), 0, st, nd + 1, m->cst1,
+                       add ? r : (const T*) res0, (T*) m->b[1], done);
+    T* const xf = add ? u : xx0;  // the fine level's first step up
     for (int l = 1; l < lt; l++)
         hipLaunchKernelGGL((k_mg_down<T, W, C>), mg_grid(C * m->cap[l + 1]), dim3(kBlock), 0, st, nd + l + 1,
                            m->child[l + 1], m->cap[l + 1], m->col[l], (const T*) m->val[l], (const T*) m->diag[l],
@@ -1402,12 +1439,13 @@ static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval,
     if (lt >= 2)  // level 1's up pass also takes the fine level's first step up
         hipLaunchKernelGGL((k_mg_up<T, W, true>), mg_grid(m->cap[1]), dim3(kBlock), 0, st, nd + 1, m->col[1],
                            (const T*) m->val[1], (const T*) m->diag[1], (const T*) m->dinv[1], m->cap[1], m->agg[1],
-                           (const T*) m->b[1], (const T*) m->x[2], (T*) m->x[1], om, cf, done, m->cst1, r, xx0);
+                           (const T*) m->b[1], (const T*) m->x[2], (T*) m->x[1], om, cf, done, m->cst1, r, xf);
     else
         hipLaunchKernelGGL((k_mg_prolong0<T>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, r, m->agg[0],
-                           (const T*) m->x[1], xx0, om, cf, done);
-    hipLaunchKernelGGL((k_mg_up0<T, IDX>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, A.slice_off, fcol, A.anc,
-                       fval, r, xx0, u, om, done);
+                           (const T*) m->x[1], xf, om, cf, done);
+    if (!add)
+        hipLaunchKernelGGL((k_mg_up0<T, IDX>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, A.slice_off, fcol,
+                           A.anc, fval, r, xx0, u, om, done);
 }
 
 template <typename T, typename IDX>

@@ -882,6 +882,7 @@ struct Sell {
     double* sdiag;       // 1/sqrt(diag) per row
     double* dgl;         // diag per row
     size_t cap;          // entry capacity
+    size_t entries = 0;  // host: entries of the last assembly (slice widths x 64)
     // debug CSR (raw values by slot), filled when capture is on
     int* raw_off;        // n + 1 (by slot)
     int* raw_col;        // slots

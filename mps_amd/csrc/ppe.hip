@@ -259,6 +259,7 @@ hipError_t assemble_sell(int dim, const GridView& g, const SceneTab& tab, int n,
     const size_t entries = (size_t) hv[1] * 64;
     *host_rows = rows;
     *host_entries = entries;
+    A.entries = entries;
     A.rows = rows;
     A.rows_dev = w.row_of + n;
     A.nslices = (rows + 63) / 64;

@@ -42,6 +42,25 @@ def test_multigrid_solves_the_same_system(precision):
     assert np.abs(p_j - p_m).max() <= 1e-5 * np.abs(p_j).max()
 
 
+@pytest.mark.parametrize("precision", [capi.MPS_PREC_FP64, capi.MPS_PREC_MIXED])
+def test_both_fine_level_forms_solve_the_same_system(precision, monkeypatch):
+    """MPS_AMD_MG_FINE forces the V-cycle's fine level: additive (no fine
+    SpMV, the default for large single scenes) or multiplicative (the
+    symmetric V(1,1), the default below).  Both are SPD preconditioners of
+    the same CG: same tolerance met, same solution to what it leaves free;
+    the additive one takes more iterations for its cheaper cycle."""
+    jac, _ = _run("dambreak2d_35k", precision, capi.MPS_PRECOND_JACOBI, 4)
+    its = {}
+    for form in ("additive", "multiplicative"):
+        monkeypatch.setenv("MPS_AMD_MG_FINE", form)
+        st, stats = _run("dambreak2d_35k", precision, capi.MPS_PRECOND_MULTIGRID, 4)
+        for s in stats:
+            assert s["converged"] == 1 and s["rel_residual"] <= 1e-9, (form, s)
+        assert np.abs(st["p"] - jac["p"]).max() <= 1e-5 * np.abs(jac["p"]).max(), form
+        its[form] = sum(s["iterations"] for s in stats)
+    assert its["multiplicative"] < its["additive"] < 3 * its["multiplicative"], its
+
+
 def test_multigrid_is_deterministic():
     a, _ = _run("dambreak2d_35k", capi.MPS_PREC_MIXED, capi.MPS_PRECOND_MULTIGRID, 4)
     b, _ = _run("dambreak2d_35k", capi.MPS_PREC_MIXED, capi.MPS_PRECOND_MULTIGRID, 4)

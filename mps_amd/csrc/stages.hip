@@ -647,6 +647,43 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_diag_join(GridView g0, Consts
     nnz[s] = k;
 }
 
+// One round of pointer jumping on the connectivity forest before the
+// hooking: parent[s] = parent[parent[s]].  k_bc_source starts each particle
+// at its smallest neighbour, so chains run across the fluid (hundreds of
+// links at 2M particles) and the hooking pass's finds would chase them with
+// uncached loads; each round halves them.  Parents only move to ancestors,
+// so a jump that reads a parent another thread has already moved is still
+// an ancestor: the races change how far a round gets, never the components.
+__global__ void __launch_bounds__(kMaxTuneBlock) k_cc_jump(int n, int hops, int* __restrict__ parent) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int p0 = parent[s];
+    int p = p0;
+    for (int k = 0; k < hops; k++) {
+        const int q = parent[p];
+        if (q == p) break;
+        p = q;
+    }
+    if (p != p0) parent[s] = p;
+}
+
+struct CcJumps {
+    int rounds, hops;
+};
+static CcJumps cc_jumps() {
+    static const CcJumps r = [] {
+        CcJumps j{4, 7};
+        if (const char* e = getenv("MPS_AMD_CC_JUMPS")) {
+            j.hops = 1;
+            sscanf(e, "%d,%d", &j.rounds, &j.hops);
+            j.rounds = std::max(0, std::min(j.rounds, 64));
+            j.hops = std::max(1, std::min(j.hops, 64));
+        }
+        return j;
+    }();
+    return r;
+}
+
 __global__ void __launch_bounds__(kMaxTuneBlock) k_cc_finalize(int n, int* parent, const int8_t* __restrict__ bc_s,
                                                         int8_t* __restrict__ surf) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -681,6 +718,8 @@ hipError_t launch_diag_cc(int dim, const GridView& g, const SceneTab& tab, int n
                           StepScratch& w, Events& ev, hipStream_t st, int bs) {
     if (n == 0) return hipSuccess;
     const dim3 gr = grid_for(n, bs), bl(bs);
+    const CcJumps jp = cc_jumps();
+    for (int r = jp.rounds; r > 0; r--) hipLaunchKernelGGL(k_cc_jump, gr, bl, 0, st, n, jp.hops, w.parent);
     LAUNCH_DM(k_diag_join, gr, bl, st, g, c, tab, n, h, w.bc_s, w.diag_s, w.nnz_s, w.parent, w.nb);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;

@@ -503,12 +503,14 @@ __device__ __forceinline__ T mg_fine_row(const int* __restrict__ slice_pos, cons
     const size_t base = (size_t) s0 * 64 + lane;
     int lo, hi;
     sell_anchors<IDX>(anc, r, lo, hi);
+    constexpr bool NT = sizeof(T) == 8;  // the FP64 matrix: streamed
     int k = 0;
     for (; k + 4 <= w; k += 4) {
         const size_t i0 = base + (size_t) k * 64;
-        const int c0 = sell_col(col, i0, r, lo, hi), c1 = sell_col(col, i0 + 64, r, lo, hi);
-        const int c2 = sell_col(col, i0 + 128, r, lo, hi), c3 = sell_col(col, i0 + 192, r, lo, hi);
-        const T v0 = val[i0], v1 = val[i0 + 64], v2 = val[i0 + 128], v3 = val[i0 + 192];
+        const int c0 = sell_col<IDX, NT>(col, i0, r, lo, hi), c1 = sell_col<IDX, NT>(col, i0 + 64, r, lo, hi);
+        const int c2 = sell_col<IDX, NT>(col, i0 + 128, r, lo, hi), c3 = sell_col<IDX, NT>(col, i0 + 192, r, lo, hi);
+        const T v0 = ld_stream<NT>(val + i0), v1 = ld_stream<NT>(val + i0 + 64);
+        const T v2 = ld_stream<NT>(val + i0 + 128), v3 = ld_stream<NT>(val + i0 + 192);
         const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
         acc += v0 * x0;
         acc += v1 * x1;
@@ -517,7 +519,7 @@ __device__ __forceinline__ T mg_fine_row(const int* __restrict__ slice_pos, cons
     }
     for (; k < w; k++) {
         const size_t idx = base + (size_t) k * 64;
-        acc += val[idx] * x[sell_col(col, idx, r, lo, hi)];
+        acc += ld_stream<NT>(val + idx) * x[sell_col<IDX, NT>(col, idx, r, lo, hi)];
     }
     return acc;
 }

@@ -895,6 +895,16 @@ struct Sell {
 // needs, so every larger system reallocates its storage.
 bool tight_caps();
 
+// *p, as a nontemporal load when NT: for data a kernel streams once per
+// pass, so that it does not evict what the next kernels read again.
+template <bool NT, typename V>
+__device__ __forceinline__ V ld_stream(const V* p) {
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
 // Column of SELL entry idx of row `row`.  int: the column itself.  short
 // (2D): column - row.  unsigned short (3D): the row's neighbours lie in
 // three z layers of cells, each a compact run of rows, so the code holds
@@ -902,14 +912,18 @@ bool tight_caps();
 // + 8192 (14 bits) from the row itself in its own layer, else from the row's
 // anchor in that layer (lo, hi: the first row of the cell directly below
 // or above the row's cell, minus the row).
-template <typename IDX>
+// NT: the load is marked nontemporal (the FP64 matrix, which a pass
+// streams once and which does not fit the Infinity Cache beside the
+// solver's vectors, so it should not push them out).
+template <typename IDX, bool NT = false>
 __device__ __forceinline__ int sell_col(const IDX* __restrict__ c, size_t idx, int row, int lo, int hi) {
+    const IDX raw = ld_stream<NT>(c + idx);
     if constexpr (sizeof(IDX) == 4) {
-        return (int) c[idx];
+        return (int) raw;
     } else if constexpr (static_cast<IDX>(-1) < 0) {
-        return row + (int) c[idx];
+        return row + (int) raw;
     } else {
-        const unsigned v = c[idx];
+        const unsigned v = raw;
         const unsigned l = v >> 14;
         return row + ((int) (v & 16383u) - 8192) + (l == 1u ? lo : (l == 2u ? hi : 0));
     }

@@ -488,8 +488,13 @@ __device__ __forceinline__ int block_uniform_done(CgScalars* sc) {
 }
 
 // acc + sum_k val[k] x[col[k]] over the row's SELL entries, gathers issued in
-// batches of 4 for memory-level parallelism.
-template <typename T, typename IDX>
+// batches of 4 for memory-level parallelism.  NT: the matrix is read with
+// nontemporal loads (the FP64 solve's apply: its 10 bytes an entry do not
+// fit the Infinity Cache beside the solver's vectors, and streaming them
+// keeps the vectors and coarse levels cached: 2M FP64 step -6 %; the FP32
+// matrix does fit and is read normally, as is the FP64 residual's, where
+// nontemporal loads made the mixed step 2 % slower).
+template <typename T, typename IDX, bool NT = false>
 __device__ __forceinline__ T sell_row(const int* __restrict__ slice_pos, const IDX* __restrict__ col,
                                       const int* __restrict__ anc, const T* __restrict__ val,
                                       const T* __restrict__ x, int row, T acc) {
@@ -501,9 +506,10 @@ __device__ __forceinline__ T sell_row(const int* __restrict__ slice_pos, const I
     int k = 0;
     for (; k + 4 <= w; k += 4) {
         const size_t i0 = base + (size_t) k * 64;
-        const int c0 = sell_col(col, i0, row, lo, hi), c1 = sell_col(col, i0 + 64, row, lo, hi);
-        const int c2 = sell_col(col, i0 + 128, row, lo, hi), c3 = sell_col(col, i0 + 192, row, lo, hi);
-        const T v0 = val[i0], v1 = val[i0 + 64], v2 = val[i0 + 128], v3 = val[i0 + 192];
+        const int c0 = sell_col<IDX, NT>(col, i0, row, lo, hi), c1 = sell_col<IDX, NT>(col, i0 + 64, row, lo, hi);
+        const int c2 = sell_col<IDX, NT>(col, i0 + 128, row, lo, hi), c3 = sell_col<IDX, NT>(col, i0 + 192, row, lo, hi);
+        const T v0 = ld_stream<NT>(val + i0), v1 = ld_stream<NT>(val + i0 + 64);
+        const T v2 = ld_stream<NT>(val + i0 + 128), v3 = ld_stream<NT>(val + i0 + 192);
         const T x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
         acc += v0 * x0;
         acc += v1 * x1;
@@ -512,7 +518,7 @@ __device__ __forceinline__ T sell_row(const int* __restrict__ slice_pos, const I
     }
     for (; k < w; k++) {
         const size_t i0 = base + (size_t) k * 64;
-        acc += val[i0] * x[sell_col(col, i0, row, lo, hi)];
+        acc += ld_stream<NT>(val + i0) * x[sell_col<IDX, NT>(col, i0, row, lo, hi)];
     }
     return acc;
 }
@@ -537,7 +543,8 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_apply(Seg sg, const int* __rest
     for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kCgBlock) {
         const T rr = r[row];
         const T uu = PRE ? x[row] : rr;
-        const T acc = sell_row<T, IDX>(slice_pos, col, anc, val, x, row, uu);
+        // the FP64 solve streams its matrix (see sell_col)
+        const T acc = sell_row<T, IDX, sizeof(T) == 8>(slice_pos, col, anc, val, x, row, uu);
         w[row] = acc;
         const double r2 = (double) rr * (double) rr;
         g += (double) rr * (double) uu;

@@ -380,7 +380,10 @@ def main():
         "step_roofline": step_roofline(settings, scene, n_total, a.steps, fp64, mixed),
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_cg_apply<float, short, false, %s>" % ("true" if a.precond == "multigrid" else "false"),
+            # 16-bit column codes: offsets (2D), layer codes (3D)
+            "kernel": "k_cg_apply<float, %s, false, %s>" % (
+                "unsigned short" if scene is not None and settings.dim == 3 else "short",
+                "true" if a.precond == "multigrid" else "false"),
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

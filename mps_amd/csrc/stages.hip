@@ -506,6 +506,7 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g0, Consts
     if (t != MPS_GHOST) {
         const double rs = c.re_s, rp = c.re_p;
         const double re = rs > rp ? rs : rp;
+        const bool same = rs == rp;  // one weight serves both sums (the usual settings)
         double ns = 0.0, np = 0.0;
         const double zi = DIM == 3 ? h.pz[s] : 0.0;
         for_neighbors<DIM>(nbs, g, h.px, h.py, h.pz, h.key[s], s, h.px[s], h.py[s], zi, re,
@@ -516,9 +517,13 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g0, Consts
                                    nb.d[e] = dist;
                                }
                                cnt++;
-                               if (dist < rs) ns += wfun(dist, rs);
+                               double ws = 0.0;
+                               if (dist < rs) {
+                                   ws = wfun(dist, rs);
+                                   ns += ws;
+                               }
                                if (dist < rp) {
-                                   np += wfun(dist, rp);
+                                   np += same ? ws : wfun(dist, rp);
                                    if (q < m) m = q;
                                }
                            });

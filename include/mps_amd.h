@@ -207,8 +207,10 @@ int mps_clear_debug_events(mps_handle* h);
  *            the process started, counted only for handles created with
  *            MPS_AMD_STAGE_STATS=1: all of them, those that staged every
  *            stencil row, those that staged some rows but not all, and of
- *            those the ones that left a row out before a later staged row.
- * Writes min(cap, 7) values. */
+ *            those the ones that left a row out before a later staged row,
+ *   out[7] = CG iterations enqueued by the handle's solves, counting those
+ *            of a chunk that return at once after convergence.
+ * Writes min(cap, 8) values. */
 int mps_debug_counters(const mps_handle* h, int64_t* out, int32_t cap);
 
 /* Bucket grid geometry (src/bucket.hpp:11-14). */

@@ -784,6 +784,7 @@ struct CgWork {
     int pred_bank = 0;
     long long graph_rebuilds = 0;  // times a solver's cached graphs were dropped
     long long vec_reallocs = 0;    // times the CG vectors were reallocated
+    long long launched = 0;        // CG iterations enqueued, including those past convergence
     Seg seg() const { return Seg{nseg, blk_off, row_off}; }
 };
 
@@ -843,9 +844,10 @@ bool tight_caps() {
     return v && atoi(v) != 0;
 }
 
-void cg_debug_counts(const CgWork* cw, long long out[2]) {
+void cg_debug_counts(const CgWork* cw, long long out[3]) {
     out[0] = cw->graph_rebuilds;
     out[1] = cw->vec_reallocs;
+    out[2] = cw->launched;
 }
 
 hipError_t cg_reserve(CgWork* cw, int rows) {
@@ -984,6 +986,7 @@ static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* c
             }
         }
         launched += chunk;
+        cw->launched += chunk;
         if (o.use_graph) {
             int slot = -1;
             for (int j = 0; j < CgWork::kGraphSlots; j++)

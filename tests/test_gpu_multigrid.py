@@ -155,3 +155,26 @@ def test_interleaved_handles_match_solo_runs():
     for k in ("p", "pos", "vel"):
         assert (both2[k] == solo2[k]).all(), k
         assert (both3[k] == solo3[k]).all(), k
+
+
+def test_enqueued_iterations_cover_every_solve():
+    """mps_debug_counters' out[7]: CG iterations enqueued in chunks.  They
+    cover every iteration the solves report; what exceeds them is the tail
+    of a chunk that returns at once after convergence, bounded by the
+    chunk policy (a chunk of the last step's count, then 2, 4, 8, ...)."""
+    s, sc, _ = configs.workload("dambreak2d_35k")
+    for precision in (capi.MPS_PREC_FP64, capi.MPS_PREC_MIXED):
+        sim = capi.Simulation(s, sc)
+        sim.set_solver(precision=precision, tolerance=1e-9, warm_start=2, preconditioner=capi.MPS_PRECOND_MULTIGRID)
+        sim.load_initial_state()
+        start = sim.debug_counters()["cg_launched"]
+        its = solves = 0
+        for _ in range(6):
+            sim.step()
+            st = sim.solver_stats()
+            its += st["iterations"]
+            # mixed: one inner solve per refinement; FP64: one plus its restarts
+            solves += max(1, st["refinements"]) if precision == capi.MPS_PREC_MIXED else 1 + st["refinements"]
+        launched = sim.debug_counters()["cg_launched"] - start
+        sim.close()
+        assert its <= launched <= 2 * its + 4 * solves, (precision, its, launched, solves)

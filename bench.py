@@ -219,8 +219,8 @@ def step_roofline(settings, scene, n_total, steps, fp64, mixed):
         ach = tot / r["seconds"] / 1e9
         out[name] = {"bytes_per_step": tot / steps, "achieved": ach, "frac": ach / HBM_PEAK_GBS,
                      "bytes_per_particle_fixed": fixed, "bytes_per_particle_iteration": per_it}
-    out["note"] = ("per-unit bytes of a matrix-free step with FP32 state on the mixed path; this build stores the "
-                   "matrix and keeps the particle state in FP64 on both paths, so its real traffic is higher")
+    out["note"] = ("per-unit bytes of a matrix-free step with FP32 state on the mixed path; the solver here stores "
+                   "the matrix and keeps the particle state in FP64 on both paths, so its real traffic is higher")
     return out
 
 

@@ -111,7 +111,7 @@ def test_vtu_writer_reproduces_reference_file(tmp_path):
         assert mh == rh
         if "Name='Pressure'" in mh or "Name='Boundary Condition'" in mh:
             # the reference printed uninitialised memory here (1.061e-312, 2);
-            # this build defines pressure 0 and BC Ignored (0) at load
+            # the loader here defines pressure 0 and BC Ignored (0)
             assert set(mv) == {"0"}
             assert len(mv) == len(rv)
         else:

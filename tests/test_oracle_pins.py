@@ -36,8 +36,9 @@ from oracle_lib import Oracle
 
 RHO = 1000.0
 
-# SURVEY Appendix B closed forms are recomputed here from the integer shells
-# of the lattice offsets -4..4 (src/mps.cpp:132-163 loops over those offsets).
+# The lattice sums (n0, lambda) are recomputed here in closed form from the
+# integer shells of the lattice offsets -4..4 (src/mps.cpp:132-163 loops over
+# those offsets).
 
 
 def _shells(dim):

@@ -675,18 +675,17 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_cc_jump(int n, int hops, int*
 struct CcJumps {
     int rounds, hops;
 };
+// Read at every step (a getenv is nothing beside a step), so one process
+// can compare settings.
 static CcJumps cc_jumps() {
-    static const CcJumps r = [] {
-        CcJumps j{4, 7};
-        if (const char* e = getenv("MPS_AMD_CC_JUMPS")) {
-            j.hops = 1;
-            sscanf(e, "%d,%d", &j.rounds, &j.hops);
-            j.rounds = std::max(0, std::min(j.rounds, 64));
-            j.hops = std::max(1, std::min(j.hops, 64));
-        }
-        return j;
-    }();
-    return r;
+    CcJumps j{4, 7};
+    if (const char* e = getenv("MPS_AMD_CC_JUMPS")) {
+        j.hops = 1;
+        sscanf(e, "%d,%d", &j.rounds, &j.hops);
+        j.rounds = std::max(0, std::min(j.rounds, 64));
+        j.hops = std::max(1, std::min(j.hops, 64));
+    }
+    return j;
 }
 
 __global__ void __launch_bounds__(kMaxTuneBlock) k_cc_finalize(int n, int* parent, const int8_t* __restrict__ bc_s,

@@ -380,6 +380,19 @@ __device__ __forceinline__ NbStage<DIM> nb_stage(const GridView& g0, const Scene
     return NbStage<DIM>{meta[0], meta, sx, sy, sz};
 }
 
+// Pass mask of cnt <= 64 candidates: bit i is set when pass(i).  The
+// candidate index runs the same in every lane of a wavefront (only the
+// count differs), so it stays scalar and the mask is built in two 32-bit
+// halves: a select and an OR per candidate instead of 64-bit shifts.
+template <typename P>
+__device__ __forceinline__ uint64_t pass_mask(int cnt, P&& pass) {
+    uint32_t lo = 0, hi = 0;
+    const int n0 = cnt < 32 ? cnt : 32;
+    for (int i = 0; i < n0; i++) lo |= (pass(i) ? 1u : 0u) << i;
+    for (int i = 32; i < cnt; i++) hi |= (pass(i) ? 1u : 0u) << (i - 32);
+    return ((uint64_t) hi << 32) | lo;
+}
+
 // scan_cell on the LDS copy (candidate slot q sits at index q + off), in two
 // passes: the candidates' prefilter results go into a 64-bit mask first (64
 // candidates at a time), then the exact test, the square root and f run once
@@ -392,14 +405,14 @@ __device__ __forceinline__ void scan_cell_lds_mask(const double* __restrict__ sx
                                                    double xi, double yi, double zi, double re, double re2, F& f) {
     for (int c0 = b; c0 < e; c0 += 64) {
         const int c1 = e - c0 < 64 ? e : c0 + 64;
-        uint64_t m = 0;
-        for (int q = c0; q < c1; q++) {
+        uint64_t m = pass_mask(c1 - c0, [&](int i) {
+            const int q = c0 + i;
             const double dx = sx[q + off] - xi;
             const double dy = sy[q + off] - yi;
             const double dz = DIM == 3 ? sz[q + off] - zi : 0.0;
             const double d2f = DIM == 3 ? fma(dz, dz, fma(dy, dy, dx * dx)) : fma(dy, dy, dx * dx);
-            if (d2f <= re2) m |= 1ull << (q - c0);
-        }
+            return d2f <= re2;
+        });
         while (m) {
             const int q = c0 + __builtin_ctzll(m);
             m &= m - 1;
@@ -616,11 +629,13 @@ __device__ __forceinline__ void for_neighbors(const NbStageF& st, const GridView
                 const int off = st.meta[1 + r];
                 for (int c0 = b; c0 < e; c0 += 64) {
                     const int c1 = e - c0 < 64 ? e : c0 + 64;
-                    uint64_t m = 0;
-                    for (int q = c0; q < c1; q++) {
+                    // the FP32 squared distance by fused multiply-adds: within
+                    // the same few ulp the slack of rf covers
+                    uint64_t m = pass_mask(c1 - c0, [&](int i) {
+                        const int q = c0 + i;
                         const float dxf = st.x[q + off] - xf, dyf = st.y[q + off] - yf, dzf = st.z[q + off] - zf;
-                        if (dxf * dxf + dyf * dyf + dzf * dzf <= rf2) m |= 1ull << (q - c0);
-                    }
+                        return fmaf(dzf, dzf, fmaf(dyf, dyf, dxf * dxf)) <= rf2;
+                    });
                     while (m) {
                         int qq[kNbBatch];
                         int np = 0;

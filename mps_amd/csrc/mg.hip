@@ -161,8 +161,8 @@ __device__ __forceinline__ int mg_stencil_of_cells(int k0, int k1, MgDims d) {
 }
 
 // Galerkin level 1 from the fine SELL matrix (unit diagonal not stored;
-// padding entries point at their own row with value 0).  Aggregate K is the
-// bucket cell coord1[K] of scene scene1[K].  A block takes A consecutive
+// padding entries point at their own row with value 0).  Aggregate K is a
+// bucket cell of scene scene1[K]; rkey holds each fine row's cell key.  A block takes A consecutive
 // aggregates, whose rows are one contiguous range, in chunks of kAsm1Rows
 // rows: one thread per row sums its entries into the row's stencil bins
 // (LDS, entry order), then one thread per (aggregate, bin) adds the rows'
@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble1(int n1, const int* _
                                                             const IDX* __restrict__ col, const int* __restrict__ anc,
                                                             const T* __restrict__ val,
                                                             const int* __restrict__ agg0,
-                                                            const int* __restrict__ coord1,
+                                                            const int* __restrict__ rkey,
                                                             const int* __restrict__ scene1,
                                                             const MgDims* __restrict__ dims, int cap1,
                                                             int* __restrict__ col1, T* __restrict__ val1,
@@ -186,7 +186,6 @@ __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble1(int n1, const int* _
     __shared__ double s_acc[A][W + 1];
     __shared__ int s_nbr[A][W];                 // the aggregate at each stencil place (-1: none)
     __shared__ int s_cst[A + 1];
-    __shared__ int s_key[A];
     __shared__ MgDims s_dim[A];
     const int tid = threadIdx.x;
     const int K0 = blockIdx.x * A;
@@ -194,7 +193,6 @@ __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble1(int n1, const int* _
     if (na <= 0) return;  // block-uniform
     if (tid <= na) s_cst[tid] = cst1[K0 + tid];
     if (tid < na) {
-        s_key[tid] = coord1[K0 + tid];
         s_dim[tid] = dims[scene1[K0 + tid]];
     }
     for (int i = tid; i < A * W; i += kAsm1Rows) s_nbr[i / W][i % W] = -1;
@@ -213,16 +211,18 @@ __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble1(int n1, const int* _
                 else
                     hi = mid - 1;
             }
-            const int K = K0 + a, key = s_key[a];
+            const int K = K0 + a, rk = rkey[r];
             const MgDims d = s_dim[a];
             const int sl = r >> 6, lane = r & 63;
             const int sp = slice_pos[sl], w = slice_pos[sl + 1] - sp;
             const size_t base = (size_t) sp * 64 + lane;
             int alo, ahi;
             sell_anchors<IDX>(anc, r, alo, ahi);
-            // four entries per round: columns and values, then their
-            // aggregates, then the aggregates' cells, each round's loads
-            // issued together (entries past w re-read the last and are dropped)
+            // four entries per round: columns and values, then the columns'
+            // aggregates and cell keys (a row's cell is its aggregate's, so
+            // the key difference to the row's own gives the stencil place),
+            // each round's loads issued together (entries past w re-read the
+            // last and are dropped)
             for (int k0 = 0; k0 < w; k0 += 4) {
                 int K2[4], cd[4];
                 double v[4];
@@ -233,17 +233,18 @@ __global__ void __launch_bounds__(kAsm1Rows) k_mg_assemble1(int n1, const int* _
                     v[q] = (double) val[idx];
                 }
 #pragma unroll
-                for (int q = 0; q < 4; q++) K2[q] = agg0[K2[q]];
-#pragma unroll
-                for (int q = 0; q < 4; q++) cd[q] = coord1[K2[q]];
+                for (int q = 0; q < 4; q++) {
+                    cd[q] = rkey[K2[q]];
+                    K2[q] = agg0[K2[q]];
+                }
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     if (k0 + q >= w) break;
-                    if (K2[q] == K) {
+                    if (cd[q] == rk) {  // the row's own cell: its own aggregate
                         s_bin[tid][W] += v[q];
                         continue;
                     }
-                    const int j = mg_stencil_of_dkey<W>(cd[q] - key, d);
+                    const int j = mg_stencil_of_dkey<W>(cd[q] - rk, d);
                     if (j < 0) {
                         atomicOr(err, 1);
                         continue;
@@ -1255,11 +1256,11 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
     const dim3 ab(kAsm1Rows);
     if (dim == 2)
         hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 9, 48>), dim3((n1 + 47) / 48), dim3(kAsm1Rows), 0, st, n1, m->cst1,
-                           A.slice_off, fcol, A.anc, fval, m->agg[0], m->coord[1], m->scene[1], m->d_sdims[1],
+                           A.slice_off, fcol, A.anc, fval, m->agg[0], m->rkey, m->scene[1], m->d_sdims[1],
                            m->cap[1], m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1], m->err);
     else
         hipLaunchKernelGGL((k_mg_assemble1<T, IDX, 27, 16>), dim3((n1 + 15) / 16), dim3(kAsm1Rows), 0, st, n1,
-                           m->cst1, A.slice_off, fcol, A.anc, fval, m->agg[0], m->coord[1], m->scene[1],
+                           m->cst1, A.slice_off, fcol, A.anc, fval, m->agg[0], m->rkey, m->scene[1],
                            m->d_sdims[1], m->cap[1], m->col[1], (T*) m->val[1], (T*) m->diag[1], (T*) m->dinv[1],
                            m->err);
     // Coarse levels 2..: level l + 1 merges 2x2 (2x2x2) cells of level l.

@@ -1240,8 +1240,10 @@ int mps_upload(mps_handle* h, const int8_t* type, const double* pos, const doubl
     if (N == 0) return MPS_OK;
     // host copies of slot_of and the current hot arrays, rewritten in slot order
     std::vector<int32_t> slot(N);
-    HIPCHK(h, hipMemcpy(slot.data(), h->bb.slot_of, N * sizeof(int32_t), hipMemcpyDeviceToHost));
     Hot c = cur(h);
+    HIPCHK(h, slot_of_ids(h->n, c.id, h->bb.slot_of, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    HIPCHK(h, hipMemcpy(slot.data(), h->bb.slot_of, N * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (type) {
         for (size_t i = 0; i < N; i++)
             if (type[i] < -1 || type[i] > 2) return fail(h, MPS_E_INVALID, "particle type out of range");

@@ -294,8 +294,7 @@ template <int DIM>
 __global__ void __launch_bounds__(kMaxTuneBlock) k_rank_permute(int ghost_key, const int* __restrict__ cs, int n,
                                                                 Hot cur, Hot nxt,
                                                          const int* __restrict__ bin_id,
-                                                         const int* __restrict__ dst,
-                                                         int* __restrict__ slot_of) {
+                                                         const int* __restrict__ dst) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int key = cur.key[s];
@@ -320,7 +319,19 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_rank_permute(int ghost_key, c
     nxt.id[t] = myid;
     nxt.type[t] = cur.type[s];
     nxt.key[t] = key;
-    slot_of[myid] = t;
+}
+
+// id -> slot from the slots' ids (the builds do not keep this table: only
+// mps_upload reads it).
+__global__ void __launch_bounds__(kBlock) k_slot_of_ids(int n, const int* __restrict__ id, int* __restrict__ slot_of) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) slot_of[id[s]] = s;
+}
+
+hipError_t slot_of_ids(int n, const int* id, int* slot_of, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_slot_of_ids, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, n, id, slot_of);
+    return hipGetLastError();
 }
 
 hipError_t build_grid(int dim, const GridView& g, const SceneTab& tab, int n, const Hot& cur, Hot& nxt,
@@ -344,11 +355,9 @@ hipError_t build_grid(int dim, const GridView& g, const SceneTab& tab, int n, co
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_bin, grid, blk, 0, st, n, cur, b.cnt, b.cs, b.bin_id, b.dst);
     if (dim == 2)
-        hipLaunchKernelGGL(k_rank_permute<2>, grid, blk, 0, st, g.ghost_key, b.cs, n, cur, nxt, b.bin_id, b.dst,
-                           b.slot_of);
+        hipLaunchKernelGGL(k_rank_permute<2>, grid, blk, 0, st, g.ghost_key, b.cs, n, cur, nxt, b.bin_id, b.dst);
     else
-        hipLaunchKernelGGL(k_rank_permute<3>, grid, blk, 0, st, g.ghost_key, b.cs, n, cur, nxt, b.bin_id, b.dst,
-                           b.slot_of);
+        hipLaunchKernelGGL(k_rank_permute<3>, grid, blk, 0, st, g.ghost_key, b.cs, n, cur, nxt, b.bin_id, b.dst);
     return hipGetLastError();
 }
 

@@ -8,8 +8,7 @@
 // (src/bucket.cpp:36-50 appends to the tail in id order).  Ghost particles sit
 // after the last cell.  Cold per-particle fields (pressure, minimum pressure,
 // number density ratio, boundary condition, warm-start solution, the stale
-// number density of ghosts) stay in id order and are reached through id[] /
-// slot_of[].
+// number density of ghosts) stay in id order and are reached through id[].
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -795,8 +794,9 @@ struct BuildBufs {
     int* bin_id;     // n: ids binned per cell (arbitrary order within a cell)
     int* dst;        // n: binned position of each slot
     int* scan_tmp;
-    int* slot_of;    // n: id -> slot
+    int* slot_of;    // n: id -> slot, filled on demand by slot_of_ids
 };
+hipError_t slot_of_ids(int n, const int* id, int* slot_of, hipStream_t st);
 hipError_t build_grid(int dim, const GridView& g, const SceneTab& tab, int n, const Hot& cur, Hot& nxt,
                       const BuildBufs& b, int* err, hipStream_t st, int bs);
 hipError_t dump_neighbor_counts(int dim, const GridView& g, const SceneTab& tab, int n, const Hot& h,

@@ -303,7 +303,15 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_rank_permute(int ghost_key, c
     if (key != ghost_key) {
         const int b = cs[key], e = cs[key + 1];
         int r = 0;
-        for (int q = b; q < e; q++) r += bin_id[q] < myid;
+        // eight loads in flight per round (indices past the cell re-read
+        // its last entry and are not counted)
+        for (int q = b; q < e; q += 8) {
+            int v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = bin_id[q + k < e ? q + k : e - 1];
+#pragma unroll
+            for (int k = 0; k < 8; k++) r += (q + k < e && v[k] < myid) ? 1 : 0;
+        }
         t = b + r;
     } else {
         t = dst[s];  // ghosts: order inside the ghost segment is immaterial

@@ -427,7 +427,7 @@ __device__ __forceinline__ void scan_cell_lds_mask(const double* __restrict__ sx
 }
 
 // for_neighbors reading the block's LDS copy when it has one.
-template <int DIM, bool GROUP = true, typename F>  // GROUP: see the 3D overload
+template <int DIM, int GROUP = kNbBatch, typename F>  // GROUP: see the 3D overload
 __device__ __forceinline__ void for_neighbors(const NbStage<DIM>& st, const GridView& g,
                                               const double* __restrict__ px, const double* __restrict__ py,
                                               const double* __restrict__ pz, int key, int s, double xi, double yi,
@@ -640,7 +640,7 @@ __device__ __forceinline__ NbStageF nb_stage_f(const GridView& g0, const SceneTa
     return NbStageF{ok, meta, sx, sy, sz, ox, oy, oz, __int_as_float(meta[1 + 3 * NR + 1])};
 }
 
-template <int DIM, bool GROUP = true, typename F>
+template <int DIM, int GROUP = kNbBatch, typename F>
 __device__ __forceinline__ void for_neighbors(const NbStageF& st, const GridView& g, const double* __restrict__ px,
                                               const double* __restrict__ py, const double* __restrict__ pz, int key,
                                               int s, double xi, double yi, double zi, double re, F&& f) {
@@ -667,91 +667,21 @@ __device__ __forceinline__ void for_neighbors(const NbStageF& st, const GridView
     cell_gaps(yi, g.ymin, g.len, iy, gyl, gyh);
     cell_gaps(zi, g.zmin, g.len, iz, gzl, gzh);
     const double rr = re * re;
-    if constexpr (!GROUP) {
-        // cell by cell, as the grouped loop below does for a (jx, jy) it
-        // cannot group
-        for (int jx = ix - 1; jx <= ix + 1; jx++) {
-            if (jx < 0 || jx >= g.nx) continue;
-            const double gx = cell_gap(jx - ix + 1, gxl, gxh);
-            for (int jy = iy - 1; jy <= iy + 1; jy++) {
-                if (jy < 0 || jy >= g.ny) continue;
-                const double gy = cell_gap(jy - iy + 1, gyl, gyh);
-                for (int jz = iz - 1; jz <= iz + 1; jz++) {
-                    if (jz < 0 || jz >= g.nz) continue;
-                    const double gz = cell_gap(jz - iz + 1, gzl, gzh);
-                    if (gx * gx + gy * gy + gz * gz > rr) continue;
-                    const int k = jx + jy * g.nx + jz * g.nx * g.ny;
-                    const int r = (jy - iy + 1) + 3 * (jz - iz + 1);
-                    const int b = cs[k], e = cs[k + 1];
-                    if (!((st.ok >> r) & 1)) {  // row left in global memory
-                        scan_cell<3>(px, py, pz, b, e, s, xi, yi, zi, re, re2, f);
-                        continue;
-                    }
-                    const int off = st.meta[1 + r];
-                    for (int c0 = b; c0 < e; c0 += 64) {
-                        const int c1 = e - c0 < 64 ? e : c0 + 64;
-                        // the FP32 squared distance by fused multiply-adds: within
-                        // the same few ulp the slack of rf covers
-                        uint64_t m = pass_mask(c1 - c0, [&](int i) {
-                            const int q = c0 + i;
-                            const float dxf = st.x[q + off] - xf, dyf = st.y[q + off] - yf, dzf = st.z[q + off] - zf;
-                            return fmaf(dzf, dzf, fmaf(dyf, dyf, dxf * dxf)) <= rf2;
-                        });
-                        while (m) {
-                            int qq[kNbBatch];
-                            int np = 0;
-#pragma unroll
-                            for (int i = 0; i < kNbBatch; i++) {  // past the last pass: the first again
-                                if (i == 0 || m) {
-                                    qq[i] = c0 + __builtin_ctzll(m);
-                                    m &= m - 1;
-                                    np = i + 1;
-                                } else {
-                                    qq[i] = qq[0];
-                                }
-                            }
-                            double x[kNbBatch], y[kNbBatch], z[kNbBatch];
-#pragma unroll
-                            for (int i = 0; i < kNbBatch; i++) {
-                                const unsigned boff = (unsigned) qq[i] * 8u;
-                                x[i] = *reinterpret_cast<const double*>(bx + boff);
-                                y[i] = *reinterpret_cast<const double*>(by + boff);
-                                z[i] = *reinterpret_cast<const double*>(bz + boff);
-                            }
-#pragma unroll
-                            for (int i = 0; i < kNbBatch; i++) {
-                                if (i >= np) break;
-                                const int q = qq[i];
-                                const double dx = x[i] - xi;
-                                const double dy = y[i] - yi;
-                                const double dz = z[i] - zi;
-                                const double d2 = (dx * dx + dy * dy) + dz * dz;
-                                if (d2 > re2) continue;
-                                const double dist = sqrt(d2);
-                                if (q != s && dist < re) f(q, dist, dx, dy, dz);
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        return;
-    }
     // One pass loop per (jx, jy) over its three jz cells: the cells' FP32
     // pass masks (64 candidates a cell at most) are taken first, then the
     // passes come out of them in the reference's order (jz, then slot) in
     // batches of kNbBatch, so a wavefront runs the FP64 tests as often as
     // its busiest lane has passes in the three cells together, not the sum
     // of the busiest lane's in each.  A (jx, jy) with a cell of more than 64
-    // candidates, or one left in global memory, goes cell by cell; so does
-    // every (jx, jy) when GROUP is false (the three words cost registers a
-    // kernel near the 128 of four waves per SIMD cannot spare).
+    // candidates, or one left in global memory, goes cell by cell.  GROUP
+    // is the batch size (a kernel near the 128 registers of four waves per
+    // SIMD takes a smaller batch to make room for the three words).
     auto batch_loop = [&](uint64_t& w0, uint64_t& w1, uint64_t& w2, int b0, int b1, int b2) {
         while (w0 | w1 | w2) {
-            int qq[kNbBatch];
+            int qq[GROUP];
             int np = 0;
 #pragma unroll
-            for (int i = 0; i < kNbBatch; i++) {  // past the last pass: the first again
+            for (int i = 0; i < GROUP; i++) {  // past the last pass: the first again
                 const int d = w0 ? 0 : (w1 ? 1 : (w2 ? 2 : 3));
                 if (d == 3) {
                     qq[i] = qq[0];
@@ -769,16 +699,16 @@ __device__ __forceinline__ void for_neighbors(const NbStageF& st, const GridView
                 qq[i] = q;
                 np = i + 1;
             }
-            double x[kNbBatch], y[kNbBatch], z[kNbBatch];
+            double x[GROUP], y[GROUP], z[GROUP];
 #pragma unroll
-            for (int i = 0; i < kNbBatch; i++) {
+            for (int i = 0; i < GROUP; i++) {
                 const unsigned boff = (unsigned) qq[i] * 8u;
                 x[i] = *reinterpret_cast<const double*>(bx + boff);
                 y[i] = *reinterpret_cast<const double*>(by + boff);
                 z[i] = *reinterpret_cast<const double*>(bz + boff);
             }
 #pragma unroll
-            for (int i = 0; i < kNbBatch; i++) {
+            for (int i = 0; i < GROUP; i++) {
                 if (i >= np) break;
                 const int q = qq[i];
                 const double dx = x[i] - xi;
@@ -821,7 +751,7 @@ __device__ __forceinline__ void for_neighbors(const NbStageF& st, const GridView
                 co[d] = st.meta[1 + r];
                 grouped = grouped && (!in || (((st.ok >> r) & 1) && ce[d] - cb[d] <= 64));
             }
-            if (GROUP && grouped) {
+            if (grouped) {
                 uint64_t w0 = fmask(cb[0], ce[0] - cb[0], co[0]);
                 uint64_t w1 = fmask(cb[1], ce[1] - cb[1], co[1]);
                 uint64_t w2 = fmask(cb[2], ce[2] - cb[2], co[2]);

@@ -372,6 +372,7 @@ constexpr int kMaxBlocks = 1024;  // per segment
 // Threads per block of the per-iteration kernels (apply, step): 1024 blocks
 // of 512 fill every SIMD with 8 waves, twice the loads in flight of 256.
 constexpr int kCgBlock = 512;
+constexpr int kStepPre = 4;  // rows a thread of k_cg_step loads ahead of the partials' reduction
 // Below this many rows the V-cycle's setup and launches cost more than the
 // iterations it saves (measured: 584-particle dam break), so small systems
 // keep the Jacobi scaling alone.
@@ -566,6 +567,23 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_step(Seg sg, T* __restrict__ y,
     const BlockRows br = block_rows(sg);
     CgScalars* sc = scs + br.seg;
     if (block_uniform_done(sc)) return;
+    // the first kStepPre rows' vectors are loaded before the partials are
+    // reduced, so the loads overlap the reduction's round trips (a block has
+    // about 4 rows a thread at 2M)
+    T vr[kStepPre], vw[kStepPre], vu[kStepPre], vp[kStepPre], vs[kStepPre], vy[kStepPre];
+#pragma unroll
+    for (int k = 0; k < kStepPre; k++) {
+        const int row = br.r0 + (int) threadIdx.x + k * kCgBlock;
+        vr[k] = vw[k] = vu[k] = vp[k] = vs[k] = vy[k] = (T) 0;
+        if (row < br.r1) {
+            vr[k] = r[row];
+            vw[k] = w[row];
+            vu[k] = PRE ? u[row] : vr[k];
+            vp[k] = p[row];
+            vs[k] = s[row];
+            vy[k] = y[row];
+        }
+    }
     double sums[3];
     reduce_partials<3>(part + (size_t) 3 * br.b0, br.nb, sums, red);
     const double g = sums[0], dd = sums[1], rho = sums[2];
@@ -599,7 +617,19 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_step(Seg sg, T* __restrict__ y,
         alpha = (T) a;
         beta = (T) b;
     }
-    for (int row = br.r0 + (int) threadIdx.x; row < br.r1; row += kCgBlock) {
+#pragma unroll
+    for (int k = 0; k < kStepPre; k++) {
+        const int row = br.r0 + (int) threadIdx.x + k * kCgBlock;
+        if (row < br.r1) {
+            const T pp = first ? vu[k] : vu[k] + beta * vp[k];
+            const T ss = first ? vw[k] : vw[k] + beta * vs[k];
+            p[row] = pp;
+            s[row] = ss;
+            y[row] = vy[k] + alpha * pp;
+            r[row] = vr[k] - alpha * ss;
+        }
+    }
+    for (int row = br.r0 + (int) threadIdx.x + kStepPre * kCgBlock; row < br.r1; row += kCgBlock) {
         const T rr = r[row], ww = w[row];
         const T uu = PRE ? u[row] : rr;
         const T pp = first ? uu : uu + beta * p[row];

@@ -630,7 +630,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_up(const int* __restrict__ nrow, 
                                                   const T* __restrict__ b, const T* __restrict__ xc,
                                                   T* __restrict__ x, T omega, T cf, MgDone done,
                                                   const int* __restrict__ cst1, const T* __restrict__ bf,
-                                                  T* __restrict__ xxf) {
+                                                  T* __restrict__ xxf, const int* __restrict__ aggf) {
 #pragma clang fp contract(fast)
     const int K = blockIdx.x * blockDim.x + threadIdx.x;
     const int Kr = K < cap ? K : 0;
@@ -644,38 +644,41 @@ __global__ void __launch_bounds__(kBlock) k_mg_up(const int* __restrict__ nrow, 
     }
     const T bK = b[Kr], dK = dinv[Kr], gK = diag[Kr];
     const int aK = agg[Kr];
-    int f0 = 0, f1 = 0;
-    if (PRO) {
-        f0 = cst1[Kr];
-        f1 = cst1[Kr + 1];
-    }
-    if (mg_done_wave(done) || K >= nr) return;
+    // the done flags are the whole launch's, so every block takes the same branch
+    if (mg_done_wave(done)) return;
+    T xK = (T) 0;
+    if (K < nr) {
 #pragma unroll
-    for (int j = 0; j < W; j++) {
-        const int cc = c[j] >= 0 ? c[j] : K;
-        bb[j] = b[cc];
-        dd[j] = dinv[cc];
-        ag[j] = agg[cc];
-    }
-    const T xcK = xc[aK];
-#pragma unroll
-    for (int j = 0; j < W; j++) xv[j] = xc[ag[j]];
-    const T xx = omega * bK * dK + cf * xcK;
-    T acc = gK * xx;
-#pragma unroll
-    for (int j = 0; j < W; j++)
-        if (c[j] >= 0) acc += v[j] * (omega * bb[j] * dd[j] + cf * xv[j]);
-    const T xK = xx + omega * dK * (bK - acc);
-    x[K] = xK;
-    if (PRO)
-        for (int r = f0; r < f1; r += 4) {  // four fine rows per round
-            T bv[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) bv[q] = bf[r + q < f1 ? r + q : f1 - 1];
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (r + q < f1) xxf[r + q] = omega * bv[q] + cf * xK;
+        for (int j = 0; j < W; j++) {
+            const int cc = c[j] >= 0 ? c[j] : K;
+            bb[j] = b[cc];
+            dd[j] = dinv[cc];
+            ag[j] = agg[cc];
         }
+        const T xcK = xc[aK];
+#pragma unroll
+        for (int j = 0; j < W; j++) xv[j] = xc[ag[j]];
+        const T xx = omega * bK * dK + cf * xcK;
+        T acc = gK * xx;
+#pragma unroll
+        for (int j = 0; j < W; j++)
+            if (c[j] >= 0) acc += v[j] * (omega * bb[j] * dd[j] + cf * xv[j]);
+        xK = xx + omega * dK * (bK - acc);
+        x[K] = xK;
+    }
+    if (PRO) {
+        // the fine level's first step up, xxf = omega bf + cf x[cell], over
+        // the block's cells' fine rows (one contiguous range) with
+        // consecutive threads on consecutive rows
+        __shared__ T s_x[kBlock];
+        s_x[threadIdx.x] = xK;
+        __syncthreads();
+        const int K0 = blockIdx.x * blockDim.x;
+        const int K1 = K0 + (int) blockDim.x < nr ? K0 + (int) blockDim.x : nr;
+        if (K0 >= K1) return;
+        const int f0 = cst1[K0], f1 = cst1[K1];
+        for (int r = f0 + (int) threadIdx.x; r < f1; r += blockDim.x) xxf[r] = omega * bf[r] + cf * s_x[aggf[r] - K0];
+    }
 }
 
 // Device table of one level's arrays, for the tail kernel.
@@ -1438,11 +1441,12 @@ static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval,
         hipLaunchKernelGGL((k_mg_up<T, W, false>), mg_grid(m->cap[l]), dim3(kBlock), 0, st, nd + l, m->col[l],
                            (const T*) m->val[l], (const T*) m->diag[l], (const T*) m->dinv[l], m->cap[l], m->agg[l],
                            (const T*) m->b[l], (const T*) m->x[l + 1], (T*) m->x[l], om, cf, done, nullptr, nullptr,
-                           nullptr);
+                           nullptr, nullptr);
     if (lt >= 2)  // level 1's up pass also takes the fine level's first step up
         hipLaunchKernelGGL((k_mg_up<T, W, true>), mg_grid(m->cap[1]), dim3(kBlock), 0, st, nd + 1, m->col[1],
                            (const T*) m->val[1], (const T*) m->diag[1], (const T*) m->dinv[1], m->cap[1], m->agg[1],
-                           (const T*) m->b[1], (const T*) m->x[2], (T*) m->x[1], om, cf, done, m->cst1, r, xf);
+                           (const T*) m->b[1], (const T*) m->x[2], (T*) m->x[1], om, cf, done, m->cst1, r, xf,
+                           m->agg[0]);
     else
         hipLaunchKernelGGL((k_mg_prolong0<T>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, r, m->agg[0],
                            (const T*) m->x[1], xf, om, cf, done);

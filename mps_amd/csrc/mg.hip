@@ -549,17 +549,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict1(const int* __restrict__
     const int K = blockIdx.x * blockDim.x + threadIdx.x;
     const int nr = *nrow;
     if (mg_done_wave(done) || K >= nr) return;
-    const int r0 = cst1[K], r1 = cst1[K + 1];
-    T s = (T) 0;
-    for (int r = r0; r < r1; r += 4) {  // four loads in flight, summed in row order
-        T v[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) v[j] = res[r + j < r1 ? r + j : r1 - 1];
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            if (r + j < r1) s += v[j];
-    }
-    b1[K] = s;
+    b1[K] = cell_sum(res, cst1[K], cst1[K + 1]);
 }
 
 // Coarse levels are small: their kernels are bound by the round trips of
@@ -1418,7 +1408,7 @@ hipError_t mg_setup(MgWork* m, const Sell& A, const Hot& h, int dim, int fp32, h
 
 template <typename T, typename IDX, int W, int C>
 static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval, const T* r, T* u, MgDone done,
-                       hipStream_t st) {
+                       bool b1_given, hipStream_t st) {
     const T om = (T) m->omega, cf = (T) m->cf;
     const int L = m->L, lt = m->tail;
     int* nd = m->nd;
@@ -1428,8 +1418,9 @@ static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval,
     if (!add)
         hipLaunchKernelGGL((k_mg_down0<T, IDX>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, A.slice_off, fcol,
                            A.anc, fval, r, res0, om, done);
-    hipLaunchKernelGGL((k_mg_restrict1<T>), mg_grid(m->cap[1]), dim3(kBlock), 0, st, nd + 1, m->cst1,
-                       add ? r : (const T*) res0, (T*) m->b[1], done);
+    if (!(add && b1_given))
+        hipLaunchKernelGGL((k_mg_restrict1<T>), mg_grid(m->cap[1]), dim3(kBlock), 0, st, nd + 1, m->cst1,
+                           add ? r : (const T*) res0, (T*) m->b[1], done);
     T* const xf = add ? u : xx0;  // the fine level's first step up
     for (int l = 1; l < lt; l++)
         hipLaunchKernelGGL((k_mg_down<T, W, C>), mg_grid(C * m->cap[l + 1]), dim3(kBlock), 0, st, nd + l + 1,
@@ -1457,24 +1448,32 @@ static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval,
 
 template <typename T, typename IDX>
 void mg_cycle(MgWork* m, const Sell& A, const IDX* fcol, const T* fval, const T* r, T* u, MgDone done, int dim,
-              hipStream_t st) {
+              bool b1_given, hipStream_t st) {
     if (dim == 2)
-        mg_cycle_t<T, IDX, 9, 4>(m, A, fcol, fval, r, u, done, st);
+        mg_cycle_t<T, IDX, 9, 4>(m, A, fcol, fval, r, u, done, b1_given, st);
     else
-        mg_cycle_t<T, IDX, 27, 8>(m, A, fcol, fval, r, u, done, st);
+        mg_cycle_t<T, IDX, 27, 8>(m, A, fcol, fval, r, u, done, b1_given, st);
+}
+
+bool mg_fine_restriction(const MgWork* m, const int** cst1, int* n1, void** b1) {
+    if (!m || !m->fine_additive || m->L < 1) return false;
+    *cst1 = m->cst1;
+    *n1 = m->n[1];
+    *b1 = m->b[1];
+    return true;
 }
 
 template void mg_cycle<float, short>(MgWork*, const Sell&, const short*, const float*, const float*, float*, MgDone,
-                                     int, hipStream_t);
+                                     int, bool, hipStream_t);
 template void mg_cycle<float, int>(MgWork*, const Sell&, const int*, const float*, const float*, float*, MgDone, int,
-                                   hipStream_t);
+                                   bool, hipStream_t);
 template void mg_cycle<double, short>(MgWork*, const Sell&, const short*, const double*, const double*, double*,
-                                      MgDone, int, hipStream_t);
+                                      MgDone, int, bool, hipStream_t);
 template void mg_cycle<double, int>(MgWork*, const Sell&, const int*, const double*, const double*, double*, MgDone,
-                                    int, hipStream_t);
+                                    int, bool, hipStream_t);
 template void mg_cycle<float, unsigned short>(MgWork*, const Sell&, const unsigned short*, const float*, const float*,
-                                              float*, MgDone, int, hipStream_t);
+                                              float*, MgDone, int, bool, hipStream_t);
 template void mg_cycle<double, unsigned short>(MgWork*, const Sell&, const unsigned short*, const double*,
-                                               const double*, double*, MgDone, int, hipStream_t);
+                                               const double*, double*, MgDone, int, bool, hipStream_t);
 
 }  // namespace mps

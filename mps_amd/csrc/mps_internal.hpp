@@ -121,6 +121,23 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
+// Sum of v[r0 .. r1) in row order, four loads in flight (the level-1
+// restriction of the multigrid cycle; the CG step forms the same sums where
+// it writes r, so both give the same bits).
+template <typename T>
+__device__ __forceinline__ T cell_sum(const T* __restrict__ v, int r0, int r1) {
+    T s = (T) 0;
+    for (int r = r0; r < r1; r += 4) {
+        T q[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) q[j] = v[r + j < r1 ? r + j : r1 - 1];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (r + j < r1) s += q[j];
+    }
+    return s;
+}
+
 // Cell coordinate as src/bucket.cpp:40-42 computes it: (int)((x-min)/len)+1.
 // Returns false if the value cannot index the grid.
 __device__ __forceinline__ bool cell_coord(double x, double xmin, double len, int n, int& ic) {
@@ -1082,10 +1099,18 @@ struct MgDone {
     int stride;
     int count;
 };
-// Enqueues u = M r (one V-cycle).
+// Enqueues u = M r (one V-cycle).  b1_given: the level-1 right-hand side
+// (the cells' sums of r, see mg_fine_restriction) is already in place and
+// the cycle does not form it; honoured only by the additive fine level.
 template <typename T, typename IDX>
 void mg_cycle(MgWork* m, const Sell& A, const IDX* fcol, const T* fval, const T* r, T* u, MgDone done, int dim,
-              hipStream_t st);
+              bool b1_given, hipStream_t st);
+// When the current hierarchy's cycle restricts the fine residual r itself
+// (the additive fine level): the first fine row of each level-1 cell
+// (n1 + 1 entries, the last = rows) and the level-1 right-hand side b1[K] =
+// sum of r over cell K's rows in row order, for a caller that forms b1
+// where it writes r.  False otherwise.
+bool mg_fine_restriction(const MgWork* m, const int** cst1, int* n1, void** b1);
 
 struct CgWork;  // defined in ppe.hip
 CgWork* cg_create();

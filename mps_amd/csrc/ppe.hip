@@ -421,6 +421,7 @@ __device__ __forceinline__ int seg_of_block(const Seg& sg, int b) {
 
 struct BlockRows {
     int seg, v, nb, b0;  // segment, block within it, its block count, its first block
+    int lb;              // the block's place among the segment's contiguous row ranges
     int r0, r1;          // rows of this block
 };
 
@@ -436,6 +437,7 @@ __device__ __forceinline__ BlockRows block_rows(const Seg& sg) {
     const int rows = sg.row_off[br.seg + 1] - base;
     const int per = ((rows + br.nb - 1) / br.nb + 63) & ~63;
     const int lb = xcd_block(br.v, br.nb);
+    br.lb = lb;
     br.r0 = base + lb * per;
     const int end = base + rows;
     br.r1 = br.r0 + per < end ? br.r0 + per : end;
@@ -556,15 +558,49 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_apply(Seg sg, const int* __rest
     write_partials<3>(v, part, red);
 }
 
-// step: convergence test on r, then the vector recurrences.
-template <typename T, bool WEIGHTED, bool PRE>
+// Row ranges of the CG step that hold whole level-1 cells (one segment):
+// entry v <= nb is (first row, first cell) of range v, its start moved from
+// v * per (per as in block_rows) to the first cell that starts there or
+// later; cst1 holds the cells' first rows, cst1[n1] = rows.
+__global__ void __launch_bounds__(kBlock) k_step_cells(int nb, int rows, int n1, const int* __restrict__ cst1,
+                                                       int2* __restrict__ out) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v > nb) return;
+    const int per = ((rows + nb - 1) / nb + 63) & ~63;
+    const long long s0 = (long long) v * per;
+    const int start = s0 < rows ? (int) s0 : rows;
+    int lo = 0, hi = n1;  // first cell K <= n1 with cst1[K] >= start
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cst1[mid] < start)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    out[v] = make_int2(cst1[lo], lo);
+}
+
+// step: convergence test on r, then the vector recurrences.  RST (one
+// segment, additive multigrid): the block takes the whole cells of
+// cells[lb] .. cells[lb + 1] and, once its rows of r are written, forms the
+// next cycle's level-1 right-hand side b1[K] = sum of r over cell K.
+template <typename T, bool WEIGHTED, bool PRE, bool RST>
 __global__ void __launch_bounds__(kCgBlock) k_cg_step(Seg sg, T* __restrict__ y, T* __restrict__ r, T* __restrict__ p,
                                                     T* __restrict__ s, const T* __restrict__ w,
                                                     const T* __restrict__ u, const double* __restrict__ part,
-                                                    CgScalars* scs, int par) {
+                                                    CgScalars* scs, int par, const int2* __restrict__ cells,
+                                                    const int* __restrict__ cst1, T* __restrict__ b1) {
 #pragma clang fp contract(fast)
     __shared__ double red[3 * kCgBlock / 64];
-    const BlockRows br = block_rows(sg);
+    BlockRows br = block_rows(sg);
+    int k0 = 0, k1 = 0;
+    if (RST) {
+        const int2 a = cells[br.lb], b = cells[br.lb + 1];
+        br.r0 = a.x;
+        br.r1 = b.x;
+        k0 = a.y;
+        k1 = b.y;
+    }
     CgScalars* sc = scs + br.seg;
     if (block_uniform_done(sc)) return;
     // the first kStepPre rows' vectors are loaded before the partials are
@@ -638,6 +674,10 @@ __global__ void __launch_bounds__(kCgBlock) k_cg_step(Seg sg, T* __restrict__ y,
         s[row] = ss;
         y[row] += alpha * pp;
         r[row] = rr - alpha * ss;
+    }
+    if (RST) {
+        __syncthreads();  // the block's rows of r, written above, are visible to all its threads
+        for (int K = k0 + (int) threadIdx.x; K < k1; K += kCgBlock) b1[K] = cell_sum<T>(r, cst1[K], cst1[K + 1]);
     }
 }
 
@@ -794,6 +834,14 @@ struct CgWork {
     float* u32 = nullptr;
     MgWork* mg = nullptr;     // multigrid preconditioner (single segment)
     int mg_on = 0;            // the current solve uses it
+    // the step forms the cycle's level-1 right-hand side (additive fine
+    // level, one segment): its cell-aligned row ranges and the hierarchy's
+    // cell starts and b1; MPS_AMD_CG_FUSE=0 leaves it to the cycle
+    bool fuse_ok = true;
+    int fused = 0;
+    int2* step_cells = nullptr;  // kMaxBlocks + 1
+    const int* f_cst1 = nullptr;
+    void* f_b1 = nullptr;
     // chunk graphs of the FP64 CG [0] and the FP32 inner CG [1], one per
     // chunk length (the first chunk of a solve is sized by prediction)
     static constexpr int kGraphSlots = 8;
@@ -823,6 +871,7 @@ void cg_prediction_bank(CgWork* cw, int bank) { cw->pred_bank = bank ? 1 : 0; }
 CgWork* cg_create() {
     CgWork* cw = new CgWork();
     if (const char* v = getenv("MPS_AMD_CG_PREDICT")) cw->predict = atoi(v) != 0;
+    if (const char* v = getenv("MPS_AMD_CG_FUSE")) cw->fuse_ok = atoi(v) != 0;
     return cw;
 }
 
@@ -839,6 +888,7 @@ void cg_destroy(CgWork* cw) {
     hipFree(cw->sc);
     hipFree(cw->blk_off);
     hipFree(cw->row_off);
+    hipFree(cw->step_cells);
     hipHostFree(cw->host);
     hipHostFree(cw->h_blk);
     hipHostFree(cw->h_row);
@@ -951,18 +1001,23 @@ static void enqueue_chunk(CgWork* cw, const Sell& A, const IDX* col, const T* va
     for (int k = 0; k < chunk; k++) {
         if (cw->mg_on) {
             const MgDone dn{&cw->sc[0].done, (int) (sizeof(CgScalars) / sizeof(int)), cw->nseg};
-            mg_cycle<T, IDX>(cw->mg, A, col, val, r, u, dn, A.dim, st);
+            // after the chunk's first step the steps leave b1 in place
+            mg_cycle<T, IDX>(cw->mg, A, col, val, r, u, dn, A.dim, cw->fused && k > 0, st);
             hipLaunchKernelGGL((k_cg_apply<T, IDX, WEIGHTED, true>), gr, bl, 0, st, sg, A.slice_off, col, A.anc, val, r,
                                u, w,
                                A.dgl, cw->part, cw->sc);
-            hipLaunchKernelGGL((k_cg_step<T, WEIGHTED, true>), gr, bl, 0, st, sg, y, r, p, s, w, u, cw->part, cw->sc,
-                               k & 1);
+            if (cw->fused)
+                hipLaunchKernelGGL((k_cg_step<T, WEIGHTED, true, true>), gr, bl, 0, st, sg, y, r, p, s, w, u, cw->part,
+                                   cw->sc, k & 1, cw->step_cells, cw->f_cst1, (T*) cw->f_b1);
+            else
+                hipLaunchKernelGGL((k_cg_step<T, WEIGHTED, true, false>), gr, bl, 0, st, sg, y, r, p, s, w, u, cw->part,
+                                   cw->sc, k & 1, nullptr, nullptr, nullptr);
         } else {
             hipLaunchKernelGGL((k_cg_apply<T, IDX, WEIGHTED, false>), gr, bl, 0, st, sg, A.slice_off, col, A.anc, val, r,
                                r,
                                w, A.dgl, cw->part, cw->sc);
-            hipLaunchKernelGGL((k_cg_step<T, WEIGHTED, false>), gr, bl, 0, st, sg, y, r, p, s, w, r, cw->part, cw->sc,
-                               k & 1);
+            hipLaunchKernelGGL((k_cg_step<T, WEIGHTED, false, false>), gr, bl, 0, st, sg, y, r, p, s, w, r, cw->part,
+                               cw->sc, k & 1, nullptr, nullptr, nullptr);
         }
     }
 }
@@ -992,7 +1047,8 @@ static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* c
             (uintptr_t) s,           (uintptr_t) w,            (uintptr_t) u_vec<T>(cw), (uintptr_t) cw->part,
             (uintptr_t) cw->sc,      (uintptr_t) cw->blk_off,  (uintptr_t) cw->row_off, (uintptr_t) A.rows_dev,
             (uintptr_t) cw->nseg,    (uintptr_t) cw->total_blocks, (uintptr_t) idx16,
-            (uintptr_t) (cw->mg_on ? mg_generation(cw->mg) + 1 : 0), (uintptr_t) cw->mg_on, 0, 0, 0};
+            (uintptr_t) (cw->mg_on ? mg_generation(cw->mg) + 1 : 0), (uintptr_t) cw->mg_on,
+            (uintptr_t) (cw->mg_on && cw->fused), (uintptr_t) cw->step_cells, 0};
         if (std::memcmp(sig, cw->graph_sig[which], sizeof(sig)) != 0) {
             for (int j = 0; j < CgWork::kGraphSlots; j++) {
                 if (cw->graph[which][j]) hipGraphExecDestroy(cw->graph[which][j]);
@@ -1127,6 +1183,20 @@ hipError_t cg_solve(CgWork* cw, const Sell& A, const Hot& h, int n, const double
             cw->mg_on = 1;
         else if (me != hipErrorInvalidValue)
             return me;
+    }
+    cw->fused = 0;
+    if (cw->mg_on && cw->fuse_ok && nseg == 1) {
+        const int* cst1 = nullptr;
+        int n1 = 0;
+        void* b1 = nullptr;
+        if (mg_fine_restriction(cw->mg, &cst1, &n1, &b1)) {
+            if (!cw->step_cells && (e = hipMalloc(&cw->step_cells, sizeof(int2) * (kMaxBlocks + 1)))) return e;
+            hipLaunchKernelGGL(k_step_cells, dim3((cw->total_blocks + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                               cw->total_blocks, rows, n1, cst1, cw->step_cells);
+            cw->fused = 1;
+            cw->f_cst1 = cst1;
+            cw->f_b1 = b1;
+        }
     }
 
     hipLaunchKernelGGL(k_cg_init, gr, bl, 0, st, sg, A.row_slot, h.id, b_s, o.warm_start ? pprev_id : nullptr,
@@ -1289,16 +1359,18 @@ static void time_launch(CgWork* cw, const Sell& A, const IDX* col, int kind, hip
                                A.val64, cw->r64, PRE ? cw->u64 : cw->r64, cw->w64, A.dgl, cw->part, cw->sc);
             break;
         case 1:
-            hipLaunchKernelGGL((k_cg_step<double, true, PRE>), gr, bl, 0, st, sg, cw->y, cw->r64, cw->p64, cw->s64,
-                               cw->w64, PRE ? cw->u64 : cw->r64, cw->part, cw->sc, 0);
+            hipLaunchKernelGGL((k_cg_step<double, true, PRE, false>), gr, bl, 0, st, sg, cw->y, cw->r64, cw->p64,
+                               cw->s64, cw->w64, PRE ? cw->u64 : cw->r64, cw->part, cw->sc, 0, nullptr, nullptr,
+                               nullptr);
             break;
         case 2:
             hipLaunchKernelGGL((k_cg_apply<float, IDX, false, PRE>), gr, bl, 0, st, sg, A.slice_off, col, A.anc,
                                A.val32, cw->r32, PRE ? cw->u32 : cw->r32, cw->w32, A.dgl, cw->part, cw->sc);
             break;
         default:
-            hipLaunchKernelGGL((k_cg_step<float, false, PRE>), gr, bl, 0, st, sg, cw->e32, cw->r32, cw->p32, cw->s32,
-                               cw->w32, PRE ? cw->u32 : cw->r32, cw->part, cw->sc, 0);
+            hipLaunchKernelGGL((k_cg_step<float, false, PRE, false>), gr, bl, 0, st, sg, cw->e32, cw->r32, cw->p32,
+                               cw->s32, cw->w32, PRE ? cw->u32 : cw->r32, cw->part, cw->sc, 0, nullptr, nullptr,
+                               nullptr);
             break;
     }
 }

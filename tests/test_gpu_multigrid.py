@@ -61,6 +61,26 @@ def test_both_fine_level_forms_solve_the_same_system(precision, monkeypatch):
     assert its["multiplicative"] < its["additive"] < 3 * its["multiplicative"], its
 
 
+@pytest.mark.parametrize("precision", [capi.MPS_PREC_FP64, capi.MPS_PREC_MIXED])
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_step_formed_restriction_changes_nothing(precision, use_graph, monkeypatch):
+    """With the additive fine level the CG step forms the next cycle's
+    level-1 right-hand side where it writes r, over row ranges that hold
+    whole cells; MPS_AMD_CG_FUSE=0 leaves it to the cycle's restriction
+    kernel.  Both sum each cell's rows in row order: the trajectories are
+    bit-identical, across chunk boundaries (check interval 3) too."""
+    monkeypatch.setenv("MPS_AMD_MG_FINE", "additive")
+    out = {}
+    for fuse in ("1", "0"):
+        monkeypatch.setenv("MPS_AMD_CG_FUSE", fuse)
+        out[fuse], stats = _run("dambreak2d_35k", precision, capi.MPS_PRECOND_MULTIGRID, 4, use_graph=use_graph,
+                                check_interval=3)
+        for st in stats:
+            assert st["converged"] == 1 and st["rel_residual"] <= 1e-9, (fuse, st)
+    for k in ("p", "pos", "vel"):
+        assert (out["1"][k] == out["0"][k]).all(), k
+
+
 def test_multigrid_is_deterministic():
     a, _ = _run("dambreak2d_35k", capi.MPS_PREC_MIXED, capi.MPS_PRECOND_MULTIGRID, 4)
     b, _ = _run("dambreak2d_35k", capi.MPS_PREC_MIXED, capi.MPS_PRECOND_MULTIGRID, 4)

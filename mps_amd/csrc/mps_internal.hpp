@@ -871,6 +871,46 @@ __device__ __forceinline__ void for_listed_g(const NbList& L, const GridView& g,
     }
 }
 
+// Calls f(q, gather(q)) for slot s's listed neighbours, in list order, when
+// re is the list's own radius (every entry passes dist < re, so neither the
+// distances nor the positions are read: 4 bytes an entry instead of 12).
+// Entries go four at a time as in for_listed_g; a slot whose list overflowed
+// is walked with for_neighbors.
+template <int DIM, typename G, typename F>
+__device__ __forceinline__ void for_listed_ids(const NbList& L, const GridView& g, const double* __restrict__ px,
+                                               const double* __restrict__ py, const double* __restrict__ pz,
+                                               int key, int s, double xi, double yi, double zi, double re,
+                                               G&& gather, F&& f) {
+    const int m = L.cnt[s];
+    if (m > L.cap) {
+        for_neighbors<DIM>(g, px, py, pz, key, s, xi, yi, zi, re,
+                           [&](int q, double, double, double, double) { f(q, gather(q)); });
+        return;
+    }
+    if (m <= 0) return;
+    int qn[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) qn[i] = L.q[(size_t) (i < m ? i : m - 1) * L.stride + s];
+    for (int k0 = 0; k0 < m; k0 += 4) {
+        int qq[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) qq[i] = qn[i];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int k = k0 + 4 + i < m ? k0 + 4 + i : m - 1;
+            qn[i] = L.q[(size_t) k * L.stride + s];
+        }
+        decltype(gather(0)) gv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) gv[i] = gather(qq[i]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (k0 + i >= m) break;
+            f(qq[i], gv[i]);
+        }
+    }
+}
+
 // for_listed_g without per-neighbour gathers: f(q, dist, dx, dy, dz).
 template <int DIM, typename F>
 __device__ __forceinline__ void for_listed(const NbList& L, const GridView& g, const double* __restrict__ px,

@@ -766,8 +766,9 @@ hipError_t launch_pressure_out(int dim, const GridView& g, const SceneTab& tab, 
 // One thread per slot walks its re_p neighbours twice: first for the minimum
 // pressure (src/mps.cpp:487-500: p_i itself and every neighbour that is not a
 // dummy wall; Ignored particles keep their old value), which needs only the
-// clamped pressures of the pass before, then for the gradient, which needs
-// only its own minimum.  The second walk finds the neighbours in cache.
+// clamped pressures of the pass before and the list's indices, then for the
+// gradient, which needs only its own minimum.  The second walk finds the
+// neighbours in cache.
 
 template <int DIM, bool MULTI>
 __global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g0, Consts c0, SceneTab tab, int n, Hot cur, Hot out, Cold cold,
@@ -793,11 +794,11 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_grad_move(GridView g0, Consts
         pmi = cold.pmin[id];
     } else {
         pmi = p_s[s];
-        for_listed_g<DIM>(nb, g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, c.re_p, nbr_tp,
-                          [&](int, double, double, double, double, const NbrTP& nq) {
-                               if (nq.type == MPS_DUMMY_WALL) return;
-                               if (pmi > nq.p) pmi = nq.p;
-                           });
+        for_listed_ids<DIM>(nb, g, cur.px, cur.py, cur.pz, cur.key[s], s, x, y, z, c.re_p, nbr_tp,
+                            [&](int, const NbrTP& nq) {
+                                if (nq.type == MPS_DUMMY_WALL) return;
+                                if (pmi > nq.p) pmi = nq.p;
+                            });
         cold.pmin[id] = pmi;
     }
     pmin_s[s] = pmi;

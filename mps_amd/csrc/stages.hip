@@ -16,6 +16,14 @@ namespace mps {
 static inline dim3 grid_for(int n, int bs) { return dim3((unsigned) ((n + bs - 1) / bs)); }
 
 // Launches kern<DIM, MULTI> for the handle's dimension and scene layout.
+// The single-scene 2D walks that stage candidates in LDS run at six waves
+// per SIMD (at most 80 VGPRs) instead of the four or five their free
+// register use gives: measured at 2M, k_visc_move 292 -> 272 us,
+// k_bc_source 190 -> 185 us, k_coll_count 72 -> 67 us.  k_grad_move spills
+// at that bound and runs slower (210 -> 225 us), and so would the ensemble
+// variants (their per-scene constants take registers): they keep their own.
+#define MPS_WALK_WAVES __attribute__((amdgpu_waves_per_eu(DIM == 2 && !MULTI ? 6 : 1)))
+
 #define LAUNCH_DM(kern, grid, block, st, ...)                                                   \
     do {                                                                                        \
         if (dim == 2) {                                                                         \
@@ -125,7 +133,7 @@ __device__ __forceinline__ typename NbStageOf<DIM>::type nb_stage_any(const Grid
 // ---------------- gravity + viscosity + explicit move (src/mps.cpp:165-215) ----------------
 
 template <int DIM, bool MULTI>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_visc_move(GridView g0, Consts c0, SceneTab tab, int n, Hot cur, Hot out, Cold cold,
+__global__ void __launch_bounds__(kMaxTuneBlock) MPS_WALK_WAVES k_visc_move(GridView g0, Consts c0, SceneTab tab, int n, Hot cur, Hot out, Cold cold,
                                                       Events ev) {
     NB_STAGE(DIM, g0, cur, n);
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -198,7 +206,7 @@ hipError_t launch_visc_move(int dim, const GridView& g, const SceneTab& tab, int
 // disjoint fluid particles and are resolved together; levels run in order.
 
 template <int DIM, bool MULTI>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_coll_count(GridView g0, Consts c0, SceneTab tab, int n, Hot h, int* __restrict__ cnt) {
+__global__ void __launch_bounds__(kMaxTuneBlock) MPS_WALK_WAVES k_coll_count(GridView g0, Consts c0, SceneTab tab, int n, Hot h, int* __restrict__ cnt) {
     NB_STAGE(DIM, g0, h, n);
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
@@ -492,7 +500,7 @@ hipError_t launch_collision(int dim, const GridView& g, const SceneTab& tab, int
 // parent = the smallest slot among the particle and its neighbours within
 // re_p (every non-ghost particle, dummy walls included, src/mps.cpp:387-397).
 template <int DIM, bool MULTI>
-__global__ void __launch_bounds__(kMaxTuneBlock) k_bc_source(GridView g0, Consts c0, SceneTab tab, int n, Hot h, Cold cold,
+__global__ void __launch_bounds__(kMaxTuneBlock) MPS_WALK_WAVES k_bc_source(GridView g0, Consts c0, SceneTab tab, int n, Hot h, Cold cold,
                                                       int8_t* __restrict__ bc_s, double* __restrict__ b_s,
                                                       int* __restrict__ parent, NbList nb) {
     NB_STAGE(DIM, g0, h, n);

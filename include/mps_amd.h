@@ -209,8 +209,10 @@ int mps_clear_debug_events(mps_handle* h);
  *            stencil row, those that staged some rows but not all, and of
  *            those the ones that left a row out before a later staged row,
  *   out[7] = CG iterations enqueued by the handle's solves, counting those
- *            of a chunk that return at once after convergence.
- * Writes min(cap, 8) values. */
+ *            of a chunk that return at once after convergence,
+ *   out[8] = CG chunk graphs captured (one per solver and chunk length while
+ *            the pointers they bake in stay; out[0] counts the drops).
+ * Writes min(cap, 9) values. */
 int mps_debug_counters(const mps_handle* h, int64_t* out, int32_t cap);
 
 /* Bucket grid geometry (src/bucket.hpp:11-14). */

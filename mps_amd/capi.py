@@ -250,11 +250,12 @@ class Simulation:
     def debug_counters(self):
         """Internal counters: graph rebuilds, CG and matrix reallocations, the
         LDS staging counts of the walks (MPS_AMD_STAGE_STATS=1 handles) and
-        the CG iterations enqueued (including a chunk's past convergence)."""
-        out = np.zeros(8, np.int64)
-        self._check(self.L.mps_debug_counters(self.h, _p(out), 8))
+        the CG iterations enqueued (including a chunk's past convergence) and
+        the CG chunk graphs captured."""
+        out = np.zeros(9, np.int64)
+        self._check(self.L.mps_debug_counters(self.h, _p(out), 9))
         keys = ("graph_rebuilds", "cg_reallocs", "matrix_reallocs", "staged_blocks", "staged_full",
-                "staged_partial", "staged_partial_gap", "cg_launched")
+                "staged_partial", "staged_partial_gap", "cg_launched", "graph_captures")
         return {k: int(v) for k, v in zip(keys, out)}
 
     def solver_stats(self):

@@ -1323,14 +1323,14 @@ int mps_debug_events(const mps_handle* hc, int64_t counts[3], int32_t* ghost_ids
 
 int mps_debug_counters(const mps_handle* h, int64_t* out, int32_t cap) {
     if (!h || (cap > 0 && !out)) return MPS_E_INVALID;
-    long long cg[3], stg[4] = {0, 0, 0, 0};
+    long long cg[4], stg[4] = {0, 0, 0, 0};
     cg_debug_counts(h->cg, cg);
     if (cap > 3) {
         const hipError_t e = stage_stats_read(stg);
         if (e != hipSuccess) return hip_fail(const_cast<mps_handle*>(h), e, "mps_debug_counters");
     }
-    const int64_t v[8] = {cg[0], cg[1], h->A.grows, stg[0], stg[1], stg[2], stg[3], cg[2]};
-    for (int k = 0; k < cap && k < 8; k++) out[k] = v[k];
+    const int64_t v[9] = {cg[0], cg[1], h->A.grows, stg[0], stg[1], stg[2], stg[3], cg[2], cg[3]};
+    for (int k = 0; k < cap && k < 9; k++) out[k] = v[k];
     return MPS_OK;
 }
 

@@ -1165,7 +1165,7 @@ void cg_prediction_bank(CgWork* cw, int bank);
 hipError_t stage_stats_read(long long out[4]);
 hipError_t cg_reserve(CgWork* cw, int rows);
 // [0] times a solver's cached graphs were dropped, [1] CG vector reallocations
-void cg_debug_counts(const CgWork* cw, long long out[3]);
+void cg_debug_counts(const CgWork* cw, long long out[4]);
 // Solves A x = b for the Inner rows.  b_s / x_s in slot order; x0_id (id order,
 // may be NULL) gives the warm start.  Fills stats.
 // stats: one entry per segment.

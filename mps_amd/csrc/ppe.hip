@@ -844,10 +844,17 @@ struct CgWork {
     void* f_b1 = nullptr;
     // chunk graphs of the FP64 CG [0] and the FP32 inner CG [1], one per
     // chunk length (the first chunk of a solve is sized by prediction)
-    static constexpr int kGraphSlots = 8;
+    // Chunk lengths are even and at most 64, so 32 slots hold every length a
+    // solver can ask for: once captured, a length is never captured again
+    // while the signature below holds (with 8 slots replaced in turn, the
+    // mixed path's inner solves cycled through more lengths than that and
+    // captured a graph again every few steps, ~1.3 ms each at 2M).
+    static constexpr int kGraphSlots = 32;
     hipGraphExec_t graph[2][kGraphSlots] = {};
     int graph_chunk[2][kGraphSlots] = {};
-    int graph_next[2] = {0, 0};  // slot to replace next
+    long long graph_used[2][kGraphSlots] = {};  // last use, for replacing the least recent
+    long long graph_clock = 0;
+    long long graph_captures = 0;  // chunk graphs captured
     // everything a captured chunk bakes in: every device pointer its kernels
     // receive, the launch shape and the multigrid hierarchy's generation.
     // Any difference (a buffer reallocated by a larger system, another
@@ -924,10 +931,11 @@ bool tight_caps() {
     return v && atoi(v) != 0;
 }
 
-void cg_debug_counts(const CgWork* cw, long long out[3]) {
+void cg_debug_counts(const CgWork* cw, long long out[4]) {
     out[0] = cw->graph_rebuilds;
     out[1] = cw->vec_reallocs;
     out[2] = cw->launched;
+    out[3] = cw->graph_captures;
 }
 
 hipError_t cg_reserve(CgWork* cw, int rows) {
@@ -1078,8 +1086,15 @@ static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* c
             for (int j = 0; j < CgWork::kGraphSlots; j++)
                 if (cw->graph[which][j] && cw->graph_chunk[which][j] == chunk) slot = j;
             if (slot < 0) {
-                slot = cw->graph_next[which];
-                cw->graph_next[which] = (slot + 1) % CgWork::kGraphSlots;
+                // a free slot, else the least recently used one
+                slot = 0;
+                for (int j = 0; j < CgWork::kGraphSlots; j++) {
+                    if (!cw->graph[which][j]) {
+                        slot = j;
+                        break;
+                    }
+                    if (cw->graph_used[which][j] < cw->graph_used[which][slot]) slot = j;
+                }
                 if (cw->graph[which][slot]) hipGraphExecDestroy(cw->graph[which][slot]);
                 cw->graph[which][slot] = nullptr;
                 hipGraph_t gph;
@@ -1090,7 +1105,9 @@ static hipError_t run_cg_loop(CgWork* cw, int which, const Sell& A, const IDX* c
                 hipGraphDestroy(gph);
                 if (e) return e;
                 cw->graph_chunk[which][slot] = chunk;
+                cw->graph_captures++;
             }
+            cw->graph_used[which][slot] = ++cw->graph_clock;
             if ((e = hipGraphLaunch(cw->graph[which][slot], st))) return e;
         } else {
             enqueue_chunk<T, IDX, WEIGHTED>(cw, A, col, val, y, r, p, s, w, chunk, st);

@@ -198,3 +198,29 @@ def test_enqueued_iterations_cover_every_solve():
         launched = sim.debug_counters()["cg_launched"] - start
         sim.close()
         assert its <= launched <= 2 * its + 4 * solves, (precision, its, launched, solves)
+
+
+def test_chunk_graphs_are_captured_once_per_length():
+    """Every CG chunk graph is kept for the chunk length it was captured
+    for: a stretch of steps run a second time from the same state (the same
+    chunk lengths, the same predictions at its start) replays every chunk
+    without capturing one, and gives the same bits."""
+    s, sc, _ = configs.workload("dambreak2d_35k")
+    sim = capi.Simulation(s, sc)
+    sim.set_solver(precision=capi.MPS_PREC_MIXED, tolerance=1e-9, warm_start=2,
+                   preconditioner=capi.MPS_PRECOND_MULTIGRID, check_interval=8)
+    sim.load_initial_state()
+    ck = sim.checkpoint()
+    runs = []
+    for _ in range(3):
+        sim.restore(ck)
+        c0 = sim.debug_counters()
+        sim.run_steps(12)
+        c1 = sim.debug_counters()
+        runs.append((c1["graph_captures"] - c0["graph_captures"], c1["graph_rebuilds"] - c0["graph_rebuilds"],
+                     sim.state()))
+    sim.close()
+    (n1, _, _), (n2, _, st2), (n3, r3, st3) = runs
+    assert n1 > 0 and n3 == 0, (n1, n2, n3, r3)
+    for k in ("p", "pos", "vel"):
+        assert (st2[k] == st3[k]).all(), k

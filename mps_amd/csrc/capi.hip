@@ -342,7 +342,10 @@ static int alloc_all(mps_handle* h) {
     HIPCHK(h, dalloc(&w.coll_pairs, 2 * (size_t) w.coll_cap));
     HIPCHK(h, dalloc(&w.coll_dist, (size_t) w.coll_cap));
     HIPCHK(h, dalloc(&w.coll_level, (size_t) w.coll_cap));
-    HIPCHK(h, dalloc(&w.coll_last, n));
+    w.coll_n2 = 1;
+    while (w.coll_n2 < w.coll_cap) w.coll_n2 <<= 1;
+    HIPCHK(h, dalloc(&w.coll_key, (size_t) w.coll_n2));
+    HIPCHK(h, dalloc(&w.coll_prev, (size_t) w.coll_cap));
     w.scan_tmp = h->bb.scan_tmp;
     // pressure-stage neighbour lists: room for 1.5x the neighbours a lattice
     // of spacing l0 has within max(re_s, re_p), plus 8 (slots beyond it are
@@ -390,7 +393,6 @@ static int alloc_all(mps_handle* h) {
     HIPCHK(h, hipHostMalloc((void**) &h->pinned_i, 16 * sizeof(int), hipHostMallocDefault));
     HIPCHK(h, hipHostMalloc((void**) &h->pinned_d, (16 + ns) * sizeof(double), hipHostMallocDefault));
     HIPCHK(h, fill_int(h->bb.cnt, 0, nc + 1, h->st));
-    HIPCHK(h, fill_int(h->w.coll_last, 0, n, h->st));
     HIPCHK(h, fill_int(h->dev_counters, 0, 4, h->st));
     HIPCHK(h, fill_double(h->cold.p, 0.0, n, h->st));
     HIPCHK(h, fill_double(h->cold.pmin, 0.0, n, h->st));
@@ -418,7 +420,7 @@ static void free_all(mps_handle* h) {
                     h->cold.z,      h->cold.vz,       h->w.bc_s,     h->w.b_s,        h->w.diag_s,    h->w.isq_s,    h->w.nnz_s,
                     h->w.parent,    h->w.surf,        h->w.inner_flag, h->w.row_of,   h->w.p_s,       h->w.pmin_s,
                     h->w.vmax,      h->w.coll_cnt,    h->w.coll_off, h->w.coll_pairs, h->w.coll_dist, h->w.coll_level,
-                    h->w.coll_last, h->x_s,           h->dbg_id,     h->tmp3a,        h->tmp3b,       h->tmp8,
+                    h->w.coll_key,  h->w.coll_prev,   h->x_s,           h->dbg_id,     h->tmp3a,        h->tmp3b,       h->tmp8,
                     h->A.slice_off, h->A.slice_w,     h->A.row_slot, h->A.sdiag,      h->A.dgl,       h->A.raw_off,
                     h->A.col,       h->A.dcol,        h->A.anc,         h->A.maxdelta, h->A.val64,       h->A.val32,    h->A.raw_col,    h->A.raw_val,   h->dev_counters,
                     h->ev.ghost_ids, h->ev.unconn_ids, h->snap_pos,  h->snap_vel,     h->d_g,         h->d_c,

@@ -994,8 +994,10 @@ struct StepScratch {
     int* coll_pairs;    // 2 * cap
     double* coll_dist;  // cap
     int* coll_level;    // cap
-    int* coll_last;     // n, zero between steps
+    unsigned long long* coll_key;  // coll_n2: sort keys of pair lists too long for LDS
+    int* coll_prev;     // cap
     int coll_cap;
+    int coll_n2;        // power of two >= coll_cap
     int* scan_tmp;
     NbList nb;          // pressure-stage neighbour lists (written by the BC pass)
 };

@@ -261,11 +261,11 @@ __global__ void __launch_bounds__(kMaxTuneBlock) k_coll_emit(GridView g0, Consts
 }
 
 constexpr int kResolveThreads = 1024;
-// Pair lists up to this long get their levels in parallel out of LDS; longer
-// ones on one thread.
+// Pair lists up to this long get their levels out of LDS; longer ones out of
+// global scratch, by the same steps.
 constexpr int kCollLds = 4096;
 
-// Levels of np <= kCollLds pairs, in parallel.  Pairs come grouped by the
+// Levels of np pairs, in parallel (one workgroup).  Pairs come grouped by the
 // id of their first (fluid) particle i, ascending, and a fluid partner j has
 // a smaller id than i, so it appears only in its own, earlier group and as
 // the partner of later pairs.  The latest earlier pair touching i is
@@ -275,11 +275,11 @@ constexpr int kCollLds = 4096;
 // higher level of those two (0 for none); rounds of that update over every
 // pair reach the same values as the sequential assignment once no level
 // changes (each round settles at least one more link of every chain).
-__device__ void coll_levels_parallel(int np, const Hot& h, const int* __restrict__ off,
-                                     const int* __restrict__ pairs, int* __restrict__ level, int& lmax) {
-    __shared__ unsigned long long s_key[kCollLds];
-    __shared__ int s_prev[kCollLds];
-    __shared__ short s_lev[kCollLds];
+// s_key (a power of two >= np entries), s_prev and s_lev (np) are LDS for
+// short lists, global scratch for long ones.
+template <typename LV>
+__device__ void coll_levels_core(int np, const Hot& h, const int* __restrict__ off, const int* __restrict__ pairs,
+                                 unsigned long long* s_key, int* s_prev, LV* s_lev, int* level, int& lmax) {
     __shared__ int s_changed, s_max;
     int n2 = 1;
     while (n2 < np) n2 <<= 1;
@@ -289,7 +289,7 @@ __device__ void coll_levels_parallel(int np, const Hot& h, const int* __restrict
             const int j = pairs[2 * p + 1];
             if (h.type[j] == MPS_FLUID) k = ((unsigned long long) (unsigned) j << 32) | (unsigned) p;
             s_prev[p] = -1;
-            s_lev[p] = 1;
+            s_lev[p] = (LV) 1;
         }
         s_key[p] = k;
     }
@@ -324,7 +324,7 @@ __device__ void coll_levels_parallel(int np, const Hot& h, const int* __restrict
         s_prev[p] = prev;
     }
     __syncthreads();
-    volatile short* lev = s_lev;
+    volatile LV* lev = s_lev;
     for (;;) {
         if (threadIdx.x == 0) s_changed = 0;
         __syncthreads();
@@ -336,7 +336,7 @@ __device__ void coll_levels_parallel(int np, const Hot& h, const int* __restrict
             if (pi >= 0 && lev[pi] + 1 > L) L = lev[pi] + 1;
             if (pj >= 0 && lev[pj] + 1 > L) L = lev[pj] + 1;
             if (L != lev[p]) {
-                lev[p] = (short) L;
+                lev[p] = (LV) L;
                 ch = 1;
             }
         }
@@ -359,38 +359,31 @@ __device__ void coll_levels_parallel(int np, const Hot& h, const int* __restrict
     __syncthreads();
 }
 
+__device__ void coll_levels_lds(int np, const Hot& h, const int* __restrict__ off, const int* __restrict__ pairs,
+                                int* __restrict__ level, int& lmax) {
+    __shared__ unsigned long long s_key[kCollLds];
+    __shared__ int s_prev[kCollLds];
+    __shared__ short s_lev[kCollLds];
+    coll_levels_core<short>(np, h, off, pairs, s_key, s_prev, s_lev, level, lmax);
+}
+
 template <bool MULTI>
 __global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts c0, SceneTab tab, int dim, Hot h,
                                                                   const int* __restrict__ off,
                                                                   const int* __restrict__ pairs,
                                                                   const double* __restrict__ pdist,
-                                                                  int* __restrict__ level, int* __restrict__ last,
-                                                                  int cap, CollTape tp) {
+                                                                  int* level, unsigned long long* gkey,
+                                                                  int* gprev, int cap, CollTape tp) {
     __shared__ int s_max;
     int np = off[n];
     if (np > cap) np = cap;
     if (np == 0) return;
-    const bool par = np <= kCollLds;
-    if (par) {
-        int mx = 0;
-        coll_levels_parallel(np, h, off, pairs, level, mx);
-        if (threadIdx.x == 0) {
-            s_max = mx;
-            if (tp.nlev) *tp.nlev = mx;
-        }
-    } else if (threadIdx.x == 0) {
-        int mx = 0;
-        for (int p = 0; p < np; p++) {
-            const int i = pairs[2 * p], j = pairs[2 * p + 1];
-            const bool jf = h.type[j] == MPS_FLUID;
-            int L = last[i];
-            if (jf && last[j] > L) L = last[j];
-            L += 1;
-            level[p] = L;
-            last[i] = L;
-            if (jf) last[j] = L;
-            if (L > mx) mx = L;
-        }
+    int mx = 0;
+    if (np <= kCollLds)
+        coll_levels_lds(np, h, off, pairs, level, mx);
+    else  // the levels themselves are the rounds' working copy
+        coll_levels_core<int>(np, h, off, pairs, gkey, gprev, level, level, mx);
+    if (threadIdx.x == 0) {
         s_max = mx;
         if (tp.nlev) *tp.nlev = mx;
     }
@@ -469,11 +462,6 @@ __global__ void __launch_bounds__(kResolveThreads) k_coll_resolve(int n, Consts 
         __threadfence();
         __syncthreads();
     }
-    if (!par)
-        for (int p = threadIdx.x; p < np; p += blockDim.x) {
-            last[pairs[2 * p]] = 0;
-            last[pairs[2 * p + 1]] = 0;
-        }
 }
 
 hipError_t launch_collision(int dim, const GridView& g, const SceneTab& tab, int n, const Consts& c, Hot& h,
@@ -487,10 +475,10 @@ hipError_t launch_collision(int dim, const GridView& g, const SceneTab& tab, int
               w.coll_dist, w.coll_cap, err, tp.dir);
     if (tab.g)
         hipLaunchKernelGGL(k_coll_resolve<true>, dim3(1), dim3(kResolveThreads), 0, st, n, c, tab, dim, h, w.coll_off,
-                           w.coll_pairs, w.coll_dist, w.coll_level, w.coll_last, w.coll_cap, tp);
+                           w.coll_pairs, w.coll_dist, w.coll_level, w.coll_key, w.coll_prev, w.coll_cap, tp);
     else
         hipLaunchKernelGGL(k_coll_resolve<false>, dim3(1), dim3(kResolveThreads), 0, st, n, c, tab, dim, h, w.coll_off,
-                           w.coll_pairs, w.coll_dist, w.coll_level, w.coll_last, w.coll_cap, tp);
+                           w.coll_pairs, w.coll_dist, w.coll_level, w.coll_key, w.coll_prev, w.coll_cap, tp);
     return hipGetLastError();
 }
 

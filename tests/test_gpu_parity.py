@@ -11,6 +11,7 @@ import dataclasses
 
 import numpy as np
 import pytest
+from scipy.spatial import cKDTree
 
 from mps_amd import capi, configs
 from oracle_lib import Oracle
@@ -196,11 +197,16 @@ def test_dense_collisions_match_sequential_order():
     assert (gpos == opos).all() and (gvel == ovel).all()
 
 
-def _packed_block(nx, ny, spacing, seed):
+def _packed_block(nx, ny, spacing, seed, domain=None):
     """A free fluid block at spacing * l0 < the collision distance, jittered,
-    with random velocities: most neighbours collide, in long chains."""
+    with random velocities: most neighbours collide, in long chains.  domain
+    (upper x and y) makes room for blocks larger than the tutorial's domain
+    holds (about 70 x 60)."""
     s = configs.Settings(gravity=(0.0, 0.0, 0.0), kinematic_viscosity=0.0)
     l0 = s.particle_distance
+    if domain is not None:
+        s.domain_x = (s.domain_x[0], domain)
+        s.domain_y = (s.domain_y[0], domain)
     rng = np.random.default_rng(seed)
     gx, gy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
     pos = np.zeros((nx * ny, 3))
@@ -213,19 +219,18 @@ def _packed_block(nx, ny, spacing, seed):
     # candidate pairs as the reference forms them (after the explicit move,
     # which is x += v dt here): fluid pairs once, dist < 0.5 l0
     x = pos + vel * s.dt
-    d = x[None, :, :] - x[:, None, :]
-    r = np.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2])
-    pairs = int(np.count_nonzero(np.triu(r < 0.5 * l0, 1)))
+    pairs = len(cKDTree(x).query_pairs(0.5 * l0))  # the count only sizes the case
     return s, sc, pairs
 
 
-@pytest.mark.parametrize("nx,ny,lo,hi", [(40, 40, 1500, 4096), (64, 64, 4097, 10 ** 6)],
-                         ids=["parallel_levels", "sequential_levels"])
-def test_packed_collisions_match_sequential_order(nx, ny, lo, hi):
+@pytest.mark.parametrize("nx,ny,lo,hi,domain", [(40, 40, 1500, 4096, None), (64, 64, 4097, 10 ** 6, None),
+                                                (128, 128, 16385, 10 ** 6, 0.7)],
+                         ids=["lds_levels", "global_levels", "global_levels_large"])
+def test_packed_collisions_match_sequential_order(nx, ny, lo, hi, domain):
     """Thousands of colliding pairs in long chains: pair levels assigned in
-    parallel (up to 4096 pairs) or on one thread (more) must both reproduce
-    the reference's sequential resolution bit for bit."""
-    s, sc, pairs = _packed_block(nx, ny, 0.45, 9)
+    parallel out of LDS (up to 4096 pairs) or out of global scratch (more)
+    must both reproduce the reference's sequential resolution bit for bit."""
+    s, sc, pairs = _packed_block(nx, ny, 0.45, 9, domain)
     assert lo <= pairs <= hi, pairs
     o, g = _pair(s, sc)
     o.load_initial_state()

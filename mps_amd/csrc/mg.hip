@@ -101,12 +101,14 @@ __global__ void __launch_bounds__(kBlock) k_mg_heads(int R, const int* __restric
 __global__ void __launch_bounds__(kBlock) k_mg_level1(int R, const int* __restrict__ head,
                                                       const int* __restrict__ hscan, const int* __restrict__ rkey,
                                                       const int* __restrict__ key0, int nscene,
-                                                      int* __restrict__ agg0, int* __restrict__ cst1,
-                                                      int* __restrict__ coord1, int* __restrict__ scene1) {
+                                                      int* __restrict__ agg0, unsigned char* __restrict__ agg0b,
+                                                      int* __restrict__ cst1, int* __restrict__ coord1,
+                                                      int* __restrict__ scene1) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     const int K = hscan[r] + head[r] - 1;
     agg0[r] = K;
+    agg0b[r] = (unsigned char) K;  // the cell's place in its up-pass block of 256 cells
     if (head[r]) {
         const int sc = mg_bucket(key0, nscene, rkey[r]);
         cst1[K] = r;
@@ -620,7 +622,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_up(const int* __restrict__ nrow, 
                                                   const T* __restrict__ b, const T* __restrict__ xc,
                                                   T* __restrict__ x, T omega, T cf, MgDone done,
                                                   const int* __restrict__ cst1, const T* __restrict__ bf,
-                                                  T* __restrict__ xxf, const int* __restrict__ aggf) {
+                                                  T* __restrict__ xxf, const unsigned char* __restrict__ aggf) {
 #pragma clang fp contract(fast)
     const int K = blockIdx.x * blockDim.x + threadIdx.x;
     const int Kr = K < cap ? K : 0;
@@ -667,7 +669,9 @@ __global__ void __launch_bounds__(kBlock) k_mg_up(const int* __restrict__ nrow, 
         const int K1 = K0 + (int) blockDim.x < nr ? K0 + (int) blockDim.x : nr;
         if (K0 >= K1) return;
         const int f0 = cst1[K0], f1 = cst1[K1];
-        for (int r = f0 + (int) threadIdx.x; r < f1; r += blockDim.x) xxf[r] = omega * bf[r] + cf * s_x[aggf[r] - K0];
+        // the low byte of a row's cell is its place among the block's cells
+        static_assert(kBlock == 256, "one byte holds a cell's place in the block");
+        for (int r = f0 + (int) threadIdx.x; r < f1; r += blockDim.x) xxf[r] = omega * bf[r] + cf * s_x[aggf[r]];
     }
 }
 
@@ -994,6 +998,7 @@ struct MgWork {
     int* head = nullptr;
     int* hscan = nullptr;
     int* rkey = nullptr;
+    unsigned char* agg0b = nullptr;  // low byte of each fine row's cell
     void* res0 = nullptr;
     void* xx0 = nullptr;
     int gcap = 0;                    // dense grid scratch
@@ -1081,7 +1086,7 @@ void mg_destroy(MgWork* m) {
         hipFree(m->d_sdims[l]);
         hipFree(m->d_goff[l]);
     }
-    void* ps[] = {m->nd,  m->err,   m->cst1,  m->head,     m->hscan, m->rkey,
+    void* ps[] = {m->nd,  m->err,   m->cst1,  m->head,     m->hscan, m->rkey,     m->agg0b,
                   m->res0, m->xx0, m->flag, m->pscan, m->scan_tmp, m->desc, m->lsz};
     for (void* p : ps) hipFree(p);
     mailbox_free(m->mb);
@@ -1197,10 +1202,11 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
         if ((e = mg_alloc(&m->desc, sizeof(MgLevelDesc) * (kMgMaxLevels + 1)))) return e;
     }
     if (R > m->fcap) {
-        void* ps[] = {m->head, m->hscan, m->rkey, m->res0, m->xx0, m->cst1};
+        void* ps[] = {m->head, m->hscan, m->rkey, m->res0, m->xx0, m->cst1, m->agg0b};
         for (void* p : ps) hipFree(p);
         const int cap = R + R / 4 + 1024;
         if ((e = mg_alloc(&m->head, sizeof(int) * (size_t) cap)) || (e = mg_alloc(&m->hscan, sizeof(int) * (cap + 1))) ||
+            (e = mg_alloc(&m->agg0b, (size_t) cap)) ||
             (e = mg_alloc(&m->rkey, sizeof(int) * (size_t) cap)) || (e = mg_alloc(&m->res0, 8 * (size_t) cap)) ||
             (e = mg_alloc(&m->xx0, 8 * (size_t) cap)) || (e = mg_alloc(&m->cst1, sizeof(int) * (cap + 1))))
             return e;
@@ -1236,7 +1242,7 @@ static hipError_t mg_setup_t(MgWork* m, const Sell& A, const Hot& h, int dim, co
     if ((e = mg_read_int(m, m->hscan + R, &n1, st))) return e;
     if ((e = mg_reserve_level(m, 1, n1))) return e;
     hipLaunchKernelGGL(k_mg_level1, mg_grid(R), dim3(kBlock), 0, st, R, m->head, m->hscan, m->rkey, A.scene_key0, ns,
-                       m->agg[0], m->cst1, m->coord[1], m->scene[1]);
+                       m->agg[0], m->agg0b, m->cst1, m->coord[1], m->scene[1]);
     m->n[0] = R;
     m->n[1] = n1;
     m->sdims[1].resize(ns);
@@ -1437,7 +1443,7 @@ static void mg_cycle_t(MgWork* m, const Sell& A, const IDX* fcol, const T* fval,
         hipLaunchKernelGGL((k_mg_up<T, W, true>), mg_grid(m->cap[1]), dim3(kBlock), 0, st, nd + 1, m->col[1],
                            (const T*) m->val[1], (const T*) m->diag[1], (const T*) m->dinv[1], m->cap[1], m->agg[1],
                            (const T*) m->b[1], (const T*) m->x[2], (T*) m->x[1], om, cf, done, m->cst1, r, xf,
-                           m->agg[0]);
+                           m->agg0b);
     else
         hipLaunchKernelGGL((k_mg_prolong0<T>), mg_grid(m->cap[0]), dim3(kBlock), 0, st, nd + 0, r, m->agg[0],
                            (const T*) m->x[1], xf, om, cf, done);

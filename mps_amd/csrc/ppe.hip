@@ -381,8 +381,28 @@ constexpr int kMgMinRows = 8192;
 // recurrence's answer misses the tolerance.
 constexpr int kMaxRestarts = 3;
 
+// Rows per block of a segment below kMaxBlocks blocks: one per thread.
+// Segments large enough to fill the chip (the 2M and 3.6M dam breaks) take
+// kMaxBlocks blocks whatever this is.  It depends on the segment's rows
+// alone, so an ensemble member keeps its solo run's block layout and bits.
+// Measured (ms/step FP64 / mixed at 256, 512 and 1024 rows a block): 64-scene
+// ensemble 16.67 / 15.13, 15.60 / 13.62, 15.71 / 13.25 (at 256 half of each
+// 512-thread block idled, and each of a segment's many blocks reduced all
+// of its partials); 2D 134k 2.76 / 2.87, 2.70 / 2.81, 2.85 / 2.91 (1024
+// leaves CUs idle); 2D 35k 1.82 / 1.98, 1.85 / 2.00, 1.93 / 2.09.
+// MPS_AMD_CG_ROWS overrides it (a tuning aid, read once).
+static int cg_rows_per_block() {
+    static const int v = [] {
+        const char* e = getenv("MPS_AMD_CG_ROWS");
+        const int r = e ? atoi(e) : 0;
+        return r >= 64 ? r : kCgBlock;
+    }();
+    return v;
+}
+
 static int cg_blocks(int rows) {
-    int nb = (rows + kBlock - 1) / kBlock;
+    const int rpb = cg_rows_per_block();
+    int nb = (rows + rpb - 1) / rpb;
     if (nb > kMaxBlocks) nb = kMaxBlocks;
     return nb < 1 ? 1 : nb;
 }
